@@ -1,0 +1,360 @@
+/*
+ * utopia_hip.h - C ABI of the MI355X (gfx950) sparse-solver backend.
+ *
+ * Two layers, both plain C (no C++ or torch types in any signature):
+ *
+ *  (i)  kernel level  - the operations utopia's PETSc backend hands to PETSc
+ *       (MatMult, MatMultTranspose, MatPtAP, VecDot, VecNorm, VecAXPY, ...),
+ *       here over device pointers and a row-block CSR descriptor;
+ *  (ii) solver level  - opaque solver objects mirroring utopia's
+ *       isolver_lsolve_* plug-out (init / update_crs / apply / set_* /
+ *       destroy) for the templated CG, Multigrid, MPRGP and
+ *       ProjectedGaussSeidel solvers.
+ *
+ * Every function returns an int status: 0 (UHIP_OK) or a negative error code;
+ * uhip_last_error() returns the message of the last failure on the calling
+ * thread. Reference citations are to zulianp/utopia (paths relative to its
+ * repository root).
+ */
+#ifndef UTOPIA_HIP_H
+#define UTOPIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UHIP_OK 0
+#define UHIP_ERR_INVALID (-1)
+#define UHIP_ERR_HIP (-2)
+#define UHIP_ERR_ALLOC (-3)
+#define UHIP_ERR_NUMERIC (-4)
+#define UHIP_ERR_UNSUPPORTED (-5)
+
+#define UHIP_F64 0
+#define UHIP_F32 1
+
+#define UHIP_NUM_SCALARS 64
+
+/* ------------------------------------------------------------------------
+ * Runtime: context (device + stream + reduction workspace), memory, errors.
+ * Replaces PETSc's object lifetime + error handling
+ * (backend/petsc/utopia_petsc_Error.hpp:12-17: PETSc errors only printed;
+ * here every call reports a status).
+ * ---------------------------------------------------------------------- */
+typedef struct uhip_context_s *uhip_ctx;
+
+const char *uhip_last_error(void);
+int uhip_device_count(int *count);
+/* stream == NULL: the context creates its own (default-priority) stream. */
+int uhip_ctx_create(int device, void *stream, uhip_ctx *out);
+int uhip_ctx_destroy(uhip_ctx ctx);
+void *uhip_ctx_stream(uhip_ctx ctx);
+int uhip_ctx_device(uhip_ctx ctx);
+/* UHIP_NUM_SCALARS device doubles owned by the context (scratch for results). */
+double *uhip_ctx_scalars(uhip_ctx ctx);
+int uhip_sync(uhip_ctx ctx);
+int uhip_malloc(uhip_ctx ctx, void **ptr, size_t bytes);
+int uhip_free(uhip_ctx ctx, void *ptr);
+int uhip_memcpy_htod(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
+int uhip_memcpy_dtoh(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
+int uhip_memcpy_dtod(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
+int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h);
+int uhip_write_scalars(uhip_ctx ctx, double *dst_d, int n, const double *src_h);
+int uhip_event_timer_start(uhip_ctx ctx, void **ev_pair);
+int uhip_event_timer_stop(uhip_ctx ctx, void *ev_pair, float *ms);
+
+/* ------------------------------------------------------------------------
+ * BLAS1 on device vectors (n entries). Suffix _f64 / _f32.
+ * Replaces PetscVector methods (backend/petsc/utopia_petsc_Vector.hpp:39-930):
+ *   set      VecSet            (.hpp:260)
+ *   copy     VecCopy           (.cpp:636-660)
+ *   axpy     VecAXPY  y+=a*x   (.cpp:671)
+ *   axpby    VecAXPBY y=a*x+b*y(.hpp:675-680)
+ *   waxpby   VecWAXPY-like w=a*x+b*y
+ *   scale    VecScale          (.cpp:624)
+ *   shift    VecShift          (.cpp:18)
+ *   abs      VecAbs (transform(Abs))
+ *   emul     VecPointwiseMult  (.cpp:588)
+ *   ediv     VecPointwiseDivide
+ *   pseudo_inv  ElementWisePseudoInverse (utopia_core/utopia_ElementWisePseudoInverse.hpp:20-56)
+ *   dot      VecDot            (.cpp:59)     result in device memory
+ *   norm2    VecNorm NORM_2    (.hpp:301-305) result (already sqrt) in device memory
+ *   dots     EvalDots split-phase VecDotBegin/End (backend/petsc/eval/utopia_petsc_EvalDotVecVecs.cpp:57-92),
+ *            k <= 4 dots xs[j].ys[j] in one pass
+ *   reduce   op 0 sum, 1 min, 2 max, 3 max|x| (norm_infty), 4 #non-finite (has_nan_or_inf)
+ *            (utopia_petsc_Vector.hpp:381-397)
+ * Reductions accumulate in double and finish in a fixed order (deterministic).
+ * ---------------------------------------------------------------------- */
+#define UHIP_DECLARE_BLAS1(SUF, T)                                                                    \
+    int uhip_vec_set_##SUF(uhip_ctx ctx, int64_t n, T v, T *x);                                       \
+    int uhip_vec_copy_##SUF(uhip_ctx ctx, int64_t n, const T *x, T *y);                               \
+    int uhip_vec_axpy_##SUF(uhip_ctx ctx, int64_t n, T a, const T *x, T *y);                          \
+    int uhip_vec_axpby_##SUF(uhip_ctx ctx, int64_t n, T a, const T *x, T b, T *y);                    \
+    int uhip_vec_waxpby_##SUF(uhip_ctx ctx, int64_t n, T a, const T *x, T b, const T *y, T *w);       \
+    int uhip_vec_scale_##SUF(uhip_ctx ctx, int64_t n, T a, T *x);                                     \
+    int uhip_vec_shift_##SUF(uhip_ctx ctx, int64_t n, T a, T *x);                                     \
+    int uhip_vec_abs_##SUF(uhip_ctx ctx, int64_t n, const T *x, T *y);                                \
+    int uhip_vec_emul_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z);                   \
+    int uhip_vec_ediv_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z);                   \
+    int uhip_vec_pseudo_inv_##SUF(uhip_ctx ctx, int64_t n, const T *x, T *y, T tol);                  \
+    int uhip_vec_dots_##SUF(uhip_ctx ctx, int64_t n, int k, const T *const *xs, const T *const *ys,  \
+                            double *result_d);                                                        \
+    int uhip_vec_dot_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, double *result_d);       \
+    int uhip_vec_norm2_##SUF(uhip_ctx ctx, int64_t n, const T *x, double *result_d);                 \
+    int uhip_vec_reduce_##SUF(uhip_ctx ctx, int64_t n, const T *x, int op, double *result_d);
+
+UHIP_DECLARE_BLAS1(f64, double)
+UHIP_DECLARE_BLAS1(f32, float)
+int uhip_vec_cast_f64_f32(uhip_ctx ctx, int64_t n, const double *x, float *y);
+int uhip_vec_cast_f32_f64(uhip_ctx ctx, int64_t n, const float *x, double *y);
+/* out = xin + a*p with dtype conversion (xin NULL = 0): the first Chebyshev step
+ * x = x + alpha*p without a following SpMV (Chebyshev3level.hpp:154), and the
+ * FP32<->FP64 boundary of the mixed-precision V-cycle. */
+int uhip_vec_axpy_cast(uhip_ctx ctx, int64_t n, double a, const void *p, int p_dtype, const void *xin, int x_dtype,
+                       void *out, int out_dtype);
+
+/* Fused CG steps on device scalars s[] (solvers/smoothers/utopia_ConjugateGradient_impl.hpp:238-276):
+ * cg_update:  alpha = s[i_num]/s[i_den]; x += alpha p; r_new = r - alpha Ap;
+ *             s[i_alpha] = alpha; s[i_rnorm] = |r_new|_2
+ * cg_pupdate: beta = s[i_num]/s[i_den]; p = z + beta p; then s[i_den] = s[i_num] */
+int uhip_cg_update_f64(uhip_ctx ctx, int64_t n, double *x, const double *p, const double *r, const double *Ap,
+                       double *r_new, double *scalars, int i_num, int i_den, int i_alpha, int i_rnorm);
+int uhip_cg_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *z, double *scalars, int i_num, int i_den);
+
+/* ------------------------------------------------------------------------
+ * CSR matrices. Replaces PetscMatrix (AIJ) (backend/petsc/utopia_petsc_Matrix.hpp:103-1028).
+ * int32 rowptr[nrows+1] / col[nnz] (sorted within each row) and values
+ * (double or float) live on the device. The row-block plan (blocks[nblocks+1])
+ * is made by uhip_csr_plan and owned by it.
+ * ---------------------------------------------------------------------- */
+typedef struct uhip_csr {
+    int32_t nrows;
+    int32_t ncols;
+    int64_t nnz;
+    int32_t *rowptr;
+    int32_t *col;
+    void *val;
+    int32_t dtype;      /* UHIP_F64 or UHIP_F32 */
+    int32_t *blocks;    /* row-block plan, device, nblocks+1 entries */
+    int32_t nblocks;
+    int32_t block_capk; /* LDS budget: 256*block_capk products per block */
+} uhip_csr;
+
+/* Build the row-block plan. block_nnz selects the LDS budget (1024, 2048 or 4096
+ * nonzeros per block; 0 = 2048). Rows longer than block_nnz/4 get a block of
+ * their own. Allocates A->blocks. */
+int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
+int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A);
+
+/* y = A x                         MatMult (utopia_petsc_Matrix.cpp:1326-1358) */
+int uhip_spmv(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y);
+/* y = alpha A x + beta y          MatMult + VecAXPBY (Eval_Binary, utopia_core/eval/utopia_Eval_Binary.hpp:88-94) */
+int uhip_spmv_axpby(uhip_ctx ctx, const uhip_csr *A, double alpha, const void *x, double beta, void *y);
+/* y = alpha A x + beta z          g = A x - rhs of MPRGP
+ * (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:114-115,205-206) */
+int uhip_spmv_axpbz(uhip_ctx ctx, const uhip_csr *A, double alpha, const void *x, double beta, const void *z,
+                    void *y);
+/* r = b - A x                     Eval<ResidualExpr> (utopia_core/eval/utopia_Eval_Residual.hpp:19-40)
+ * b_dtype may differ from A->dtype (FP64 right-hand side of an FP32 level);
+ * x and r have A->dtype. */
+int uhip_spmv_residual(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *b, int b_dtype, void *r);
+/* y = A x and dot_d[0] = x.y      A.apply(p,Ap) + dot(p,Ap) of ConjugateGradient
+ * (solvers/smoothers/utopia_ConjugateGradient_impl.hpp:237-238) */
+int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, double *dot_d);
+/* y = mask .* (A x), mask may be NULL   IPRTransfer::restrict + MultiLevelMask::apply
+ * (solvers/multilevel/transfer/utopia_IPRTransfer.hpp:130-134, utopia_MultiLevelMask.hpp:34-37) */
+int uhip_spmv_masked(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *mask, void *y);
+/* y += A x                         IPRTransfer::interpolate + c += c_I (utopia_Multigrid.hpp:317,343) */
+int uhip_spmv_add(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y);
+
+/* One fused step of Chebyshev3level::solve (solvers/smoothers/utopia_Chebyshev3level.hpp:138-175):
+ * with s = A pprev:  r_k = rprev + a_prev*s ; p_k = -r_k + beta*pprev ;
+ * xout = xin + a_prev*pprev (+ a_cur*p_k when last). rprev NULL: rprev = -pprev.
+ * xin NULL: zero initial guess. rout/pout written only when last == 0.
+ * pprev has p_dtype, xout has x_dtype (the FP32 level may hand its result to an
+ * FP64 vector); xin, rprev, rout and pout have A->dtype. */
+typedef struct uhip_cheb_step {
+    const void *pprev;
+    int32_t p_dtype;
+    const void *rprev;
+    const void *xin;
+    void *xout;
+    int32_t x_dtype;
+    void *rout;
+    void *pout;
+    double a_prev, beta, a_cur;
+    int32_t last;
+} uhip_cheb_step;
+int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *step);
+
+/* Diagonal:  d_i = A_ii (0 if absent)   MatGetDiagonal (utopia_petsc_Matrix.cpp:744-756) */
+int uhip_csr_diag(uhip_ctx ctx, const uhip_csr *A, void *d);
+/* A_ii += 1 where |A_ii| < 1e-12  LinearMultiLevel::fix_semidefinite_operator
+ * (solvers/multilevel/utopia_LinearMultiLevel.hpp:89-113) */
+int uhip_csr_fix_semidefinite(uhip_ctx ctx, uhip_csr *A);
+/* Values cast f64 -> f32 of the same pattern (for the FP32 V-cycle). */
+int uhip_csr_cast_values_f64_f32(uhip_ctx ctx, int64_t nnz, const double *src, float *dst);
+/* mask for the finest level: 1 where the row has an off-diagonal |a_ij| > 1e6*eps,
+ * MultiLevelMask::generate_mask_from_matrix with on=0/off=1 followed by shift(-1), abs
+ * (solvers/multilevel/utopia_MultiLevelMask.hpp:39-115): output is the final mask. */
+int uhip_csr_offdiag_indicator(uhip_ctx ctx, const uhip_csr *A, double tol, void *ind);
+/* ind_c = min(1, Rbool * ind_f) where Rbool_ij = |R_ij| > tol
+ * IPRTransfer::boolean_restrict_or (solvers/multilevel/transfer/utopia_IPRTransfer.hpp:144-173) */
+int uhip_csr_boolean_restrict_or(uhip_ctx ctx, const uhip_csr *R, double tol, const void *ind_f, void *ind_c);
+
+/* ------------------------------------------------------------------------
+ * Sparse matrix products (setup). Pattern arithmetic is integer-exact.
+ * ---------------------------------------------------------------------- */
+/* B = A^T (structure and values; cols sorted)   MatTranspose (utopia_petsc_Matrix.cpp:462),
+ * used by IPRTransfer's R = transpose(I) (utopia_IPRTransfer.hpp:28-39).
+ * Allocates B->rowptr/col/val (free with uhip_csr_free). */
+int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B);
+/* C = A * B (Gustavson, structural pattern, sorted cols). Allocates C arrays. */
+int uhip_csr_spgemm(uhip_ctx ctx, const uhip_csr *A, const uhip_csr *B, uhip_csr *C);
+/* Ac = P^T A P  (Galerkin triple product)   MatPtAP
+ * (backend/petsc/eval/utopia_petsc_Eval_Multiply.cpp:12-61; IPRTransfer::restrict(M) :183-187).
+ * Allocates Ac arrays. */
+int uhip_csr_ptap(uhip_ctx ctx, const uhip_csr *A, const uhip_csr *P, uhip_csr *Ac);
+/* Free arrays allocated by transpose/spgemm/ptap/alloc (and the plan). */
+int uhip_csr_free(uhip_ctx ctx, uhip_csr *A);
+/* Dense row-major copy (nrows x ncols doubles) for the coarsest level. */
+int uhip_csr_to_dense(uhip_ctx ctx, const uhip_csr *A, double *dense);
+
+/* ------------------------------------------------------------------------
+ * Dense coarsest-level solve. Replaces Factorization<PetscMatrix,PetscVector>
+ * (backend/petsc/solvers/linear/utopia_petsc_Factorization.hpp:13-84, KSPPREONLY+PCLU):
+ * A = L L^T (FP64, MFMA trailing updates), then A^-1 = L^-T L^-1 kept on the
+ * device so that each apply is one GEMV.
+ * ---------------------------------------------------------------------- */
+int uhip_dense_cholesky(uhip_ctx ctx, int32_t n, double *A, int32_t lda, int *info_h);
+int uhip_dense_cholesky_inverse(uhip_ctx ctx, int32_t n, const double *L, int32_t lda, double *Ainv);
+/* y = M x (row-major n x n, FP64), x/y of dtype x_dtype / y_dtype */
+int uhip_dense_gemv(uhip_ctx ctx, int32_t n, const double *M, const void *x, int x_dtype, void *y, int y_dtype);
+
+/* ------------------------------------------------------------------------
+ * Box-constrained QP kernels (FP64).
+ * ---------------------------------------------------------------------- */
+/* out_i = (lb_i >= x_i) ? lb_i : ((ub_i <= x_i) ? ub_i : x_i)
+ * VariableBoundSolverInterface::project (solvers/nonlinear/constrained/utopia_VariableBoundSolverInterface.hpp:107-155) */
+int uhip_box_project_f64(uhip_ctx ctx, int64_t n, const double *x, const double *lb, const double *ub, double *out);
+/* MPRGP::get_fi / get_beta (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:266-287,324-348),
+ * gp = fi + beta and dots_d[0..2] = beta.beta, fi.fi, gp.gp in one pass (:238-243). fi/beta/gp may be NULL. */
+int uhip_mprgp_state_f64(uhip_ctx ctx, int64_t n, const double *x, const double *g, const double *lb, const double *ub,
+                         double *fi, double *beta, double *gp, double *dots_d);
+int uhip_mprgp_free_grad_f64(uhip_ctx ctx, int64_t n, const double *x, const double *g, const double *lb,
+                             const double *ub, double *fi);
+/* MPRGP::get_alpha_f + multi_min (utopia_MPRGP.hpp:289-322; utopia_core/eval/utopia_MultiReduce.hpp:10-25) */
+int uhip_mprgp_alpha_f_f64(uhip_ctx ctx, int64_t n, const double *x, const double *p, const double *lb,
+                           const double *ub, double *result_d);
+/* CG step (utopia_MPRGP.hpp:167-182): x -= alpha p; g -= alpha Ap; fi = get_fi(x,g); s[i_fiAp] = fi.Ap */
+int uhip_mprgp_cg_step_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *p, const double *Ap,
+                           const double *lb, const double *ub, double *fi, double alpha, double *scalars, int i_fiAp);
+/* p = fi - (s[i_num]/s[i_den]) p */
+int uhip_mprgp_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *fi, const double *scalars, int i_num,
+                           int i_den);
+/* expansion step (utopia_MPRGP.hpp:184-198): x = P(x - af p - abar get_fi(x - af p, g - af Ap)) */
+int uhip_mprgp_expand_f64(uhip_ctx ctx, int64_t n, double *x, const double *g, const double *p, const double *Ap,
+                          const double *lb, const double *ub, double alpha_f, double alpha_bar);
+/* proportioning step (utopia_MPRGP.hpp:223-235): x -= a beta; g -= a Abeta; p = get_fi(x, g) */
+int uhip_mprgp_proportion_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *beta, const double *Abeta,
+                              const double *lb, const double *ub, double *p, double alpha);
+/* Jones-Plassmann colouring of a structurally symmetric pattern: color[i] in [0, ncolors),
+ * no two coupled rows share a colour. */
+int uhip_graph_color(uhip_ctx ctx, const uhip_csr *A, uint64_t seed, int32_t *color, int32_t *ncolors);
+/* rows grouped by colour; offsets_h (host, ncolors+1) */
+int uhip_color_groups(uhip_ctx ctx, int32_t n, const int32_t *color, int32_t ncolors, int32_t *rows,
+                      int32_t *offsets_h);
+/* d_inv_i = 1/a_ii where |a_ii| > 0 else 0 (ProjectedScalarGaussSeidelSweep::init_from_local_matrix,
+ * solvers/nonlinear/constrained/quadratic_programming/utopia_ProjectedGaussSeidelSweep.hpp:95-104) */
+int uhip_pgs_dinv(uhip_ctx ctx, const uhip_csr *A, double *dinv);
+/* One colour of the projected (constrained != 0) or plain Gauss-Seidel sweep
+ * (utopia_ProjectedGaussSeidelSweep.hpp:161-235), bounds lb - x, ub - x. */
+int uhip_pgs_color_sweep(uhip_ctx ctx, const uhip_csr *A, int32_t cnt, const int32_t *rows, const double *dinv,
+                         const double *r, const double *x, const double *lb, const double *ub, double *c,
+                         int constrained);
+
+/* ------------------------------------------------------------------------
+ * Solver level: isolver_lsolve_* shaped objects
+ * (plug_in_and_out/isolver/utopia_isolver_lsolve.cpp:42-220).
+ * ---------------------------------------------------------------------- */
+typedef struct uhip_solver_s *uhip_solver;
+#define UHIP_SOLVER_CG 0          /* ConjugateGradient<..., HOMEMADE>, no preconditioner */
+#define UHIP_SOLVER_CG_JACOBI 1   /* + InvDiagPreconditioner */
+#define UHIP_SOLVER_CG_MG 2       /* + Multigrid (FP64 V-cycle), max_it 1 */
+#define UHIP_SOLVER_CG_MG_MIXED 3 /* + Multigrid on FP32 levels (FP64 CG, residual, coarse solve) */
+#define UHIP_SOLVER_MG 4          /* Multigrid as a solver */
+#define UHIP_SOLVER_MPRGP 5       /* MPRGP box-constrained QP */
+#define UHIP_SOLVER_PGS 6         /* ProjectedGaussSeidel (multicolour) */
+
+/* isolver_lsolve_init */
+int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out);
+/* isolver_lsolve_destroy */
+int uhip_solver_destroy(uhip_solver s);
+/* read(Input&)-style parameters: atol rtol stol max_it verbose reset_initial_guess refine
+ * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
+ * smoother(0 Chebyshev3level, 1 Jacobi) scale_max_eig scale_min_eig power_method_tol power_method_max_it
+ * block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each. Read at the next update. */
+int uhip_solver_set_param(uhip_solver s, const char *key, double value);
+/* Multigrid::set_transfer_operators (utopia_LinearMultiLevel.hpp:65-79), host CSR, coarse to fine */
+int uhip_solver_set_transfer_operators(uhip_solver s, int n, const int32_t *nrows, const int32_t *ncols,
+                                       const int32_t *const *rowptr, const int32_t *const *col,
+                                       const double *const *val);
+/* isolver_lsolve_update_crs: host CSR arrays (copied to the device) */
+int uhip_solver_update_crs(uhip_solver s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val);
+/* same on caller-owned device arrays (PetscMatrix::wrap semantics: not freed) */
+int uhip_solver_update_crs_device(uhip_solver s, int32_t n, int64_t nnz, int32_t *rowptr, int32_t *col, double *val);
+/* QPSolver::set_box_constraints (solvers/nonlinear/constrained/quadratic_programming/utopia_QPSolver.hpp:34-44);
+ * NULL bound = none (filled with -/+DBL_MAX, BoxConstraints::fill_empty_bounds). Call after update. */
+int uhip_solver_set_box_constraints(uhip_solver s, const double *lb, const double *ub);
+/* isolver_lsolve_apply: host arrays, x is the initial guess and the result */
+int uhip_solver_apply(uhip_solver s, const double *rhs, double *x);
+int uhip_solver_apply_device(uhip_solver s, const double *rhs_d, double *x_d);
+/* SolutionStatus (iterates, gradient_norm, reason) and solve wall time */
+int uhip_solver_get_status(uhip_solver s, int64_t *iterates, double *gradient_norm, int *reason, double *seconds);
+/* setup_seconds solve_seconds total_iterations refinements n_levels coarse_n eig_max_<l> eig_min_<l>
+ * level_nnz_<l> level_rows_<l> ncolors */
+int uhip_solver_get_info(uhip_solver s, const char *key, double *value);
+int uhip_solver_get_level(uhip_solver s, int l, int32_t *rowptr, int32_t *col, double *val);
+int uhip_solver_get_mask(uhip_solver s, int l, double *mask);
+int uhip_solver_set_level_eigs(uhip_solver s, int l, double eig_max, double eig_min);
+int uhip_lsolve_set_max_iterations(uhip_solver s, int max_it);
+int uhip_lsolve_set_atol(uhip_solver s, double v);
+int uhip_lsolve_set_rtol(uhip_solver s, double v);
+int uhip_lsolve_set_stol(uhip_solver s, double v);
+int uhip_lsolve_set_verbosity(uhip_solver s, int level);
+
+/* ------------------------------------------------------------------------
+ * Problem generators (host, seeded): the inputs every path consumes.
+ * splitmix64 stream; uniform in [0,1) = (u >> 11) * 2^-53; normals by
+ * Box-Muller. Arrays are caller-allocated host memory.
+ * ---------------------------------------------------------------------- */
+/* nnz of the stencil operators below (for allocation). */
+int64_t uhip_gen_nnz(int kind, int32_t nx, int32_t ny, int32_t nz);
+/* kind 0: 2D 5-point Laplacian [-1,-1,4,-1,-1] on nx*ny interior (Dirichlet eliminated)
+ * kind 1: 3D 7-point Laplacian on nx*ny*nz (diag 6)
+ * kind 2: 3D 7-point variable coefficient: cell kappa = exp(N(0,1)) from seed,
+ *         face coefficient 2 ka kb/(ka+kb), boundary faces use the cell value
+ * kind 3: 2D 9-point Laplacian (diag 8, all eight neighbours -1) + shift*I
+ * kind 4: 2D 5-point with per-edge coefficients exp(N(0, 0.5)) from seed */
+int uhip_gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift,
+                     int32_t *rowptr, int32_t *col, double *val);
+/* Trilinear (bilinear / linear for nz==1 / ny==1) prolongation from the coarse
+ * grid (nx/2, ny/2, nz/2) to the fine grid (nx, ny, nz), nx, ny, nz even (or 1).
+ * Fine point f (0-based) takes coarse point (f-1)/2 with weight 1 when f is odd,
+ * coarse points f/2-1 and f/2 with weight 1/2 each (those inside the grid) when
+ * f is even. */
+int64_t uhip_gen_prolongation_nnz(int32_t nx, int32_t ny, int32_t nz);
+int uhip_gen_prolongation(int32_t nx, int32_t ny, int32_t nz, int32_t *rowptr, int32_t *col, double *val);
+/* x_i = lo + (hi-lo)*u_i */
+int uhip_gen_uniform(uint64_t seed, int64_t n, double lo, double hi, double *x);
+/* x_i ~ N(mu, sigma) */
+int uhip_gen_normal(uint64_t seed, int64_t n, double mu, double sigma, double *x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UTOPIA_HIP_H */

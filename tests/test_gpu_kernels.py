@@ -1,0 +1,301 @@
+"""Kernel-level parity on the GPU: the C ABI against the CPU oracle.
+
+SpMV rows are summed sequentially from LDS, so every y_i must equal the
+oracle's sequential CSR loop bit for bit (FP64 and FP32). Pattern work
+(transpose, SpGEMM, RAP) must match exactly, and since both sides accumulate
+C_ij in the same visiting order, so must the values.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as orc
+import utopia_amd as ua
+from utopia_amd import petsc_io
+
+pytestmark = pytest.mark.gpu
+
+L = ua.lib
+
+
+def P(x):
+    return ctypes.c_void_p(x.ptr)
+
+
+def csr_random(n, m, density, seed, empty_rows=(), long_row=None, long_len=0):
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, m, density=density, random_state=seed, format="lil")
+    for r in empty_rows:
+        M[r, :] = 0
+    if long_row is not None:
+        cols = rng.choice(m, size=long_len, replace=False)
+        M[long_row, cols] = rng.standard_normal(long_len)
+    C = ua.Csr.from_scipy(sp.csr_matrix(M))
+    return C
+
+
+MATRICES = {
+    "poisson2d": lambda: ua.gen_stencil(ua.STENCIL_2D5, 37, 29),
+    "varcoef3d": lambda: ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 13, 11, 9, seed=3),
+    "nine_point": lambda: ua.gen_stencil(ua.STENCIL_2D9, 40, 33, shift=1e-3),
+    "prolongation": lambda: ua.gen_prolongation(16, 12, 10),
+    "ragged_empty_rows": lambda: csr_random(3000, 2500, 0.003, 1, empty_rows=range(100, 900)),
+    "one_row": lambda: csr_random(1, 50, 0.5, 2),
+    "rows_up_to_900": lambda: csr_random(400, 5000, 0.1, 3, long_row=17, long_len=900),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATRICES))
+@pytest.mark.parametrize("block_nnz", [1024, 2048, 4096])
+def test_spmv_fp64_bit_exact(ctx, name, block_nnz):
+    A = MATRICES[name]()
+    x = ua.gen_uniform(11, A.shape[1])
+    D = ua.DeviceCsr(ctx, A, np.float64, block_nnz)
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    yd = ua.DeviceArray(ctx, A.shape[0])
+    D.spmv(xd, yd)
+    y = yd.numpy()
+    ref = orc.spmv(A, x)
+    assert np.array_equal(y, ref), np.abs(y - ref).max()
+    blocks = D.blocks()
+    assert blocks[0] == 0 and blocks[-1] == A.shape[0] and np.all(np.diff(blocks) > 0)
+
+
+@pytest.mark.parametrize("name", sorted(MATRICES))
+def test_spmv_fp32_bit_exact(ctx, name):
+    A = MATRICES[name]()
+    x = ua.gen_uniform(12, A.shape[1]).astype(np.float32)
+    D = ua.DeviceCsr(ctx, A, np.float32)
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    yd = ua.DeviceArray(ctx, A.shape[0], np.float32)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv_f32(A, x))
+
+
+def test_spmv_long_rows_take_block_reduction(ctx):
+    # rows longer than the LDS budget are reduced by a whole workgroup (tree order)
+    A = csr_random(64, 20000, 0.001, 5, long_row=5, long_len=15000)
+    x = ua.gen_uniform(13, A.shape[1])
+    D = ua.DeviceCsr(ctx, A, np.float64, 1024)
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    yd = ua.DeviceArray(ctx, A.shape[0])
+    D.spmv(xd, yd)
+    y, ref = yd.numpy(), orc.spmv(A, x)
+    bound = 1e-14 * (np.abs(A.to_scipy()) @ np.abs(x)) + 1e-300
+    assert np.all(np.abs(y - ref) <= 20 * bound)
+    mask = np.ones(A.shape[0], bool)
+    mask[5] = False
+    assert np.array_equal(y[mask], ref[mask])
+
+
+def test_spmv_empty_matrix_rows(ctx):
+    A = ua.Csr(np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0), (5, 4))
+    D = ua.DeviceCsr(ctx, A)
+    xd = ua.DeviceArray.from_numpy(ctx, np.ones(4))
+    yd = ua.DeviceArray.from_numpy(ctx, np.full(5, 7.0))
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), np.zeros(5))
+
+
+def test_fused_spmv_epilogues(ctx):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 12, 10, 8, seed=9)
+    n = A.shape[0]
+    x, b, z, m = (ua.gen_uniform(s, n) for s in (1, 2, 3, 4))
+    m = (m > 0).astype(np.float64)
+    Ax = orc.spmv(A, x)
+    D = ua.DeviceCsr(ctx, A)
+    xd, bd, zd, md = (ua.DeviceArray.from_numpy(ctx, v) for v in (x, b, z, m))
+    out = ua.DeviceArray(ctx, n)
+    h = ctx.h
+    ua.check(L().uhip_spmv_residual(h, D.ref, P(xd), P(bd), ua.F64, P(out)))
+    assert np.array_equal(out.numpy(), b - Ax)
+    ua.check(L().uhip_spmv_masked(h, D.ref, P(xd), P(md), P(out)))
+    assert np.array_equal(out.numpy(), m * Ax)
+    out.upload(z)
+    ua.check(L().uhip_spmv_add(h, D.ref, P(xd), P(out)))
+    assert np.array_equal(out.numpy(), z + Ax)
+    out.upload(z)
+    ua.check(L().uhip_spmv_axpby(h, D.ref, 2.5, P(xd), -0.5, P(out)))
+    assert np.array_equal(out.numpy(), 2.5 * Ax + (-0.5) * z)
+    ua.check(L().uhip_spmv_axpbz(h, D.ref, 1.0, P(xd), -1.0, P(bd), P(out)))
+    assert np.array_equal(out.numpy(), Ax - b)
+    s = ctx.scalars
+    ua.check(L().uhip_spmv_dot(h, D.ref, P(xd), P(out), ctypes.c_void_p(s)))
+    assert np.array_equal(out.numpy(), Ax)
+    got = ctx.read_scalars(s, 1)[0]
+    assert abs(got - np.dot(x, Ax)) <= 1e-13 * np.dot(np.abs(x), np.abs(Ax))
+
+
+def test_mixed_residual_fp64_rhs_on_fp32_matrix(ctx):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 10, 9, 8, seed=4)
+    n = A.shape[0]
+    x = ua.gen_uniform(5, n).astype(np.float32)
+    b = ua.gen_uniform(6, n)
+    D = ua.DeviceCsr(ctx, A, np.float32)
+    xd, bd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b)
+    r = ua.DeviceArray(ctx, n, np.float32)
+    D.residual(xd, bd, r)
+    assert np.array_equal(r.numpy(), b.astype(np.float32) - orc.spmv_f32(A, x))
+
+
+def test_blas1_and_reductions(ctx):
+    n = 100_003
+    x, y = ua.gen_uniform(21, n), ua.gen_uniform(22, n)
+    h, l = ctx.h, L()
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, y)
+    w = ua.DeviceArray(ctx, n)
+    ua.check(l.uhip_vec_waxpby_f64(h, n, 2.0, P(xd), -3.0, P(yd), P(w)))
+    assert np.array_equal(w.numpy(), 2.0 * x + (-3.0) * y)
+    w.upload(y)
+    ua.check(l.uhip_vec_axpy_f64(h, n, 0.25, P(xd), P(w)))
+    assert np.array_equal(w.numpy(), y + 0.25 * x)
+    w.upload(y)
+    ua.check(l.uhip_vec_axpby_f64(h, n, 0.25, P(xd), 3.0, P(w)))
+    assert np.array_equal(w.numpy(), 0.25 * x + 3.0 * y)
+    ua.check(l.uhip_vec_emul_f64(h, n, P(xd), P(yd), P(w)))
+    assert np.array_equal(w.numpy(), x * y)
+    ua.check(l.uhip_vec_ediv_f64(h, n, P(xd), P(yd), P(w)))
+    assert np.array_equal(w.numpy(), x / y)
+    xs = x.copy()
+    xs[::7] = 1e-17
+    xsd = ua.DeviceArray.from_numpy(ctx, xs)
+    ua.check(l.uhip_vec_pseudo_inv_f64(h, n, P(xsd), P(w), np.finfo(float).eps))
+    ref = np.where(np.abs(xs) > np.finfo(float).eps, 1.0 / np.where(xs == 0, 1, xs), 0.0)
+    assert np.array_equal(w.numpy(), ref)
+    w.upload(x)
+    ua.check(l.uhip_vec_shift_f64(h, n, -1.0, P(w)))
+    ua.check(l.uhip_vec_abs_f64(h, n, P(w), P(w)))
+    assert np.array_equal(w.numpy(), np.abs(x + (-1.0)))
+    ua.check(l.uhip_vec_scale_f64(h, n, -2.0, P(w)))
+    ua.check(l.uhip_vec_set_f64(h, 17, 3.5, P(w)))
+    assert np.array_equal(w.numpy()[:17], np.full(17, 3.5))
+    s = ctx.scalars
+    arr = (ctypes.c_void_p * 4)
+    xs4 = arr(xd.ptr, xd.ptr, yd.ptr, xd.ptr)
+    ys4 = arr(yd.ptr, xd.ptr, yd.ptr, xd.ptr)
+    ua.check(l.uhip_vec_dots_f64(h, n, 4, xs4, ys4, ctypes.c_void_p(s)))
+    got = ctx.read_scalars(s, 4)
+    exact = [np.dot(x, y), np.dot(x, x), np.dot(y, y), np.dot(x, x)]
+    for g, e, a in zip(got, exact, [np.abs(x) @ np.abs(y), x @ x, y @ y, x @ x]):
+        assert abs(g - e) <= 1e-13 * a
+    ua.check(l.uhip_vec_norm2_f64(h, n, P(xd), ctypes.c_void_p(s)))
+    assert abs(ctx.read_scalars(s, 1)[0] - np.linalg.norm(x)) <= 1e-14 * np.linalg.norm(x)
+    bad = x.copy()
+    bad[[3, 99]] = [np.nan, np.inf]
+    bd = ua.DeviceArray.from_numpy(ctx, bad)
+    for op, ref in [(0, x.sum()), (1, x.min()), (2, x.max()), (3, np.abs(x).max())]:
+        ua.check(l.uhip_vec_reduce_f64(h, n, P(xd), op, ctypes.c_void_p(s)))
+        v = ctx.read_scalars(s, 1)[0]
+        assert (v == ref) if op else abs(v - ref) <= 1e-12 * np.abs(x).sum()
+    ua.check(l.uhip_vec_reduce_f64(h, n, P(bd), 4, ctypes.c_void_p(s)))
+    assert ctx.read_scalars(s, 1)[0] == 2
+    # deterministic: the same reduction twice gives the same bits
+    ua.check(l.uhip_vec_dot_f64(h, n, P(xd), P(yd), ctypes.c_void_p(s)))
+    a1 = ctx.read_scalars(s, 1)[0]
+    ua.check(l.uhip_vec_dot_f64(h, n, P(xd), P(yd), ctypes.c_void_p(s)))
+    assert ctx.read_scalars(s, 1)[0] == a1
+
+
+def test_axpy_cast_boundary(ctx):
+    n = 5000
+    p = ua.gen_uniform(31, n)
+    x = ua.gen_uniform(32, n).astype(np.float32)
+    pd, xd = ua.DeviceArray.from_numpy(ctx, p), ua.DeviceArray.from_numpy(ctx, x)
+    out = ua.DeviceArray(ctx, n)
+    ua.check(L().uhip_vec_axpy_cast(ctx.h, n, 0.5, P(pd), ua.F64, P(xd), ua.F32, P(out), ua.F64))
+    assert np.array_equal(out.numpy(), x.astype(np.float64) + 0.5 * p)
+    out32 = ua.DeviceArray(ctx, n, np.float32)
+    ua.check(L().uhip_vec_axpy_cast(ctx.h, n, 0.5, P(pd), ua.F64, None, ua.F32, P(out32), ua.F32))
+    assert np.array_equal(out32.numpy(), (np.float32(0.5) * p.astype(np.float32)).astype(np.float32))
+
+
+def _owned(ctx, m):
+    C = ua.csr_from_raw(ctx, m)
+    ua.check(L().uhip_csr_free(ctx.h, ctypes.byref(m)))
+    return C
+
+
+@pytest.mark.parametrize("case", ["varcoef3d", "fixture_fA"])
+def test_transpose_spgemm_ptap_match_oracle_exactly(ctx, golden, case):
+    if case == "varcoef3d":
+        A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 8, 8, 8, seed=2)
+        Pm = ua.gen_prolongation(8, 8, 8)
+    else:
+        A = petsc_io.read_mat(golden / "utopia_data" / "f_A")
+        Pm = petsc_io.read_mat(golden / "utopia_data" / "I_3")
+    DA, DP = ua.DeviceCsr(ctx, A), ua.DeviceCsr(ctx, Pm)
+    T = ua.uhip_csr()
+    ua.check(L().uhip_csr_transpose(ctx.h, DP.ref, ctypes.byref(T)))
+    Th = _owned(ctx, T)
+    rp, col, val, shape = orc.transpose(Pm)
+    assert Th.shape == shape and np.array_equal(Th.rowptr, rp) and np.array_equal(Th.col, col)
+    assert np.array_equal(Th.val, val)
+    C = ua.uhip_csr()
+    ua.check(L().uhip_csr_spgemm(ctx.h, DA.ref, DP.ref, ctypes.byref(C)))
+    Ch = _owned(ctx, C)
+    rp, col, val, shape = orc.spgemm(A, Pm)
+    assert np.array_equal(Ch.rowptr, rp) and np.array_equal(Ch.col, col) and np.array_equal(Ch.val, val)
+    Ac = ua.uhip_csr()
+    ua.check(L().uhip_csr_ptap(ctx.h, DA.ref, DP.ref, ctypes.byref(Ac)))
+    Ah = _owned(ctx, Ac)
+    rp, col, val, shape = orc.ptap(A, Pm)
+    assert np.array_equal(Ah.rowptr, rp) and np.array_equal(Ah.col, col) and np.array_equal(Ah.val, val)
+
+
+def test_spgemm_wide_rows_use_large_workgroups(ctx):
+    # candidate products per row > 1024 select the 256-thread / 4096-8192 entry path
+    A = csr_random(300, 300, 0.2, 8)
+    B = csr_random(300, 400, 0.1, 9)
+    DA, DB = ua.DeviceCsr(ctx, A), ua.DeviceCsr(ctx, B)
+    C = ua.uhip_csr()
+    ua.check(L().uhip_csr_spgemm(ctx.h, DA.ref, DB.ref, ctypes.byref(C)))
+    Ch = _owned(ctx, C)
+    rp, col, val, _ = orc.spgemm(A, B)
+    assert np.array_equal(Ch.rowptr, rp) and np.array_equal(Ch.col, col) and np.array_equal(Ch.val, val)
+
+
+@pytest.mark.parametrize("n", [1, 63, 200, 333])
+def test_dense_cholesky_and_inverse(ctx, n):
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, n))
+    A = M @ M.T + n * np.eye(n)
+    Ad = ua.DeviceArray.from_numpy(ctx, A.ravel())
+    info = ctypes.c_int()
+    ua.check(L().uhip_dense_cholesky(ctx.h, n, P(Ad), n, ctypes.byref(info)))
+    assert info.value == 0
+    Lg = np.tril(Ad.numpy().reshape(n, n))
+    Lref = np.linalg.cholesky(A)
+    assert np.abs(Lg - Lref).max() <= 1e-12 * np.abs(Lref).max()
+    Inv = ua.DeviceArray(ctx, n * n)
+    ua.check(L().uhip_dense_cholesky_inverse(ctx.h, n, P(Ad), n, P(Inv)))
+    Ainv = Inv.numpy().reshape(n, n)
+    assert np.abs(Ainv @ A - np.eye(n)).max() <= 1e-12
+    x = rng.standard_normal(n)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, n)
+    ua.check(L().uhip_dense_gemv(ctx.h, n, P(Inv), P(xd), ua.F64, P(yd), ua.F64))
+    assert np.abs(yd.numpy() - Ainv @ x).max() <= 1e-13 * np.abs(Ainv).sum(1).max() * np.abs(x).max()
+
+
+def test_dense_cholesky_reports_indefinite(ctx):
+    A = np.diag([1.0, 2.0, -1.0, 4.0])
+    Ad = ua.DeviceArray.from_numpy(ctx, A.ravel())
+    info = ctypes.c_int()
+    ua.check(L().uhip_dense_cholesky(ctx.h, 4, P(Ad), 4, ctypes.byref(info)))
+    assert info.value == 3
+
+
+def test_graph_coloring_is_proper(ctx):
+    A = ua.gen_stencil(ua.STENCIL_2D9, 60, 50, shift=1e-3)
+    D = ua.DeviceCsr(ctx, A)
+    n = A.shape[0]
+    color = ua.DeviceArray(ctx, n, np.int32)
+    k = ctypes.c_int32()
+    ua.check(L().uhip_graph_color(ctx.h, D.ref, ctypes.c_uint64(7), P(color), ctypes.byref(k)))
+    c = color.numpy()
+    assert c.min() == 0 and c.max() == k.value - 1
+    rows = np.repeat(np.arange(n), np.diff(A.rowptr))
+    off = rows != A.col
+    assert not np.any(c[rows[off]] == c[A.col[off]])
+    assert k.value <= 12

@@ -1,0 +1,475 @@
+// utopia HIP backend: HipVector / HipCrsMatrix.
+//
+// The tensor types the solver templates run on, in place of PetscVector
+// (backend/petsc/utopia_petsc_Vector.hpp:39-930) and PetscMatrix
+// (backend/petsc/utopia_petsc_Matrix.hpp:103-1028). Every numeric method is one
+// call into the C ABI of include/utopia_hip.h; no HIP type appears here.
+// Errors from the C ABI are raised as utopia::HipError.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "utopia_hip.h"
+
+namespace utopia {
+
+// Backend tag next to PETSC=10, TRILINOS=10000, BLAS=1 (utopia_core/utopia_Traits.hpp:12-21).
+enum { HIP = 1000000 };
+
+class HipError : public std::runtime_error {
+public:
+    HipError(int code, const std::string &what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline void hip_check(int status, const char *what) {
+    if (status != UHIP_OK) throw HipError(status, std::string(what) + ": " + uhip_last_error());
+}
+
+// One device + one stream + reduction workspace; shared by everything made from it.
+class HipContext {
+public:
+    explicit HipContext(int device = 0, void *stream = nullptr) {
+        hip_check(uhip_ctx_create(device, stream, &ctx_), "uhip_ctx_create");
+    }
+    ~HipContext() { uhip_ctx_destroy(ctx_); }
+    HipContext(const HipContext &) = delete;
+    HipContext &operator=(const HipContext &) = delete;
+    uhip_ctx get() const { return ctx_; }
+    double *scalars() const { return uhip_ctx_scalars(ctx_); }
+    void sync() const { hip_check(uhip_sync(ctx_), "uhip_sync"); }
+    void read_scalars(const double *src, int n, double *dst) const {
+        hip_check(uhip_read_scalars(ctx_, src, n, dst), "uhip_read_scalars");
+    }
+
+private:
+    uhip_ctx ctx_ = nullptr;
+};
+
+using HipContextPtr = std::shared_ptr<HipContext>;
+
+template <typename T>
+struct HipDtype;
+template <>
+struct HipDtype<double> {
+    static constexpr int value = UHIP_F64;
+};
+template <>
+struct HipDtype<float> {
+    static constexpr int value = UHIP_F32;
+};
+
+// ---------------------------------------------------------------------------
+// HipVectorT: contiguous device vector (FP64: HipVector, FP32: HipVector32).
+// Deep copy on copy-construction (PetscVector semantics, utopia_petsc_Vector.hpp:441-458),
+// move supported, wrap() for caller-owned memory (no free).
+// ---------------------------------------------------------------------------
+template <typename T>
+class HipVectorT {
+public:
+    using Scalar = T;
+    using SizeType = int64_t;
+    static constexpr int dtype = HipDtype<T>::value;
+
+    HipVectorT() = default;
+    HipVectorT(HipContextPtr ctx, int64_t n) : ctx_(std::move(ctx)) { allocate(n); }
+    HipVectorT(HipContextPtr ctx, int64_t n, T value) : ctx_(std::move(ctx)) {
+        allocate(n);
+        set(value);
+    }
+    ~HipVectorT() { release(); }
+
+    HipVectorT(const HipVectorT &o) : ctx_(o.ctx_) {
+        if (o.n_ > 0 || o.d_) {
+            allocate(o.n_);
+            copy_from(o);
+        }
+    }
+    HipVectorT(HipVectorT &&o) noexcept : ctx_(std::move(o.ctx_)), d_(o.d_), n_(o.n_), own_(o.own_) {
+        o.d_ = nullptr;
+        o.n_ = 0;
+    }
+    HipVectorT &operator=(const HipVectorT &o) {
+        if (this == &o) return *this;
+        if (!ctx_) ctx_ = o.ctx_;
+        if (n_ != o.n_ || !d_) {
+            release();
+            allocate(o.n_);
+        }
+        copy_from(o);
+        return *this;
+    }
+    HipVectorT &operator=(HipVectorT &&o) noexcept {
+        if (this == &o) return *this;
+        release();
+        ctx_ = std::move(o.ctx_);
+        d_ = o.d_;
+        n_ = o.n_;
+        own_ = o.own_;
+        o.d_ = nullptr;
+        o.n_ = 0;
+        return *this;
+    }
+
+    // PetscVector::wrap: caller-owned device memory, never freed here.
+    void wrap(HipContextPtr ctx, T *ptr, int64_t n) {
+        release();
+        ctx_ = std::move(ctx);
+        d_ = ptr;
+        n_ = n;
+        own_ = false;
+    }
+
+    void zeros(int64_t n) { values(n, T(0)); }
+    void values(int64_t n, T v) {
+        if (n != n_ || !d_) {
+            release();
+            allocate(n);
+        }
+        set(v);
+    }
+    void zeros_like(const HipVectorT &o) {
+        if (!ctx_) ctx_ = o.ctx_;
+        zeros(o.size());
+    }
+
+    int64_t size() const { return n_; }
+    bool empty() const { return d_ == nullptr; }
+    T *data() { return d_; }
+    const T *data() const { return d_; }
+    const HipContextPtr &context() const { return ctx_; }
+    uhip_ctx ctx() const { return ctx_->get(); }
+    bool is_alias(const HipVectorT &o) const { return d_ == o.d_; }
+
+    // BLAS1 -----------------------------------------------------------------
+    void set(T v);
+    void copy_from(const HipVectorT &o);
+    void axpy(T a, const HipVectorT &x);           // this += a x
+    void axpby(T a, const HipVectorT &x, T b);     // this = a x + b this
+    void scale(T a);
+    void shift(T a);
+    void abs();
+    void e_mul(const HipVectorT &x);               // this = this .* x
+    double dot(const HipVectorT &y) const;
+    double norm2() const;
+    double norm_infty() const;
+    double sum() const;
+    double min() const;
+    double max() const;
+    bool has_nan_or_inf() const;
+
+    void upload(const T *host) { hip_check(uhip_memcpy_htod(ctx(), d_, host, sizeof(T) * n_), "upload"); }
+    void download(T *host) const { hip_check(uhip_memcpy_dtoh(ctx(), host, d_, sizeof(T) * n_), "download"); }
+    std::vector<T> to_host() const {
+        std::vector<T> h((size_t)n_);
+        download(h.data());
+        return h;
+    }
+
+private:
+    void allocate(int64_t n) {
+        if (!ctx_) throw HipError(UHIP_ERR_INVALID, "HipVector: no context");
+        void *p = nullptr;
+        hip_check(uhip_malloc(ctx(), &p, sizeof(T) * (size_t)(n > 0 ? n : 1)), "HipVector alloc");
+        d_ = (T *)p;
+        n_ = n;
+        own_ = true;
+    }
+    void release() {
+        if (d_ && own_ && ctx_) uhip_free(ctx(), d_);
+        d_ = nullptr;
+        n_ = 0;
+        own_ = true;
+    }
+    double reduce(int op) const {
+        double *s = ctx_->scalars() + (UHIP_NUM_SCALARS - 1);
+        int st = dtype == UHIP_F64 ? uhip_vec_reduce_f64(ctx(), n_, (const double *)d_, op, s)
+                                   : uhip_vec_reduce_f32(ctx(), n_, (const float *)d_, op, s);
+        hip_check(st, "reduce");
+        double h = 0;
+        ctx_->read_scalars(s, 1, &h);
+        return h;
+    }
+
+    HipContextPtr ctx_;
+    T *d_ = nullptr;
+    int64_t n_ = 0;
+    bool own_ = true;
+};
+
+using HipVector = HipVectorT<double>;
+using HipVector32 = HipVectorT<float>;
+
+#define UTOPIA_HIP_BLAS1(T, SUF)                                                                              \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::set(T v) {                                                                     \
+        hip_check(uhip_vec_set_##SUF(ctx(), n_, v, d_), "set");                                               \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::copy_from(const HipVectorT<T> &o) {                                            \
+        hip_check(uhip_vec_copy_##SUF(ctx(), n_, o.d_, d_), "copy");                                          \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::axpy(T a, const HipVectorT<T> &x) {                                            \
+        hip_check(uhip_vec_axpy_##SUF(ctx(), n_, a, x.d_, d_), "axpy");                                       \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::axpby(T a, const HipVectorT<T> &x, T b) {                                      \
+        hip_check(uhip_vec_axpby_##SUF(ctx(), n_, a, x.d_, b, d_), "axpby");                                  \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::scale(T a) {                                                                   \
+        hip_check(uhip_vec_scale_##SUF(ctx(), n_, a, d_), "scale");                                           \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::shift(T a) {                                                                   \
+        hip_check(uhip_vec_shift_##SUF(ctx(), n_, a, d_), "shift");                                           \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::abs() {                                                                        \
+        hip_check(uhip_vec_abs_##SUF(ctx(), n_, d_, d_), "abs");                                              \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::e_mul(const HipVectorT<T> &x) {                                                \
+        hip_check(uhip_vec_emul_##SUF(ctx(), n_, d_, x.d_, d_), "e_mul");                                     \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline double HipVectorT<T>::dot(const HipVectorT<T> &y) const {                                          \
+        double *s = ctx_->scalars() + (UHIP_NUM_SCALARS - 1);                                                 \
+        hip_check(uhip_vec_dot_##SUF(ctx(), n_, d_, y.d_, s), "dot");                                         \
+        double h = 0;                                                                                         \
+        ctx_->read_scalars(s, 1, &h);                                                                         \
+        return h;                                                                                             \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline double HipVectorT<T>::norm2() const {                                                              \
+        double *s = ctx_->scalars() + (UHIP_NUM_SCALARS - 1);                                                 \
+        hip_check(uhip_vec_norm2_##SUF(ctx(), n_, d_, s), "norm2");                                           \
+        double h = 0;                                                                                         \
+        ctx_->read_scalars(s, 1, &h);                                                                         \
+        return h;                                                                                             \
+    }
+
+UTOPIA_HIP_BLAS1(double, f64)
+UTOPIA_HIP_BLAS1(float, f32)
+#undef UTOPIA_HIP_BLAS1
+
+template <typename T>
+double HipVectorT<T>::norm_infty() const {
+    return reduce(3);
+}
+template <typename T>
+double HipVectorT<T>::sum() const {
+    return reduce(0);
+}
+template <typename T>
+double HipVectorT<T>::min() const {
+    return reduce(1);
+}
+template <typename T>
+double HipVectorT<T>::max() const {
+    return reduce(2);
+}
+template <typename T>
+bool HipVectorT<T>::has_nan_or_inf() const {
+    return reduce(4) > 0;
+}
+
+template <typename T>
+inline double dot(const HipVectorT<T> &a, const HipVectorT<T> &b) {
+    return a.dot(b);
+}
+template <typename T>
+inline double norm2(const HipVectorT<T> &a) {
+    return a.norm2();
+}
+
+// ---------------------------------------------------------------------------
+// HipCrsMatrixT: CSR on the device (int32 indices) + row-block SpMV plan.
+// ---------------------------------------------------------------------------
+template <typename T>
+class HipCrsMatrixT {
+public:
+    using Scalar = T;
+    using SizeType = int64_t;
+    using Vector = HipVectorT<T>;
+    static constexpr int dtype = HipDtype<T>::value;
+
+    HipCrsMatrixT() { std::memset(&m_, 0, sizeof(m_)); }
+    ~HipCrsMatrixT() { release(); }
+    HipCrsMatrixT(const HipCrsMatrixT &) = delete;
+    HipCrsMatrixT &operator=(const HipCrsMatrixT &) = delete;
+    HipCrsMatrixT(HipCrsMatrixT &&o) noexcept : ctx_(std::move(o.ctx_)), m_(o.m_), own_(o.own_) {
+        std::memset(&o.m_, 0, sizeof(o.m_));
+    }
+    HipCrsMatrixT &operator=(HipCrsMatrixT &&o) noexcept {
+        if (this != &o) {
+            release();
+            ctx_ = std::move(o.ctx_);
+            m_ = o.m_;
+            own_ = o.own_;
+            std::memset(&o.m_, 0, sizeof(o.m_));
+        }
+        return *this;
+    }
+
+    // Copy host CSR arrays to the device (PetscMatrix::crs, utopia_petsc_Matrix.hpp:362).
+    void from_host(HipContextPtr ctx, int32_t rows, int32_t cols, const int32_t *rowptr, const int32_t *col,
+                   const T *val, int32_t block_nnz = 0) {
+        release();
+        ctx_ = std::move(ctx);
+        own_ = true;
+        m_.nrows = rows;
+        m_.ncols = cols;
+        m_.nnz = rowptr[rows];
+        m_.dtype = dtype;
+        void *p = nullptr;
+        hip_check(uhip_malloc(c(), &p, sizeof(int32_t) * ((size_t)rows + 1)), "alloc rowptr");
+        m_.rowptr = (int32_t *)p;
+        hip_check(uhip_malloc(c(), &p, sizeof(int32_t) * (size_t)(m_.nnz > 0 ? m_.nnz : 1)), "alloc col");
+        m_.col = (int32_t *)p;
+        hip_check(uhip_malloc(c(), &p, sizeof(T) * (size_t)(m_.nnz > 0 ? m_.nnz : 1)), "alloc val");
+        m_.val = p;
+        hip_check(uhip_memcpy_htod(c(), m_.rowptr, rowptr, sizeof(int32_t) * ((size_t)rows + 1)), "h2d rowptr");
+        hip_check(uhip_memcpy_htod(c(), m_.col, col, sizeof(int32_t) * (size_t)m_.nnz), "h2d col");
+        hip_check(uhip_memcpy_htod(c(), m_.val, val, sizeof(T) * (size_t)m_.nnz), "h2d val");
+        plan(block_nnz);
+    }
+
+    // PetscMatrix::wrap (utopia_petsc_Matrix.hpp:651-659): device arrays owned by the caller.
+    void wrap(HipContextPtr ctx, int32_t rows, int32_t cols, int64_t nnz, int32_t *rowptr, int32_t *col, T *val,
+              int32_t block_nnz = 0) {
+        release();
+        ctx_ = std::move(ctx);
+        own_ = false;
+        m_.nrows = rows;
+        m_.ncols = cols;
+        m_.nnz = nnz;
+        m_.rowptr = rowptr;
+        m_.col = col;
+        m_.val = val;
+        m_.dtype = dtype;
+        plan(block_nnz);
+    }
+
+    // Take ownership of arrays produced by the C ABI (transpose / ptap).
+    void adopt(HipContextPtr ctx, const uhip_csr &m, int32_t block_nnz = 0) {
+        release();
+        ctx_ = std::move(ctx);
+        own_ = true;
+        m_ = m;
+        m_.blocks = nullptr;
+        m_.nblocks = 0;
+        plan(block_nnz);
+    }
+
+    void plan(int32_t block_nnz) { hip_check(uhip_csr_plan(c(), &m_, block_nnz), "uhip_csr_plan"); }
+
+    int64_t rows() const { return m_.nrows; }
+    int64_t cols() const { return m_.ncols; }
+    int64_t nnz() const { return m_.nnz; }
+    bool empty() const { return m_.rowptr == nullptr; }
+    const uhip_csr &raw() const { return m_; }
+    uhip_csr &raw() { return m_; }
+    const HipContextPtr &context() const { return ctx_; }
+    uhip_ctx c() const { return ctx_->get(); }
+
+    // y = A x  (PetscMatrix::multiply, utopia_petsc_Matrix.hpp:570)
+    void multiply(const Vector &x, Vector &y) const {
+        if (y.size() != rows() || y.empty()) y = Vector(ctx_, rows());
+        if (x.is_alias(y)) {  // PetscMatrix::multiply copies aliased input (utopia_petsc_Matrix.cpp:1328-1333)
+            Vector tmp(x);
+            hip_check(uhip_spmv(c(), &m_, tmp.data(), y.data()), "spmv");
+            return;
+        }
+        hip_check(uhip_spmv(c(), &m_, x.data(), y.data()), "spmv");
+    }
+    void apply(const Vector &x, Vector &y) const { multiply(x, y); }
+
+    // r = b - A x  (Eval<ResidualExpr>)
+    void residual(const Vector &x, const Vector &b, Vector &r) const {
+        hip_check(uhip_spmv_residual(c(), &m_, x.data(), b.data(), dtype, r.data()), "residual");
+    }
+
+    // d = diag(A)  (PetscMatrix::build_diag, utopia_petsc_Matrix.hpp:814)
+    void build_diag(Vector &d) const {
+        if (d.size() != rows() || d.empty()) d = Vector(ctx_, rows());
+        hip_check(uhip_csr_diag(c(), &m_, d.data()), "diag");
+    }
+
+    // B = A^T (PetscMatrix::transpose, utopia_petsc_Matrix.hpp:601)
+    HipCrsMatrixT transpose() const {
+        static_assert(std::is_same<T, double>::value, "transpose: FP64 setup only");
+        uhip_csr B;
+        std::memset(&B, 0, sizeof(B));
+        hip_check(uhip_csr_transpose(c(), &m_, &B), "transpose");
+        HipCrsMatrixT out;
+        out.adopt(ctx_, B);
+        return out;
+    }
+
+    // FP32 copy of the values, same pattern.
+    HipCrsMatrixT<float> cast_f32() const;
+
+private:
+    void release() {
+        if (!ctx_) return;
+        if (own_) {
+            uhip_csr_free(c(), &m_);
+        } else {
+            uhip_csr_plan_destroy(c(), &m_);
+        }
+        std::memset(&m_, 0, sizeof(m_));
+    }
+
+    HipContextPtr ctx_;
+    uhip_csr m_;
+    bool own_ = true;
+
+    template <typename U>
+    friend class HipCrsMatrixT;
+};
+
+using HipCrsMatrix = HipCrsMatrixT<double>;
+using HipCrsMatrix32 = HipCrsMatrixT<float>;
+
+template <>
+inline HipCrsMatrixT<float> HipCrsMatrixT<double>::cast_f32() const {
+    HipCrsMatrixT<float> o;
+    uhip_csr m = m_;
+    void *p = nullptr;
+    hip_check(uhip_malloc(c(), &p, sizeof(int32_t) * ((size_t)m_.nrows + 1)), "alloc rowptr");
+    m.rowptr = (int32_t *)p;
+    hip_check(uhip_malloc(c(), &p, sizeof(int32_t) * (size_t)(m_.nnz > 0 ? m_.nnz : 1)), "alloc col");
+    m.col = (int32_t *)p;
+    hip_check(uhip_malloc(c(), &p, sizeof(float) * (size_t)(m_.nnz > 0 ? m_.nnz : 1)), "alloc val");
+    m.val = p;
+    m.dtype = UHIP_F32;
+    hip_check(uhip_memcpy_dtod(c(), m.rowptr, m_.rowptr, sizeof(int32_t) * ((size_t)m_.nrows + 1)), "copy rowptr");
+    hip_check(uhip_memcpy_dtod(c(), m.col, m_.col, sizeof(int32_t) * (size_t)m_.nnz), "copy col");
+    hip_check(uhip_csr_cast_values_f64_f32(c(), m_.nnz, (const double *)m_.val, (float *)m.val), "cast vals");
+    o.adopt(ctx_, m, m_.block_capk == 5 ? 1024 : (m_.block_capk == 20 ? 4096 : 2048));
+    return o;
+}
+
+// Ac = P^T A P  (utopia::ptap -> MatPtAP, backend/petsc/eval/utopia_petsc_Eval_Multiply.cpp:12-61)
+inline HipCrsMatrix ptap(const HipCrsMatrix &A, const HipCrsMatrix &P) {
+    uhip_csr Ac;
+    std::memset(&Ac, 0, sizeof(Ac));
+    hip_check(uhip_csr_ptap(A.c(), &A.raw(), &P.raw(), &Ac), "ptap");
+    HipCrsMatrix out;
+    out.adopt(A.context(), Ac);
+    return out;
+}
+
+}  // namespace utopia

@@ -1,0 +1,471 @@
+// Multigrid over the HIP backend: Galerkin hierarchy, masks, smoothers, dense
+// coarse solve, and the fused V-cycle engine (FP64 or FP32 levels).
+#pragma once
+
+#include <chrono>
+
+#include "utopia_hip_solvers.hpp"
+
+namespace utopia {
+
+// Factorization<M,V> on the coarsest level: dense FP64 Cholesky, then the
+// explicit inverse so that every coarse solve is a single GEMV.
+class Factorization final : public LinearSolver<HipCrsMatrix, HipVector> {
+public:
+    ~Factorization() override { release(); }
+    void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
+        LinearSolver::update(op);
+        factorize(*op);
+    }
+    void factorize(const HipCrsMatrix &A) {
+        release();
+        ctx_ = A.context();
+        n_ = (int32_t)A.rows();
+        if (A.rows() != A.cols()) throw HipError(UHIP_ERR_INVALID, "Factorization: square matrix required");
+        void *p = nullptr, *q = nullptr;
+        hip_check(uhip_malloc(ctx_->get(), &p, sizeof(double) * (size_t)n_ * n_), "alloc dense");
+        hip_check(uhip_malloc(ctx_->get(), &q, sizeof(double) * (size_t)n_ * n_), "alloc inverse");
+        double *L = (double *)p;
+        ainv_ = (double *)q;
+        hip_check(uhip_csr_to_dense(ctx_->get(), &A.raw(), L), "to_dense");
+        int info = 0;
+        hip_check(uhip_dense_cholesky(ctx_->get(), n_, L, n_, &info), "cholesky");
+        if (info != 0) {
+            uhip_free(ctx_->get(), L);
+            throw HipError(UHIP_ERR_NUMERIC, "Factorization: coarse operator not SPD (pivot " + std::to_string(info) + ")");
+        }
+        hip_check(uhip_dense_cholesky_inverse(ctx_->get(), n_, L, n_, ainv_), "cholesky inverse");
+        hip_check(uhip_free(ctx_->get(), L), "free");
+    }
+    bool apply(const HipVector &rhs, HipVector &sol) override {
+        if (sol.size() != rhs.size() || sol.empty()) sol.zeros_like(rhs);
+        return solve_raw(rhs.data(), UHIP_F64, sol.data(), UHIP_F64);
+    }
+    bool solve_raw(const void *rhs, int rhs_dt, void *sol, int sol_dt) const {
+        hip_check(uhip_dense_gemv(ctx_->get(), n_, ainv_, rhs, rhs_dt, sol, sol_dt), "coarse gemv");
+        return true;
+    }
+    int32_t n() const { return n_; }
+    const double *inverse() const { return ainv_; }
+
+private:
+    void release() {
+        if (ainv_ && ctx_) uhip_free(ctx_->get(), ainv_);
+        ainv_ = nullptr;
+    }
+    HipContextPtr ctx_;
+    double *ainv_ = nullptr;
+    int32_t n_ = 0;
+};
+
+// Smoother prototypes handed to Multigrid (cloned per level by the reference).
+template <class Matrix, class Vector>
+class Chebyshev3level final : public IterativeSolver<Matrix, Vector> {
+public:
+    using T = typename Vector::Scalar;
+    Chebyshev3level() { this->max_it_ = 1; }
+    void scale_max_eig(double v) { scale_max_eig_ = v; }
+    void scale_min_eig(double v) { scale_min_eig_ = v; }
+    double scale_max_eig() const { return scale_max_eig_; }
+    double scale_min_eig() const { return scale_min_eig_; }
+    void power_method_tol(double t) { pm_.tol(t); }
+    void power_method_max_it(int64_t m) { pm_.max_it(m); }
+    const PowerMethod<Matrix, Vector> &power_method() const { return pm_; }
+    void set_eigs(double emax, double emin) {
+        eig_max_ = emax;
+        eig_min_ = emin;
+    }
+    double eig_max() const { return eig_max_; }
+    double eig_min() const { return eig_min_; }
+
+    // update -> init_eigs (utopia_Chebyshev3level.hpp:90-121)
+    void update(const std::shared_ptr<const Matrix> &op) override {
+        IterativeSolver<Matrix, Vector>::update(op);
+        eig_max_ = scale_max_eig_ * pm_.get_max_eig(*op);
+        eig_min_ = scale_min_eig_ * eig_max_;
+        const int64_t n = op->rows();
+        for (Vector *v : {&rk_, &pa_, &pb_}) *v = Vector(op->context(), n, 0);
+    }
+    // smooth: max_it(sweeps) steps from the given x (utopia_Chebyshev3level.hpp:109-116)
+    bool apply(const Vector &b, Vector &x) override {
+        if (x.empty() || x.size() != b.size()) x.zeros_like(b);
+        std::vector<double> a, be;
+        chebyshev_coefficients(eig_max_, eig_min_, (int)this->max_it_, a, be);
+        chebyshev_smooth_fused<T>(*this->op_, b.data(), Vector::dtype, x.data(), false, x.data(), Vector::dtype,
+                                  (int)this->max_it_, a, be, rk_.data(), pa_.data(), pb_.data());
+        return true;
+    }
+
+private:
+    double scale_max_eig_ = 1.2, scale_min_eig_ = 0.06, eig_max_ = 0, eig_min_ = 0;
+    PowerMethod<Matrix, Vector> pm_;
+    Vector rk_, pa_, pb_;
+};
+
+// Jacobi (utopia_Jacobi.hpp:89-140): r = rhs - A x; x += d_inv .* r per sweep, d_inv = pseudo_inv(diag, 0)
+template <class Matrix, class Vector>
+class Jacobi final : public IterativeSolver<Matrix, Vector> {
+public:
+    using T = typename Vector::Scalar;
+    Jacobi() { this->max_it_ = 1; }
+    void update(const std::shared_ptr<const Matrix> &op) override {
+        IterativeSolver<Matrix, Vector>::update(op);
+        op->build_diag(d_inv_);
+        InvDiagPreconditioner<Matrix, Vector>::pseudo_inv(d_inv_, T(0));
+        r_ = Vector(op->context(), op->rows(), 0);
+    }
+    bool apply(const Vector &b, Vector &x) override {
+        if (x.empty() || x.size() != b.size()) x.zeros_like(b);
+        for (int64_t s = 0; s < this->max_it_; ++s) {
+            this->op_->residual(x, b, r_);
+            r_.e_mul(d_inv_);
+            x.axpy(T(1), r_);
+        }
+        return true;
+    }
+    const Vector &d_inv() const { return d_inv_; }
+
+private:
+    Vector d_inv_, r_;
+};
+
+// ---------------------------------------------------------------------------
+// The V-cycle engine on levels with values T. Level 0 is the coarsest.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct MGLevel {
+    std::shared_ptr<const HipCrsMatrixT<T>> A;  // level operator
+    std::shared_ptr<const HipCrsMatrixT<T>> R;  // restriction to level l-1 (l >= 1)
+    std::shared_ptr<const HipCrsMatrixT<T>> P;  // interpolation from level l-1 (l >= 1)
+    HipVectorT<T> mask;                        // mask of this level (empty: masks off)
+    HipVectorT<T> r, c, rR, rk, pa, pb, dinv;
+    std::vector<double> alpha, beta;            // Chebyshev coefficients for pre / post
+    std::vector<double> alpha_post, beta_post;
+    double eig_max = 0, eig_min = 0;
+};
+
+enum MGSmoother { MG_CHEBYSHEV = 0, MG_JACOBI = 1 };
+
+template <typename T>
+class VCycle {
+public:
+    static constexpr int dt = HipDtype<T>::value;
+    std::vector<MGLevel<T>> levels;
+    const Factorization *coarse = nullptr;
+    int pre = 3, post = 3, mg_type = 1, smoother = MG_CHEBYSHEV;
+    bool masks = true;
+
+    void allocate_work() {
+        for (size_t l = 0; l < levels.size(); ++l) {
+            auto &L = levels[l];
+            const int64_t n = L.A->rows();
+            auto ctx = L.A->context();
+            for (HipVectorT<T> *v : {&L.r, &L.c, &L.rR, &L.rk, &L.pa, &L.pb}) *v = HipVectorT<T>(ctx, n, T(0));
+            if (smoother == MG_CHEBYSHEV && l > 0) {
+                chebyshev_coefficients(L.eig_max, L.eig_min, pre, L.alpha, L.beta);
+                chebyshev_coefficients(L.eig_max, L.eig_min, post, L.alpha_post, L.beta_post);
+            }
+        }
+    }
+
+    // Multigrid::standard_cycle (utopia_Multigrid.hpp:252-367). r: level rhs
+    // (dtype r_dt); the correction c_l is accumulated in levels[l].c and the
+    // final post-smoothed value is written to out (dtype out_dt).
+    void cycle(int l, const void *r, int r_dt, void *out, int out_dt) {
+        auto &L = levels[(size_t)l];
+        const int64_t n = L.A->rows();
+        uhip_ctx c = L.A->c();
+        if (l == 0) {
+            coarse->solve_raw(r, r_dt, out, out_dt);
+            return;
+        }
+        auto &Lc = levels[(size_t)l - 1];
+        for (int k = 0; k < mg_type; ++k) {
+            const bool first = (k == 0);
+            const bool final_pass = (k == mg_type - 1);
+            smooth(L, r, r_dt, /*zero_guess=*/first, L.c.data(), dt, pre, L.alpha, L.beta);
+            // r_R = r - A c
+            hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), r, r_dt, L.rR.data()), "mg residual");
+            // r_{l-1} = mask .* (R r_R)
+            hip_check(uhip_spmv_masked(c, &L.R->raw(), L.rR.data(), masks && !Lc.mask.empty() ? Lc.mask.data() : nullptr,
+                                       Lc.r.data()),
+                      "mg restrict");
+            cycle(l - 1, Lc.r.data(), dt, Lc.c.data(), dt);
+            // c += I c_{l-1}
+            hip_check(uhip_spmv_add(c, &L.P->raw(), Lc.c.data(), L.c.data()), "mg interpolate");
+            void *o = final_pass ? out : (void *)L.c.data();
+            const int odt = final_pass ? out_dt : dt;
+            smooth(L, r, r_dt, false, o, odt, post, L.alpha_post, L.beta_post);
+        }
+        (void)n;
+    }
+
+private:
+    void smooth(MGLevel<T> &L, const void *b, int b_dt, bool zero_guess, void *out, int out_dt, int nu,
+                const std::vector<double> &a, const std::vector<double> &be) {
+        if (nu <= 0) {
+            if (out != (void *)L.c.data())
+                hip_check(uhip_vec_axpy_cast(L.A->c(), L.A->rows(), 0.0, L.c.data(), dt, L.c.data(), dt, out, out_dt),
+                          "copy");
+            return;
+        }
+        if (smoother == MG_CHEBYSHEV) {
+            chebyshev_smooth_fused<T>(*L.A, b, b_dt, L.c.data(), zero_guess, out, out_dt, nu, a, be, L.rk.data(),
+                                      L.pa.data(), L.pb.data());
+            return;
+        }
+        // Jacobi sweeps: r = b - A x; x += d_inv .* r (x = 0 on a zero guess)
+        uhip_ctx c = L.A->c();
+        const int64_t n = L.A->rows();
+        if (zero_guess) L.c.set(T(0));
+        for (int s = 0; s < nu; ++s) {
+            hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), b, b_dt, L.rk.data()), "jacobi r");
+            L.rk.e_mul(L.dinv);
+            const bool last = (s == nu - 1);
+            hip_check(uhip_vec_axpy_cast(c, n, 1.0, L.rk.data(), dt, L.c.data(), dt, last ? out : (void *)L.c.data(),
+                                         last ? out_dt : dt),
+                      "jacobi x");
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Multigrid<HipCrsMatrix, HipVector>
+// ---------------------------------------------------------------------------
+class Multigrid : public IterativeSolver<HipCrsMatrix, HipVector> {
+public:
+    using Smoother = IterativeSolver<HipCrsMatrix, HipVector>;
+    static const int V_CYCLE = 1;
+    static const int W_CYCLE = 2;
+
+    Multigrid(const std::shared_ptr<Smoother> &smoother, const std::shared_ptr<Factorization> &coarse_solver)
+        : smoother_(smoother), coarse_(coarse_solver) {}
+
+    // Interpolation operators ordered coarse to fine; R = transpose(I) kept
+    // explicitly (LinearMultiLevel::set_transfer_operators + IPRTransfer).
+    bool set_transfer_operators(const std::vector<std::shared_ptr<HipCrsMatrix>> &interpolations) {
+        I_.clear();
+        R_.clear();
+        for (auto &I : interpolations) {
+            I_.push_back(I);
+            R_.push_back(std::make_shared<HipCrsMatrix>(I->transpose()));
+        }
+        n_levels_ = (int)interpolations.size() + 1;
+        return true;
+    }
+
+    void pre_smoothing_steps(int n) { pre_ = n; }
+    void post_smoothing_steps(int n) { post_ = n; }
+    int pre_smoothing_steps() const { return pre_; }
+    int post_smoothing_steps() const { return post_; }
+    void mg_type(int t) { mg_type_ = t; }
+    void must_generate_masks(bool v) { masks_ = v; }
+    bool must_generate_masks() const { return masks_; }
+    void fix_semidefinite_operators(bool v) { fix_semi_ = v; }
+    int n_levels() const { return n_levels_; }
+    const HipCrsMatrix &level_operator(int l) const { return *A_[(size_t)l]; }
+    const HipVector &mask(int l) const { return mask_[(size_t)l]; }
+    double setup_seconds() const { return setup_s_; }
+    const VCycle<double> &engine() const { return eng_; }
+    const std::shared_ptr<Factorization> &coarse_solver() const { return coarse_; }
+    // Chebyshev bounds of level l (for tests that compare against fixed bounds).
+    void set_level_eigs(int l, double emax, double emin) {
+        eng_.levels[(size_t)l].eig_max = emax;
+        eng_.levels[(size_t)l].eig_min = emin;
+        eng_.allocate_work();
+    }
+    double level_eig_max(int l) const { return eng_.levels[(size_t)l].eig_max; }
+    double level_eig_min(int l) const { return eng_.levels[(size_t)l].eig_min; }
+
+    // Multigrid::update (utopia_Multigrid.hpp:129-153)
+    void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
+        const auto t0 = std::chrono::steady_clock::now();
+        IterativeSolver::update(op);
+        const int L = n_levels_;
+        if (L < 2 || (int)I_.size() != L - 1) throw HipError(UHIP_ERR_INVALID, "Multigrid: set_transfer_operators first");
+        auto ctx = op->context();
+        A_.assign((size_t)L, nullptr);
+        A_[(size_t)L - 1] = op;
+        // masks (MultiLevelMask::generate_masks, before the Galerkin products)
+        mask_.assign((size_t)L, HipVector());
+        if (masks_) {
+            HipVector &mf = mask_[(size_t)L - 1];
+            mf = HipVector(ctx, op->rows());
+            hip_check(uhip_csr_offdiag_indicator(op->c(), &op->raw(), std::numeric_limits<double>::epsilon() * 1e6,
+                                                 mf.data()),
+                      "mask");
+            mf.scale(-1.0);  // off_value 1, on_value 0 where an off-diagonal exists
+            mf.shift(1.0);
+            for (int l = L - 1; l > 0; --l) {
+                HipVector &mc = mask_[(size_t)l - 1];
+                mc = HipVector(ctx, R_[(size_t)l - 1]->rows());
+                hip_check(uhip_csr_boolean_restrict_or(op->c(), &R_[(size_t)l - 1]->raw(),
+                                                       std::numeric_limits<double>::epsilon() * 1e6,
+                                                       mask_[(size_t)l].data(), mc.data()),
+                          "boolean_restrict_or");
+            }
+            for (auto &m : mask_) {
+                m.shift(-1.0);
+                m.abs();
+            }
+        }
+        // Galerkin assembly (LinearMultiLevel::galerkin_assembly)
+        for (int l = L - 1; l > 0; --l) {
+            auto Ac = std::make_shared<HipCrsMatrix>(ptap(*A_[(size_t)l], *I_[(size_t)l - 1]));
+            if (fix_semi_) hip_check(uhip_csr_fix_semidefinite(Ac->c(), &Ac->raw()), "fix_semidefinite");
+            A_[(size_t)l - 1] = Ac;
+        }
+        // smoothers, levels 1..L-1 in this order (rand() draws of the power method)
+        auto cheb = std::dynamic_pointer_cast<Chebyshev3level<HipCrsMatrix, HipVector>>(smoother_);
+        auto jac = std::dynamic_pointer_cast<Jacobi<HipCrsMatrix, HipVector>>(smoother_);
+        if (!cheb && !jac) throw HipError(UHIP_ERR_UNSUPPORTED, "Multigrid: smoother must be Chebyshev3level or Jacobi");
+        eng_.levels.clear();
+        eng_.levels.resize((size_t)L);
+        eng_.pre = pre_;
+        eng_.post = post_;
+        eng_.mg_type = mg_type_;
+        eng_.masks = masks_;
+        eng_.smoother = cheb ? MG_CHEBYSHEV : MG_JACOBI;
+        for (int l = 0; l < L; ++l) {
+            auto &lv = eng_.levels[(size_t)l];
+            lv.A = A_[(size_t)l];
+            if (l > 0) {
+                lv.R = R_[(size_t)l - 1];
+                lv.P = I_[(size_t)l - 1];
+            }
+            if (masks_) lv.mask = mask_[(size_t)l];
+        }
+        for (int l = 1; l < L; ++l) {
+            auto &lv = eng_.levels[(size_t)l];
+            if (cheb) {
+                PowerMethod<HipCrsMatrix, HipVector> pm;
+                pm.tol(cheb->power_method().tol());
+                pm.max_it(cheb->power_method().max_it());
+                lv.eig_max = cheb->scale_max_eig() * pm.get_max_eig(*A_[(size_t)l]);
+                lv.eig_min = cheb->scale_min_eig() * lv.eig_max;
+            } else {
+                A_[(size_t)l]->build_diag(lv.dinv);
+                InvDiagPreconditioner<HipCrsMatrix, HipVector>::pseudo_inv(lv.dinv, 0.0);
+            }
+        }
+        coarse_->factorize(*A_[0]);
+        eng_.coarse = coarse_.get();
+        eng_.allocate_work();
+        r_top_ = HipVector(ctx, op->rows(), 0);
+        c_top_ = HipVector(ctx, op->rows(), 0);
+        op->context()->sync();
+        setup_s_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+    // Multigrid::apply (utopia_Multigrid.hpp:162-238), x is the initial guess.
+    bool apply(const HipVector &rhs, HipVector &x) override {
+        const HipCrsMatrix &A = *A_.back();
+        if (x.empty() || x.size() != rhs.size()) x.zeros_like(rhs);
+        const int l = n_levels_ - 1;
+        A.residual(x, rhs, r_top_);
+        double r_norm = r_top_.norm2();
+        const double r0_norm = r_norm;
+        int64_t it = 1;
+        bool converged = false;
+        this->status_.clear();
+        while (!converged) {
+            eng_.cycle(l, r_top_.data(), UHIP_F64, c_top_.data(), UHIP_F64);
+            x.axpy(1.0, c_top_);
+            A.residual(x, rhs, r_top_);
+            r_norm = r_top_.norm2();
+            const double rel_norm = r_norm / r0_norm;
+            this->print_iter(it, r_norm);
+            converged = this->check_convergence(it, r_norm, rel_norm, 1);
+            it++;
+        }
+        return true;
+    }
+
+    // Preconditioner use (CG calls it with z = 0): with max_it == 1 the cycle
+    // result is the answer: r = rhs - A*0 = rhs, x = 0 + c = c, and the closing
+    // residual only feeds check_convergence, which stops at it >= max_it anyway.
+    bool apply_zero_initial_guess(const HipVector &rhs, HipVector &z) override {
+        if (this->max_it_ != 1 || this->verbose_) return LinearSolver::apply_zero_initial_guess(rhs, z);
+        if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
+        eng_.cycle(n_levels_ - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64);
+        return true;
+    }
+
+    const std::vector<std::shared_ptr<HipCrsMatrix>> &interpolations() const { return I_; }
+    const std::vector<std::shared_ptr<HipCrsMatrix>> &restrictions() const { return R_; }
+    const std::vector<std::shared_ptr<const HipCrsMatrix>> &levels() const { return A_; }
+
+private:
+    std::shared_ptr<Smoother> smoother_;
+    std::shared_ptr<Factorization> coarse_;
+    std::vector<std::shared_ptr<HipCrsMatrix>> I_, R_;
+    std::vector<std::shared_ptr<const HipCrsMatrix>> A_;
+    std::vector<HipVector> mask_;
+    int n_levels_ = 0, pre_ = 3, post_ = 3, mg_type_ = 1;
+    bool masks_ = true, fix_semi_ = false;
+    VCycle<double> eng_;
+    HipVector r_top_, c_top_;
+    double setup_s_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Mixed precision: the V-cycle of an FP64-assembled Multigrid run on FP32
+// copies of every level operator, transfer and mask (int32 indices shared
+// layout), with the FP64 coarse inverse. The FP64 residual enters and the FP64
+// correction leaves through the first / last kernels of the cycle.
+// ---------------------------------------------------------------------------
+class MixedPrecisionMultigrid final : public LinearSolver<HipCrsMatrix, HipVector> {
+public:
+    explicit MixedPrecisionMultigrid(std::shared_ptr<Multigrid> mg) : mg_(std::move(mg)) {}
+    void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
+        LinearSolver::update(op);
+        mg_->update(op);
+        const auto &src = mg_->engine();
+        eng_.levels.clear();
+        eng_.levels.resize(src.levels.size());
+        eng_.pre = src.pre;
+        eng_.post = src.post;
+        eng_.mg_type = src.mg_type;
+        eng_.masks = src.masks;
+        eng_.smoother = src.smoother;
+        eng_.coarse = src.coarse;
+        if (eng_.smoother != MG_CHEBYSHEV) throw HipError(UHIP_ERR_UNSUPPORTED, "mixed precision: Chebyshev smoother only");
+        for (size_t l = 0; l < src.levels.size(); ++l) {
+            auto &d = eng_.levels[l];
+            const auto &s = src.levels[l];
+            d.A = std::make_shared<HipCrsMatrix32>(s.A->cast_f32());
+            if (l > 0) {
+                d.R = std::make_shared<HipCrsMatrix32>(s.R->cast_f32());
+                d.P = std::make_shared<HipCrsMatrix32>(s.P->cast_f32());
+            }
+            if (!s.mask.empty()) {
+                d.mask = HipVector32(op->context(), s.mask.size());
+                hip_check(uhip_vec_cast_f64_f32(op->c(), s.mask.size(), s.mask.data(), d.mask.data()), "mask cast");
+            }
+            d.eig_max = s.eig_max;
+            d.eig_min = s.eig_min;
+        }
+        eng_.allocate_work();
+    }
+    bool apply(const HipVector &rhs, HipVector &sol) override {
+        if (sol.empty() || sol.size() != rhs.size()) sol.zeros_like(rhs);
+        HipVector c(rhs.context(), rhs.size(), 0.0), r(rhs);
+        const HipCrsMatrix &A = *this->op_;
+        A.residual(sol, rhs, r);
+        eng_.cycle((int)eng_.levels.size() - 1, r.data(), UHIP_F64, c.data(), UHIP_F64);
+        sol.axpy(1.0, c);
+        return true;
+    }
+    bool apply_zero_initial_guess(const HipVector &rhs, HipVector &z) override {
+        if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
+        eng_.cycle((int)eng_.levels.size() - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64);
+        return true;
+    }
+    const Multigrid &fp64() const { return *mg_; }
+
+private:
+    std::shared_ptr<Multigrid> mg_;
+    VCycle<float> eng_;
+};
+
+}  // namespace utopia

@@ -1,0 +1,331 @@
+// Row-block plan, SpMV family (MatMult and its fused callers), and the
+// per-row matrix utilities (diagonal, masks, semidefinite fix).
+#include <hipcub/hipcub.hpp>
+
+#include "uhip_spmv.hpp"
+
+namespace uhip {
+
+// Row r starts a block when its cost window (rowptr[r] + r) / B differs from the
+// previous row's, or when r or r-1 is a long row (> B/4 nonzeros). A multi-row
+// block then holds < B + B/4 <= 256*capk nonzeros.
+__global__ __launch_bounds__(kBlock) void k_plan_flags(int32_t n, const int32_t *__restrict__ rowptr, int32_t B,
+                                                       uint8_t *__restrict__ flags) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t lim = B / 4;
+    const bool long_r = (rowptr[r + 1] - rowptr[r]) > lim;
+    uint8_t f = 1;
+    if (r > 0) {
+        const int64_t w0 = ((int64_t)rowptr[r - 1] + (r - 1)) / B;
+        const int64_t w1 = ((int64_t)rowptr[r] + r) / B;
+        const bool long_prev = (rowptr[r] - rowptr[r - 1]) > lim;
+        f = (w0 != w1) || long_r || long_prev;
+    }
+    flags[r] = f;
+}
+
+__global__ void k_plan_tail(int32_t *__restrict__ blocks, const int32_t *__restrict__ count, int32_t n) {
+    blocks[*count] = n;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const int32_t *__restrict__ rowptr,
+                                                 const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                 T *__restrict__ d) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    T v = T(0);
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j)
+        if (col[j] == r) v = val[j];
+    d[r] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fix_semidef(int32_t n, const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col, T *__restrict__ val) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j)
+        if (col[j] == r && fabs((double)val[j]) < 1e-12) val[j] = val[j] + T(1);
+}
+
+// ind_r = 1 if row r has an off-diagonal entry with |a| > tol, else 0.
+template <typename TA, typename TV>
+__global__ __launch_bounds__(kBlock) void k_offdiag_ind(int32_t n, const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col,
+                                                        const TA *__restrict__ val, double tol,
+                                                        TV *__restrict__ ind) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    TV v = TV(0);
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j)
+        if (col[j] != r && fabs((double)val[j]) > tol) v = TV(1);
+    ind[r] = v;
+}
+
+// out_r = min(1, sum_j [|R_rj| > tol] * in_j)
+template <typename TA, typename TV>
+__global__ __launch_bounds__(kBlock) void k_bool_restrict(int32_t n, const int32_t *__restrict__ rowptr,
+                                                          const int32_t *__restrict__ col,
+                                                          const TA *__restrict__ val, double tol,
+                                                          const TV *__restrict__ in, TV *__restrict__ out) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    TV s = TV(0);
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j)
+        s += (fabs((double)val[j]) > tol ? TV(1) : TV(0)) * in[col[j]];
+    out[r] = s > TV(1) ? TV(1) : s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_vals(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        b[i] = (float)a[i];
+}
+
+template <typename TA, typename TV, typename TAcc = TV>
+static int spmv_typed(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
+    EpiStore<TAcc, TV> e{(TV *)y};
+    return launch_rowblock<TAcc>(ctx, A, (const TV *)x, e);
+}
+
+}  // namespace uhip
+
+using namespace uhip;
+
+extern "C" {
+
+int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz) {
+    UHIP_REQUIRE(ctx && A && A->rowptr, "uhip_csr_plan: null argument");
+    UHIP_REQUIRE(A->nrows >= 0 && A->ncols >= 0, "uhip_csr_plan: bad shape");
+    if (block_nnz == 0) block_nnz = 2048;
+    int capk;
+    if (block_nnz == 1024) capk = 5;
+    else if (block_nnz == 2048) capk = 10;
+    else if (block_nnz == 4096) capk = 20;
+    else return set_error(UHIP_ERR_INVALID, "uhip_csr_plan: block_nnz %d not in {1024,2048,4096}", block_nnz);
+    if (A->blocks) UHIP_TRY(uhip_csr_plan_destroy(ctx, A));
+    const int32_t n = A->nrows;
+    hipStream_t s = ctx->stream;
+    A->block_capk = capk;
+    if (n == 0) {
+        A->nblocks = 0;
+        UHIP_CHECK_HIP(hipMalloc(&A->blocks, sizeof(int32_t)));
+        UHIP_CHECK_HIP(hipMemsetAsync(A->blocks, 0, sizeof(int32_t), s));
+        return UHIP_OK;
+    }
+    uint8_t *flags = nullptr;
+    int32_t *out = nullptr, *count = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    UHIP_CHECK_HIP(hipMalloc(&flags, n));
+    UHIP_CHECK_HIP(hipMalloc(&out, sizeof(int32_t) * ((size_t)n + 1)));
+    UHIP_CHECK_HIP(hipMalloc(&count, sizeof(int32_t)));
+    hipLaunchKernelGGL(k_plan_flags, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr, block_nnz,
+                       flags);
+    UHIP_CHECK_LAUNCH();
+    hipcub::CountingInputIterator<int32_t> it(0);
+    UHIP_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, it, flags, out, count, n, s));
+    UHIP_CHECK_HIP(hipMalloc(&tmp, tmp_bytes));
+    UHIP_CHECK_HIP(hipcub::DeviceSelect::Flagged(tmp, tmp_bytes, it, flags, out, count, n, s));
+    hipLaunchKernelGGL(k_plan_tail, dim3(1), dim3(1), 0, s, out, count, n);
+    UHIP_CHECK_LAUNCH();
+    int32_t nb = 0;
+    UHIP_CHECK_HIP(hipMemcpyAsync(&nb, count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    UHIP_CHECK_HIP(hipMalloc(&A->blocks, sizeof(int32_t) * ((size_t)nb + 1)));
+    UHIP_CHECK_HIP(hipMemcpyAsync(A->blocks, out, sizeof(int32_t) * ((size_t)nb + 1), hipMemcpyDeviceToDevice, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    hipFree(flags);
+    hipFree(out);
+    hipFree(count);
+    hipFree(tmp);
+    A->nblocks = nb;
+    return UHIP_OK;
+}
+
+int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
+    UHIP_REQUIRE(A, "uhip_csr_plan_destroy: null matrix");
+    if (A->blocks) {
+        if (ctx) UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        UHIP_CHECK_HIP(hipFree(A->blocks));
+    }
+    A->blocks = nullptr;
+    A->nblocks = 0;
+    return UHIP_OK;
+}
+
+int uhip_spmv(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
+    UHIP_REQUIRE(ctx && A && x && y, "uhip_spmv: null argument");
+    if (A->dtype == UHIP_F64) return spmv_typed<double, double>(ctx, A, x, y);
+    if (A->dtype == UHIP_F32) return spmv_typed<float, float>(ctx, A, x, y);
+    return set_error(UHIP_ERR_INVALID, "uhip_spmv: dtype %d", A->dtype);
+}
+
+int uhip_spmv_axpby(uhip_ctx ctx, const uhip_csr *A, double alpha, const void *x, double beta, void *y) {
+    UHIP_REQUIRE(ctx && A && x && y, "uhip_spmv_axpby: null argument");
+    if (A->dtype == UHIP_F64) {
+        EpiAxpby<double, double> e{(double *)y, alpha, beta};
+        return launch_rowblock<double>(ctx, A, (const double *)x, e);
+    }
+    EpiAxpby<float, float> e{(float *)y, (float)alpha, (float)beta};
+    return launch_rowblock<float>(ctx, A, (const float *)x, e);
+}
+
+int uhip_spmv_axpbz(uhip_ctx ctx, const uhip_csr *A, double alpha, const void *x, double beta, const void *z,
+                    void *y) {
+    UHIP_REQUIRE(ctx && A && x && z && y, "uhip_spmv_axpbz: null argument");
+    if (A->dtype == UHIP_F64) {
+        EpiAxpbz<double, double> e{(const double *)z, (double *)y, alpha, beta};
+        return launch_rowblock<double>(ctx, A, (const double *)x, e);
+    }
+    EpiAxpbz<float, float> e{(const float *)z, (float *)y, (float)alpha, (float)beta};
+    return launch_rowblock<float>(ctx, A, (const float *)x, e);
+}
+
+int uhip_spmv_residual(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *b, int b_dtype, void *r) {
+    UHIP_REQUIRE(ctx && A && x && b && r, "uhip_spmv_residual: null argument");
+    if (A->dtype == UHIP_F64) {
+        UHIP_REQUIRE(b_dtype == UHIP_F64, "uhip_spmv_residual: FP64 matrix needs FP64 b");
+        EpiResidual<double, double, double> e{(const double *)b, (double *)r};
+        return launch_rowblock<double>(ctx, A, (const double *)x, e);
+    }
+    if (b_dtype == UHIP_F64) {
+        EpiResidual<float, double, float> e{(const double *)b, (float *)r};
+        return launch_rowblock<float>(ctx, A, (const float *)x, e);
+    }
+    EpiResidual<float, float, float> e{(const float *)b, (float *)r};
+    return launch_rowblock<float>(ctx, A, (const float *)x, e);
+}
+
+int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, double *dot_d) {
+    UHIP_REQUIRE(ctx && A && x && y && dot_d, "uhip_spmv_dot: null argument");
+    UHIP_REQUIRE(A->nrows == A->ncols, "uhip_spmv_dot: matrix must be square");
+    if (A->dtype == UHIP_F64) {
+        EpiStoreDot<double, double, double> e{(const double *)x, (double *)y};
+        UHIP_TRY(launch_rowblock<double>(ctx, A, (const double *)x, e));
+    } else {
+        EpiStoreDot<float, float, float> e{(const float *)x, (float *)y};
+        UHIP_TRY(launch_rowblock<float>(ctx, A, (const float *)x, e));
+    }
+    return finish_sums(ctx, 1, A->nblocks, ctx->max_partials, dot_d);
+}
+
+int uhip_spmv_masked(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *mask, void *y) {
+    UHIP_REQUIRE(ctx && A && x && y, "uhip_spmv_masked: null argument");
+    if (A->dtype == UHIP_F64) {
+        EpiMaskStore<double, double> e{(const double *)mask, (double *)y};
+        return launch_rowblock<double>(ctx, A, (const double *)x, e);
+    }
+    EpiMaskStore<float, float> e{(const float *)mask, (float *)y};
+    return launch_rowblock<float>(ctx, A, (const float *)x, e);
+}
+
+int uhip_spmv_add(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
+    UHIP_REQUIRE(ctx && A && x && y, "uhip_spmv_add: null argument");
+    if (A->dtype == UHIP_F64) {
+        EpiAddTo<double, double> e{(double *)y};
+        return launch_rowblock<double>(ctx, A, (const double *)x, e);
+    }
+    EpiAddTo<float, float> e{(float *)y};
+    return launch_rowblock<float>(ctx, A, (const float *)x, e);
+}
+
+int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *st) {
+    UHIP_REQUIRE(ctx && A && st && st->pprev && st->xout, "uhip_cheb_step_apply: null argument");
+    UHIP_REQUIRE(st->last || (st->rout && st->pout), "uhip_cheb_step_apply: rout/pout needed when last == 0");
+    if (A->dtype == UHIP_F64) {
+        UHIP_REQUIRE(st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F64, "uhip_cheb_step_apply: FP64 level");
+        EpiCheb<double, double, double, double, double> e{
+            (const double *)st->pprev, (const double *)st->rprev, (const double *)st->xin, (double *)st->xout,
+            (double *)st->rout,        (double *)st->pout,        st->a_prev,              st->beta,
+            st->a_cur,                 st->last};
+        return launch_rowblock<double>(ctx, A, (const double *)st->pprev, e);
+    }
+    const float ap = (float)st->a_prev, be = (float)st->beta, ac = (float)st->a_cur;
+#define UHIP_CHEB_CASE(TP, TX)                                                                                     \
+    {                                                                                                              \
+        EpiCheb<float, TP, float, float, TX> e{(const TP *)st->pprev, (const float *)st->rprev,                   \
+                                               (const float *)st->xin, (TX *)st->xout,                            \
+                                               (float *)st->rout,      (float *)st->pout,                         \
+                                               ap,                     be,                                        \
+                                               ac,                     st->last};                                 \
+        return launch_rowblock<float>(ctx, A, (const TP *)st->pprev, e);                                           \
+    }
+    if (st->p_dtype == UHIP_F32 && st->x_dtype == UHIP_F32) UHIP_CHEB_CASE(float, float)
+    if (st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F32) UHIP_CHEB_CASE(double, float)
+    if (st->p_dtype == UHIP_F32 && st->x_dtype == UHIP_F64) UHIP_CHEB_CASE(float, double)
+    if (st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F64) UHIP_CHEB_CASE(double, double)
+#undef UHIP_CHEB_CASE
+    return set_error(UHIP_ERR_INVALID, "uhip_cheb_step_apply: dtypes");
+}
+
+int uhip_csr_diag(uhip_ctx ctx, const uhip_csr *A, void *d) {
+    UHIP_REQUIRE(ctx && A && d, "uhip_csr_diag: null argument");
+    const int32_t n = A->nrows;
+    if (n == 0) return UHIP_OK;
+    dim3 g((n + kBlock - 1) / kBlock);
+    if (A->dtype == UHIP_F64)
+        hipLaunchKernelGGL(k_diag<double>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (const double *)A->val, (double *)d);
+    else
+        hipLaunchKernelGGL(k_diag<float>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (const float *)A->val, (float *)d);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_csr_fix_semidefinite(uhip_ctx ctx, uhip_csr *A) {
+    UHIP_REQUIRE(ctx && A, "uhip_csr_fix_semidefinite: null argument");
+    const int32_t n = A->nrows;
+    if (n == 0) return UHIP_OK;
+    dim3 g((n + kBlock - 1) / kBlock);
+    if (A->dtype == UHIP_F64)
+        hipLaunchKernelGGL(k_fix_semidef<double>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (double *)A->val);
+    else
+        hipLaunchKernelGGL(k_fix_semidef<float>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (float *)A->val);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_csr_cast_values_f64_f32(uhip_ctx ctx, int64_t nnz, const double *src, float *dst) {
+    UHIP_REQUIRE(ctx && (nnz == 0 || (src && dst)), "uhip_csr_cast_values_f64_f32: null argument");
+    if (nnz == 0) return UHIP_OK;
+    hipLaunchKernelGGL(k_cast_vals, dim3(grid_for(nnz)), dim3(kBlock), 0, ctx->stream, nnz, src, dst);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_csr_offdiag_indicator(uhip_ctx ctx, const uhip_csr *A, double tol, void *ind) {
+    UHIP_REQUIRE(ctx && A && ind, "uhip_csr_offdiag_indicator: null argument");
+    const int32_t n = A->nrows;
+    if (n == 0) return UHIP_OK;
+    dim3 g((n + kBlock - 1) / kBlock);
+    if (A->dtype == UHIP_F64)
+        hipLaunchKernelGGL((k_offdiag_ind<double, double>), g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (const double *)A->val, tol, (double *)ind);
+    else
+        hipLaunchKernelGGL((k_offdiag_ind<float, float>), g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (const float *)A->val, tol, (float *)ind);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_csr_boolean_restrict_or(uhip_ctx ctx, const uhip_csr *R, double tol, const void *ind_f, void *ind_c) {
+    UHIP_REQUIRE(ctx && R && ind_f && ind_c, "uhip_csr_boolean_restrict_or: null argument");
+    const int32_t n = R->nrows;
+    if (n == 0) return UHIP_OK;
+    dim3 g((n + kBlock - 1) / kBlock);
+    if (R->dtype == UHIP_F64)
+        hipLaunchKernelGGL((k_bool_restrict<double, double>), g, dim3(kBlock), 0, ctx->stream, n, R->rowptr, R->col,
+                           (const double *)R->val, tol, (const double *)ind_f, (double *)ind_c);
+    else
+        hipLaunchKernelGGL((k_bool_restrict<float, float>), g, dim3(kBlock), 0, ctx->stream, n, R->rowptr, R->col,
+                           (const float *)R->val, tol, (const float *)ind_f, (float *)ind_c);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+}  // extern "C"
