@@ -312,6 +312,8 @@ int uhip_solver_set_box_constraints(uhip_solver s, const double *lb, const doubl
 /* isolver_lsolve_apply: host arrays, x is the initial guess and the result */
 int uhip_solver_apply(uhip_solver s, const double *rhs, double *x);
 int uhip_solver_apply_device(uhip_solver s, const double *rhs_d, double *x_d);
+/* z = M rhs with the preconditioner of a CG solver (host arrays) */
+int uhip_solver_precondition(uhip_solver s, const double *rhs, double *z);
 /* SolutionStatus (iterates, gradient_norm, reason) and solve wall time */
 int uhip_solver_get_status(uhip_solver s, int64_t *iterates, double *gradient_norm, int *reason, double *seconds);
 /* setup_seconds solve_seconds total_iterations refinements n_levels coarse_n eig_max_<l> eig_min_<l>

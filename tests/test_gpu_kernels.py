@@ -58,7 +58,12 @@ def test_spmv_fp64_bit_exact(ctx, name, block_nnz):
     D.spmv(xd, yd)
     y = yd.numpy()
     ref = orc.spmv(A, x)
-    assert np.array_equal(y, ref), np.abs(y - ref).max()
+    # rows that fit the block's LDS budget (1.25 * block_nnz) are summed sequentially: bit-exact;
+    # longer rows are reduced by the whole workgroup: rounding-level agreement
+    seq = np.diff(A.rowptr) <= block_nnz + block_nnz // 4
+    assert np.array_equal(y[seq], ref[seq]), np.abs(y - ref)[seq].max()
+    bound = 1e-14 * (np.abs(A.to_scipy()) @ np.abs(x))
+    assert np.all(np.abs(y - ref)[~seq] <= 20 * bound[~seq])
     blocks = D.blocks()
     assert blocks[0] == 0 and blocks[-1] == A.shape[0] and np.all(np.diff(blocks) > 0)
 
@@ -298,4 +303,4 @@ def test_graph_coloring_is_proper(ctx):
     rows = np.repeat(np.arange(n), np.diff(A.rowptr))
     off = rows != A.col
     assert not np.any(c[rows[off]] == c[A.col[off]])
-    assert k.value <= 12
+    assert k.value <= 9  # at most max degree + 1 for the 9-point stencil

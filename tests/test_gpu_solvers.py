@@ -1,0 +1,244 @@
+"""Solver-level parity on the GPU (through the solver C ABI) against the CPU
+oracle on the same seeded inputs.
+
+Bars (BASELINE workloads): iteration counts within +-1 of the oracle, FP64
+solution relative error <= 1e-10 for MG-PCG at rtol 1e-10 (1e-6 for the
+Jacobi-PCG run at rtol 1e-8, whose ~400 iterations amplify dot-product
+rounding), mixed precision within 1e-8 of the FP64 solution, QP active sets
+identical up to ties at 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import utopia_amd as ua
+from utopia_amd import petsc_io
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_jacobi_pcg_poisson_256(ctx):
+    """2D 5-point 256x256, Jacobi-PCG, rtol 1e-8 (atol = 1e-8 |b|), x0 = 0."""
+    A = ua.gen_stencil(ua.STENCIL_2D5, 256, 256)
+    b = ua.gen_uniform(1, A.shape[0])
+    atol = 1e-8 * np.linalg.norm(b)
+    cg = ua.ConjugateGradient(ctx, "jacobi", atol=atol, rtol=0, stol=0, max_it=5000)
+    x = cg.solve(A, b)
+    st = cg.status
+    xo, so = orc.cg_solve(A, b, precond=True, atol=atol, rtol=0, stol=0, max_it=5000)
+    print(f"jacobi-pcg 256^2: gpu it={st['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(st["iterates"] - so["iterates"]) <= 1
+    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol
+    assert rel(x, xo) <= 1e-6
+    assert st["reason"] == ua.CONVERGED_FNORM_ABS
+
+
+def test_unpreconditioned_cg(ctx):
+    A = ua.gen_stencil(ua.STENCIL_2D5, 40, 30)
+    b = ua.gen_uniform(2, A.shape[0])
+    atol = 1e-10 * np.linalg.norm(b)
+    cg = ua.ConjugateGradient(ctx, None, atol=atol, rtol=0, stol=0, max_it=2000)
+    x = cg.solve(A, b)
+    xo, so = orc.cg_solve(A, b, precond=False, atol=atol, rtol=0, stol=0, max_it=2000)
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-9
+
+
+def _mg_pcg(ctx, kind, n, levels, seed, pre=2, post=2, mixed=False, refine=None):
+    A = ua.gen_stencil(kind, n, n, n, seed=seed)
+    b = ua.gen_uniform(seed + 100, A.shape[0])
+    interps = ua.mg_hierarchy(n, n, n, levels)
+    atol = 1e-10 * np.linalg.norm(b)
+    ua.srand(1)
+    cg = ua.ConjugateGradient(ctx, "multigrid_mixed" if mixed else "multigrid", atol=atol, rtol=0, stol=0,
+                              max_it=500, pre_smoothing_steps=pre, post_smoothing_steps=post)
+    if refine is not None:
+        cg.set_param("refine", refine)
+    cg.set_transfer_operators(interps)
+    x = cg.solve(A, b)
+    ua.srand(1)
+    mg = orc.MG(interps, pre=pre, post=post)
+    mg.update(A)
+    xo, so = mg.cg_solve(b, atol=atol, max_it=500)
+    return A, b, atol, cg, x, mg, xo, so
+
+
+@pytest.mark.parametrize("kind,n,levels", [(ua.STENCIL_3D7, 32, 4), (ua.STENCIL_3D7_VARCOEF, 32, 4),
+                                           (ua.STENCIL_3D7, 24, 3)])
+def test_mg_pcg_fp64_matches_oracle(ctx, kind, n, levels):
+    A, b, atol, cg, x, mg, xo, so = _mg_pcg(ctx, kind, n, levels, seed=3)
+    st = cg.status
+    print(f"mg-pcg {n}^3 kind={kind}: gpu it={st['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(st["iterates"] - so["iterates"]) <= 1
+    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol
+    assert rel(x, xo) <= 1e-10
+
+
+def test_mg_hierarchy_masks_and_eigs_match_oracle(ctx):
+    n = 16
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=5)
+    interps = ua.mg_hierarchy(n, n, n, 3)
+    ua.srand(1)
+    s = ua.Multigrid(ctx, pre_smoothing_steps=2, post_smoothing_steps=2, max_it=1)
+    s.set_transfer_operators(interps)
+    s.update(A)
+    ua.srand(1)
+    mg = orc.MG(interps, pre=2, post=2)
+    mg.update(A)
+    for l in range(3):
+        G = s.level(l)
+        rp, col, val, shape = mg.level(l)
+        assert np.array_equal(G.rowptr, rp) and np.array_equal(G.col, col) and np.array_equal(G.val, val)
+        assert np.array_equal(s.mask(l), mg.mask(l))
+    for l in (1, 2):
+        emax, emin = mg.eigs(l)
+        assert abs(s.info(f"eig_max_{l}") - emax) <= 1e-10 * emax
+        assert abs(s.info(f"eig_min_{l}") - emin) <= 1e-10 * emin
+
+
+@pytest.mark.parametrize("pre,post,mg_type", [(2, 2, 1), (3, 3, 1), (1, 2, 2)])
+def test_mg_cycles_match_oracle_with_pinned_eigs(ctx, pre, post, mg_type):
+    n = 16
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=6)
+    b = ua.gen_uniform(7, A.shape[0])
+    interps = ua.mg_hierarchy(n, n, n, 3)
+    ua.srand(1)
+    s = ua.Multigrid(ctx, pre_smoothing_steps=pre, post_smoothing_steps=post, mg_type=mg_type, max_it=3,
+                     atol=0, rtol=0, stol=0)
+    s.set_transfer_operators(interps)
+    s.update(A)
+    ua.srand(1)
+    mg = orc.MG(interps, pre=pre, post=post, mg_type=mg_type)
+    mg.update(A)
+    for l in (1, 2):
+        emax, emin = mg.eigs(l)
+        s.set_level_eigs(l, emax, emin)
+    x = s.apply(b)
+    xo = mg.apply(b, max_it=3, atol=0, rtol=0)
+    # identical cycles up to the coarse solve (explicit inverse vs triangular solves)
+    assert rel(x, xo) <= 1e-12
+
+
+def test_cg_mg_reference_fixture(ctx, golden):
+    """PetscLinearSolverTest::petsc_cg_mg on utopia/data/laplace/matrices_for_petsc."""
+    d = golden / "utopia_data"
+    A = petsc_io.read_mat(d / "f_A")
+    rhs = petsc_io.read_vec(d / "f_rhs")
+    Is = [petsc_io.read_mat(d / f) for f in ("I_1", "I_2", "I_3")]
+    ua.srand(1)
+    cg = ua.ConjugateGradient(ctx, "multigrid", atol=1e-15, rtol=1e-18, stol=1e-18, max_it=200,
+                              fix_semidefinite_operators=1, must_generate_masks=1)
+    cg.set_transfer_operators(Is)
+    x = cg.solve(A, rhs, rhs)
+    assert np.abs(A.to_scipy() @ x - rhs).max() < 1e-6
+    ua.srand(1)
+    mg = orc.MG(Is, masks=True, fix_semidefinite=True)
+    mg.update(A)
+    xo, so = mg.cg_solve(rhs, x0=rhs, atol=1e-15, max_it=200)
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+
+
+def test_mixed_precision_mg_pcg(ctx):
+    n = 32
+    A, b, atol, cg, x, mg, xo, so = _mg_pcg(ctx, ua.STENCIL_3D7_VARCOEF, n, 4, seed=9, mixed=True)
+    st = cg.status
+    err = rel(x, xo)
+    total = cg.info("total_iterations")
+    print(f"mixed mg-pcg {n}^3: iterations {st['iterates']} (total {total}, refinements "
+          f"{cg.info('refinements')}), fp64 oracle {so['iterates']}, rel.err vs fp64 {err:.2e}")
+    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol
+    assert err <= 1e-8
+    assert total <= so["iterates"] + 1 + 5
+
+
+def test_mixed_preconditioner_is_fp32_accurate(ctx):
+    """The FP32 V-cycle agrees with the FP64 one to single precision, not better."""
+    n = 32
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=9)
+    r = ua.gen_uniform(77, A.shape[0])
+    out = {}
+    for kind in ("multigrid", "multigrid_mixed"):
+        ua.srand(1)
+        cg = ua.ConjugateGradient(ctx, kind, pre_smoothing_steps=2, post_smoothing_steps=2)
+        cg.set_transfer_operators(ua.mg_hierarchy(n, n, n, 4))
+        cg.update(A)
+        out[kind] = cg.precondition(r)
+    d = rel(out["multigrid_mixed"], out["multigrid"])
+    print(f"mixed vs fp64 V-cycle: rel. difference {d:.2e}")
+    assert 1e-9 < d < 1e-5
+
+
+def _qp(n, seed):
+    A = ua.gen_stencil(ua.STENCIL_2D9, n, n, shift=1e-3)
+    b = ua.gen_uniform(seed, n * n)
+    lb = -0.05 - 0.15 * ua.gen_uniform(seed + 1, n * n, 0, 1)
+    return A, b, lb
+
+
+def _kkt(A, b, x, lb, tol):
+    g = A.to_scipy() @ x - b
+    act = np.abs(x - lb) < 1e-12
+    assert np.all(x >= lb)
+    assert np.all(g[act] >= -tol)
+    assert np.abs(g[~act]).max() <= tol
+    return act
+
+
+def test_mprgp_matches_oracle(ctx):
+    A, b, lb = _qp(64, 11)
+    q = ua.MPRGP(ctx, atol=1e-9, rtol=0, max_it=100000)
+    q.update(A)
+    q.set_box_constraints(lb, None)
+    x = q.apply(b)
+    xo, so = orc.mprgp_solve(A, b, lb=lb, atol=1e-9, rtol=0, max_it=100000)
+    st = q.status
+    print(f"mprgp 64^2: gpu it={st['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    act = _kkt(A, b, x, lb, 1e-8)
+    act_o = np.abs(xo - lb) < 1e-12
+    ties = np.abs(x - lb) < 1e-9
+    assert np.all((act == act_o) | ties)
+    assert rel(x, xo) <= 1e-8
+
+
+def test_pgs_converges_to_the_qp_solution(ctx):
+    A, b, lb = _qp(24, 21)
+    q = ua.ProjectedGaussSeidel(ctx, stol=1e-13, max_it=20000)
+    q.update(A)
+    q.set_box_constraints(lb, None)
+    x = q.apply(b)
+    xo, _ = orc.mprgp_solve(A, b, lb=lb, atol=1e-12, rtol=0, max_it=100000)
+    assert 2 <= q.info("ncolors") <= 9
+    assert rel(x, xo) <= 1e-9
+    _kkt(A, b, x, lb, 1e-8)
+
+
+def test_pgs_unconstrained_is_symmetric_gauss_seidel(ctx):
+    A = ua.gen_stencil(ua.STENCIL_2D5, 20, 20)
+    b = ua.gen_uniform(4, 400)
+    q = ua.ProjectedGaussSeidel(ctx, stol=1e-14, max_it=50000)
+    q.update(A)
+    x = q.apply(b)
+    import scipy.sparse.linalg as spla
+    xs = spla.spsolve(A.to_scipy().tocsc(), b)
+    assert rel(x, xs) <= 1e-10
+
+
+def test_error_paths_report_status(ctx):
+    # indefinite coarse operator -> Cholesky failure reported as an error code, no crash
+    A = ua.gen_stencil(ua.STENCIL_3D7, 8, 8, 8)
+    A = ua.Csr(A.rowptr, A.col, -A.val, A.shape)
+    s = ua.Multigrid(ctx, max_it=1)
+    s.set_transfer_operators(ua.mg_hierarchy(8, 8, 8, 2))
+    with pytest.raises(ua.HipError):
+        s.update(A)
+    # NaN right-hand side -> CG stops on the NaN step length (utopia_ConjugateGradient_impl.hpp:241-244)
+    A = ua.gen_stencil(ua.STENCIL_2D5, 10, 10)
+    b = ua.gen_uniform(1, 100)
+    b[5] = np.nan
+    cg = ua.ConjugateGradient(ctx, "jacobi", atol=1e-10, max_it=50)
+    cg.solve(A, b)
+    assert cg.status["iterates"] == 0

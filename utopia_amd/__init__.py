@@ -340,6 +340,13 @@ class Solver:
     def apply_device(self, b: DeviceArray, x: DeviceArray):
         check(lib().uhip_solver_apply_device(self.h, ctypes.c_void_p(b.ptr), ctypes.c_void_p(x.ptr)), "apply_device")
 
+    def precondition(self, r: np.ndarray) -> np.ndarray:
+        """z = M r with this CG solver's preconditioner."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.zeros_like(r)
+        check(lib().uhip_solver_precondition(self.h, _ptr(r), _ptr(z)), "precondition")
+        return z
+
     def solve(self, A: Csr, b: np.ndarray, x: Optional[np.ndarray] = None) -> np.ndarray:
         self.update(A)
         return self.apply(b, x)

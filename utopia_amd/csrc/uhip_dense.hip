@@ -12,6 +12,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 // C(i,j) = beta*C(i,j) + alpha * sum_k A(i,k) * B(j,k) for a 64x64 tile of C per
 // workgroup (4 waves, each a 32x32 quarter = 2x2 MFMA 16x16 tiles).
+// Operand lanes of v_mfma_f64_16x16x4f64: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16] (k = l/16).
 // Element (i,k) of A is A[i*sai + k*sak], of B is B[j*sbj + k*sbk], C is row-major ldc.
 // lower_only: skip tiles strictly above the diagonal.
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, double alpha, const double *__restrict__ A,
@@ -58,7 +60,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, double al
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gi = ti + wi + a * 16 + 4 * (lane >> 4) + r;
+                // v_mfma_f64_16x16x4f64 result: item r of lane l is D[4*r + l/16][l%16]
+                const int gi = ti + wi + a * 16 + 4 * r + (lane >> 4);
                 const int gj = tj + wj + b * 16 + (lane & 15);
                 if (gi < M && gj < N && (!lower_only || gj <= gi)) {
                     double *c = C + (int64_t)gi * ldc + gj;

@@ -24,6 +24,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
                                                       double *__restrict__ s, int i_num, int i_den, int i_alpha,
                                                       double *__restrict__ partials) {
     const double alpha = s[i_num] / s[i_den];
+    if (!isfinite(alpha)) {  // CG stops before touching x (utopia_ConjugateGradient_impl.hpp:241-244)
+        if (blockIdx.x == 0 && threadIdx.x == 0) s[i_alpha] = alpha;
+        return;
+    }
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         x[i] = x[i] + alpha * p[i];

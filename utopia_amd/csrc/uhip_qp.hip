@@ -166,11 +166,13 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// Jones-Plassmann round: an uncolored row whose (hash, index) beats every
-// uncolored neighbour takes colour k. Needs a structurally symmetric pattern.
+// Jones-Plassmann round: an uncoloured row whose (hash, index) beats every
+// uncoloured neighbour takes the smallest colour none of its coloured
+// neighbours has. Rows chosen in one round are never adjacent, so they cannot
+// conflict. Needs a structurally symmetric pattern.
 __global__ __launch_bounds__(kBlock) void k_jp_round(int32_t n, const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col, int32_t *__restrict__ color,
-                                                     int32_t k, uint64_t seed, unsigned int *__restrict__ remaining) {
+                                                     uint64_t seed, unsigned int *__restrict__ remaining) {
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n || color[i] >= 0) return;
     const uint64_t hi = mix64((uint64_t)i ^ seed);
@@ -178,23 +180,32 @@ __global__ __launch_bounds__(kBlock) void k_jp_round(int32_t n, const int32_t *_
     for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
         const int32_t j = col[e];
         if (j == i) continue;
-        const int32_t cj = color[j];
-        if (cj >= 0 && cj < k) continue;  // coloured in an earlier round
+        if (color[j] >= 0) continue;  // coloured in an earlier round
         const uint64_t hj = mix64((uint64_t)j ^ seed);
         if (hj > hi || (hj == hi && j > i)) {
             best = false;
             break;
         }
     }
-    // Colours of this round are written after every row has decided: the
-    // caller flips pending (-2 - k) into k with k_jp_commit.
-    if (best) color[i] = -2 - k;
-    else atomicAdd(remaining, 1u);
+    if (!best) {
+        atomicAdd(remaining, 1u);
+        return;
+    }
+    // smallest colour not used by a coloured neighbour (neighbours chosen in
+    // this round are marked pending, <= -2, and are never adjacent to i)
+    int32_t c = 0;
+    for (;;) {
+        bool used = false;
+        for (int32_t e = rowptr[i]; e < rowptr[i + 1] && !used; ++e) used = (col[e] != i) && (color[col[e]] == c);
+        if (!used) break;
+        ++c;
+    }
+    color[i] = -2 - c;  // pending until k_jp_commit, after every row of the round has decided
 }
 
-__global__ __launch_bounds__(kBlock) void k_jp_commit(int32_t n, int32_t *__restrict__ color, int32_t k) {
+__global__ __launch_bounds__(kBlock) void k_jp_commit(int32_t n, int32_t *__restrict__ color) {
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n && color[i] == -2 - k) color[i] = k;
+    if (i < n && color[i] <= -2) color[i] = -2 - color[i];
 }
 
 // One colour of the projected sweep (ProjectedScalarGaussSeidelSweep::apply,
@@ -343,17 +354,30 @@ int uhip_graph_color(uhip_ctx ctx, const uhip_csr *A, uint64_t seed, int32_t *co
     UHIP_CHECK_HIP(hipMemsetAsync(color, 0xff, sizeof(int32_t) * (size_t)n, ctx->stream));  // -1
     unsigned int *rem = ctx->counters + 1;
     dim3 g((n + kBlock - 1) / kBlock);
-    for (int32_t k = 0; k < n; ++k) {
+    // every round colours at least the uncoloured row of largest (hash, index)
+    for (int32_t round = 0; round < n; ++round) {
         UHIP_CHECK_HIP(hipMemsetAsync(rem, 0, sizeof(unsigned int), ctx->stream));
-        hipLaunchKernelGGL(k_jp_round, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col, color, k, seed, rem);
+        hipLaunchKernelGGL(k_jp_round, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col, color, seed, rem);
         UHIP_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_jp_commit, g, dim3(kBlock), 0, ctx->stream, n, color, k);
+        hipLaunchKernelGGL(k_jp_commit, g, dim3(kBlock), 0, ctx->stream, n, color);
         UHIP_CHECK_LAUNCH();
         unsigned int h = 0;
         UHIP_CHECK_HIP(hipMemcpyAsync(&h, rem, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
         UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
         if (h == 0) {
-            *ncolors = k + 1;
+            int32_t *mx = nullptr;
+            void *tmp = nullptr;
+            size_t tb = 0;
+            UHIP_CHECK_HIP(hipMalloc(&mx, sizeof(int32_t)));
+            UHIP_CHECK_HIP(hipcub::DeviceReduce::Max(nullptr, tb, color, mx, n, ctx->stream));
+            UHIP_CHECK_HIP(hipMalloc(&tmp, tb));
+            UHIP_CHECK_HIP(hipcub::DeviceReduce::Max(tmp, tb, color, mx, n, ctx->stream));
+            int32_t m = 0;
+            UHIP_CHECK_HIP(hipMemcpyAsync(&m, mx, sizeof(m), hipMemcpyDeviceToHost, ctx->stream));
+            UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+            hipFree(mx);
+            hipFree(tmp);
+            *ncolors = m + 1;
             return UHIP_OK;
         }
     }

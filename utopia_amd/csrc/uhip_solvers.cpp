@@ -293,6 +293,21 @@ int uhip_solver_apply_device(uhip_solver s, const double *rhs_d, double *x_d) {
     }
 }
 
+// z = M r with the preconditioner of a CG solver (z starts at zero, as in the CG loop).
+int uhip_solver_precondition(uhip_solver s, const double *rhs, double *z) {
+    if (!s || !s->cg || !s->cg->get_preconditioner() || !rhs || !z) return UHIP_ERR_INVALID;
+    try {
+        const int64_t n = s->A->rows();
+        HipVector b(s->ctx, n), zv(s->ctx, n, 0.0);
+        b.upload(rhs);
+        s->cg->get_preconditioner()->apply_zero_initial_guess(b, zv);
+        zv.download(z);
+        return UHIP_OK;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 int uhip_solver_get_status(uhip_solver s, int64_t *iterates, double *gradient_norm, int *reason, double *seconds) {
     if (!s) return UHIP_ERR_INVALID;
     if (iterates) *iterates = s->status.iterates;
@@ -311,6 +326,8 @@ int uhip_solver_get_info(uhip_solver s, const char *key, double *value) {
     else if (k == "refinements") *value = (double)s->refinements;
     else if (k == "n_levels") *value = s->mg ? s->mg->n_levels() : 0;
     else if (k == "coarse_n") *value = s->mg ? s->mg->level_operator(0).rows() : 0;
+    else if (k == "ncolors" && s->pgs) *value = s->pgs->ncolors();
+    else if (k == "alpha_bar" && s->mprgp) *value = s->mprgp->alpha_bar();
     else if (k.rfind("eig_max_", 0) == 0 && s->mg) *value = s->mg->level_eig_max(std::stoi(k.substr(8)));
     else if (k.rfind("eig_min_", 0) == 0 && s->mg) *value = s->mg->level_eig_min(std::stoi(k.substr(8)));
     else if (k.rfind("level_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(10))).nnz();
