@@ -47,6 +47,8 @@ extern "C" {
 typedef struct uhip_context_s *uhip_ctx;
 
 const char *uhip_last_error(void);
+/* sizeof of the ABI structs (0: uhip_csr, 1: uhip_cheb_step) for binding checks */
+int64_t uhip_abi_sizeof(int which);
 int uhip_device_count(int *count);
 /* stream == NULL: the context creates its own (default-priority) stream. */
 int uhip_ctx_create(int device, void *stream, uhip_ctx *out);
@@ -138,15 +140,32 @@ typedef struct uhip_csr {
     int32_t *col;
     void *val;
     int32_t dtype;      /* UHIP_F64 or UHIP_F32 */
+    /* ---- SpMV plan (made by uhip_csr_plan*, owned by it) ---- */
+    int32_t format;     /* UHIP_FMT_ROWBLOCK or UHIP_FMT_SELL64 */
     int32_t *blocks;    /* row-block plan, device, nblocks+1 entries */
     int32_t nblocks;
     int32_t block_capk; /* LDS budget: 256*block_capk products per block */
+    int32_t nslices;    /* SELL-64: slices of 64 consecutive rows */
+    int32_t *slice_off; /* SELL-64: nslices+1 offsets (entries) */
+    int32_t *sell_col;  /* SELL-64: column-major within a slice, padded with zeros */
+    void *sell_val;
+    int32_t nparts;     /* workgroups of one SpMV launch (partial sums of fused dots) */
 } uhip_csr;
 
+#define UHIP_FMT_AUTO (-1)
+#define UHIP_FMT_ROWBLOCK 0
+#define UHIP_FMT_SELL64 1
+
 /* Build the row-block plan. block_nnz selects the LDS budget (1024, 2048 or 4096
- * nonzeros per block; 0 = 2048). Rows longer than block_nnz/4 get a block of
- * their own. Allocates A->blocks. */
+ * nonzeros per block; 0 = automatic format choice, see uhip_csr_plan_format).
+ * Rows longer than block_nnz/4 get a block of their own. */
 int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
+/* Plan with an explicit format. UHIP_FMT_SELL64 copies the matrix into slices
+ * of 64 rows stored column-major (one wavefront per slice, every load
+ * coalesced); UHIP_FMT_AUTO takes SELL-64 when its padding adds at most 25%
+ * to nnz and the row-block CSR kernel otherwise. Each y_i is summed in column
+ * order in either format. */
+int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz);
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A);
 
 /* y = A x                         MatMult (utopia_petsc_Matrix.cpp:1326-1358) */

@@ -23,6 +23,12 @@ def test_library_exports_every_declared_symbol():
         assert getattr(lib, name) is not None
 
 
+def test_ctypes_structs_match_c_layout():
+    import ctypes
+    assert ua.lib().uhip_abi_sizeof(0) == ctypes.sizeof(ua.uhip_csr)
+    assert ua.lib().uhip_abi_sizeof(1) == ctypes.sizeof(ua.uhip_cheb_step)
+
+
 def test_library_carries_gfx950_code_object():
     # the offload bundle id of our own build names its target
     data = _abi.LIB_PATH.read_bytes()

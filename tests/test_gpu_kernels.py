@@ -69,14 +69,52 @@ def test_spmv_fp64_bit_exact(ctx, name, block_nnz):
 
 
 @pytest.mark.parametrize("name", sorted(MATRICES))
-def test_spmv_fp32_bit_exact(ctx, name):
+@pytest.mark.parametrize("fmt", [ua.FMT_ROWBLOCK, ua.FMT_SELL64])
+def test_spmv_fp32_bit_exact(ctx, name, fmt):
     A = MATRICES[name]()
     x = ua.gen_uniform(12, A.shape[1]).astype(np.float32)
-    D = ua.DeviceCsr(ctx, A, np.float32)
+    D = ua.DeviceCsr(ctx, A, np.float32, fmt=fmt)
+    assert D.format == fmt
     xd = ua.DeviceArray.from_numpy(ctx, x)
     yd = ua.DeviceArray(ctx, A.shape[0], np.float32)
     D.spmv(xd, yd)
     assert np.array_equal(yd.numpy(), orc.spmv_f32(A, x))
+
+
+@pytest.mark.parametrize("name", sorted(MATRICES))
+def test_spmv_sell64_fp64_bit_exact(ctx, name):
+    A = MATRICES[name]()
+    x = ua.gen_uniform(14, A.shape[1])
+    D = ua.DeviceCsr(ctx, A, np.float64, fmt=ua.FMT_SELL64)
+    assert D.format == ua.FMT_SELL64
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    yd = ua.DeviceArray(ctx, A.shape[0])
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv(A, x))
+
+
+def test_auto_format_choice(ctx):
+    # regular stencils take SELL-64; a matrix with one long row per slice keeps row-block CSR
+    assert ua.DeviceCsr(ctx, ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 16, 16, seed=1)).format == ua.FMT_SELL64
+    assert ua.DeviceCsr(ctx, MATRICES["rows_up_to_900"]()).format == ua.FMT_ROWBLOCK
+    assert ua.DeviceCsr(ctx, ua.gen_prolongation(16, 16, 16)).format == ua.FMT_ROWBLOCK
+
+
+@pytest.mark.parametrize("fmt", [ua.FMT_ROWBLOCK, ua.FMT_SELL64])
+def test_fused_epilogues_both_formats_with_dot(ctx, fmt):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 20, 18, 13, seed=9)
+    n = A.shape[0]
+    x, b = ua.gen_uniform(1, n), ua.gen_uniform(2, n)
+    Ax = orc.spmv(A, x)
+    D = ua.DeviceCsr(ctx, A, fmt=fmt)
+    xd, bd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b)
+    out = ua.DeviceArray(ctx, n)
+    D.residual(xd, bd, out)
+    assert np.array_equal(out.numpy(), b - Ax)
+    s = ctx.scalars
+    ua.check(L().uhip_spmv_dot(ctx.h, D.ref, P(xd), P(out), ctypes.c_void_p(s)))
+    assert np.array_equal(out.numpy(), Ax)
+    assert abs(ctx.read_scalars(s, 1)[0] - np.dot(x, Ax)) <= 1e-13 * np.dot(np.abs(x), np.abs(Ax))
 
 
 def test_spmv_long_rows_take_block_reduction(ctx):

@@ -22,6 +22,7 @@ import numpy as np
 from ._abi import LibraryMissing, lib, uhip_cheb_step, uhip_csr  # noqa: F401
 
 F64, F32 = 0, 1
+FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64 = -1, 0, 1
 _NP = {F64: np.float64, F32: np.float32}
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
@@ -210,7 +211,8 @@ class DeviceArray:
 class DeviceCsr:
     """uhip_csr on the device with its row-block plan."""
 
-    def __init__(self, ctx: Context, A: Csr, dtype=np.float64, block_nnz: int = 0):
+    def __init__(self, ctx: Context, A: Csr, dtype=np.float64, block_nnz: int = 0, fmt: Optional[int] = None):
+        """fmt: None (block_nnz > 0: row-block CSR, 0: automatic), or FMT_AUTO / FMT_ROWBLOCK / FMT_SELL64."""
         self.ctx = ctx
         self.shape = A.shape
         self._rp = DeviceArray.from_numpy(ctx, A.rowptr.astype(np.int32))
@@ -221,7 +223,14 @@ class DeviceCsr:
         m.rowptr, m.col, m.val = self._rp.ptr, self._col.ptr, self._val.ptr
         m.dtype = _DT[np.dtype(dtype)]
         self.m = m
-        check(lib().uhip_csr_plan(ctx.h, ctypes.byref(m), block_nnz), "uhip_csr_plan")
+        if fmt is None:
+            check(lib().uhip_csr_plan(ctx.h, ctypes.byref(m), block_nnz), "uhip_csr_plan")
+        else:
+            check(lib().uhip_csr_plan_format(ctx.h, ctypes.byref(m), fmt, block_nnz), "uhip_csr_plan_format")
+
+    @property
+    def format(self) -> int:
+        return self.m.format
 
     @property
     def ref(self):

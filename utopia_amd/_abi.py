@@ -28,9 +28,15 @@ class uhip_csr(ctypes.Structure):
         ("col", ctypes.c_void_p),
         ("val", ctypes.c_void_p),
         ("dtype", ctypes.c_int32),
+        ("format", ctypes.c_int32),
         ("blocks", ctypes.c_void_p),
         ("nblocks", ctypes.c_int32),
         ("block_capk", ctypes.c_int32),
+        ("nslices", ctypes.c_int32),
+        ("slice_off", ctypes.c_void_p),
+        ("sell_col", ctypes.c_void_p),
+        ("sell_val", ctypes.c_void_p),
+        ("nparts", ctypes.c_int32),
     ]
 
 

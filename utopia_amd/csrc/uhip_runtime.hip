@@ -62,6 +62,12 @@ extern "C" {
 
 const char *uhip_last_error(void) { return g_last_error.c_str(); }
 
+int64_t uhip_abi_sizeof(int which) {
+    if (which == 0) return (int64_t)sizeof(uhip_csr);
+    if (which == 1) return (int64_t)sizeof(uhip_cheb_step);
+    return -1;
+}
+
 int uhip_device_count(int *count) {
     UHIP_REQUIRE(count, "uhip_device_count: null output");
     UHIP_CHECK_HIP(hipGetDeviceCount(count));

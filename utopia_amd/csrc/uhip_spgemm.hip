@@ -221,13 +221,10 @@ int uhip_csr_free(uhip_ctx ctx, uhip_csr *A) {
     if (A->rowptr) hipFree(A->rowptr);
     if (A->col) hipFree(A->col);
     if (A->val) hipFree(A->val);
-    if (A->blocks) hipFree(A->blocks);
     A->rowptr = nullptr;
     A->col = nullptr;
     A->val = nullptr;
-    A->blocks = nullptr;
-    A->nblocks = 0;
-    return UHIP_OK;
+    return uhip_csr_plan_destroy(ctx, A);
 }
 
 int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {

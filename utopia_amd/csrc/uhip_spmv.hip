@@ -29,6 +29,81 @@ __global__ void k_plan_tail(int32_t *__restrict__ blocks, const int32_t *__restr
     blocks[*count] = n;
 }
 
+// entries of slice s = 64 * (longest row of the slice)
+__global__ __launch_bounds__(kBlock) void k_slice_width(int32_t n, int32_t nslices, const int32_t *__restrict__ rowptr,
+                                                        int32_t *__restrict__ width) {
+    const int32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s > nslices) return;
+    if (s == nslices) {
+        width[s] = 0;
+        return;
+    }
+    int32_t w = 0;
+    const int32_t r1 = min(n, (s + 1) * 64);
+    for (int32_t r = s * 64; r < r1; ++r) w = max(w, rowptr[r + 1] - rowptr[r]);
+    width[s] = 64 * w;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int32_t n, const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                      const int32_t *__restrict__ slice_off,
+                                                      int32_t *__restrict__ scol, T *__restrict__ sval) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int32_t s = r >> 6, lane = r & 63;
+    const int32_t o = slice_off[s], w = (slice_off[s + 1] - o) >> 6;
+    const int32_t a = rowptr[r], len = rowptr[r + 1] - a;
+    for (int32_t j = 0; j < w; ++j) {
+        const int64_t d = (int64_t)o + 64 * j + lane;
+        scol[d] = j < len ? col[a + j] : 0;
+        sval[d] = j < len ? val[a + j] : T(0);
+    }
+}
+
+// Plan SELL-64 arrays; returns UHIP_ERR_UNSUPPORTED (nothing allocated) when the
+// padding exceeds max_ratio * nnz.
+static int plan_sell(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
+    hipStream_t s = ctx->stream;
+    const int32_t n = A->nrows;
+    const int32_t ns = (n + 63) / 64;
+    int32_t *w = nullptr, *off = nullptr;
+    UHIP_CHECK_HIP(hipMalloc(&w, sizeof(int32_t) * ((size_t)ns + 1)));
+    UHIP_CHECK_HIP(hipMalloc(&off, sizeof(int32_t) * ((size_t)ns + 1)));
+    hipLaunchKernelGGL(k_slice_width, dim3(ns / kBlock + 1), dim3(kBlock), 0, s, n, ns, A->rowptr, w);
+    UHIP_CHECK_LAUNCH();
+    void *tmp = nullptr;
+    size_t tb = 0;
+    UHIP_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w, off, ns + 1, s));
+    UHIP_CHECK_HIP(hipMalloc(&tmp, tb));
+    UHIP_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, w, off, ns + 1, s));
+    int32_t total = 0;
+    UHIP_CHECK_HIP(hipMemcpyAsync(&total, off + ns, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    hipFree(tmp);
+    hipFree(w);
+    if ((double)total > max_ratio * (double)A->nnz || total < 0) {
+        hipFree(off);
+        return UHIP_ERR_UNSUPPORTED;
+    }
+    const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
+    UHIP_CHECK_HIP(hipMalloc(&A->sell_col, sizeof(int32_t) * (size_t)std::max(total, 1)));
+    UHIP_CHECK_HIP(hipMalloc(&A->sell_val, vb * (size_t)std::max(total, 1)));
+    if (A->dtype == UHIP_F64)
+        hipLaunchKernelGGL(k_sell_fill<double>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr,
+                           A->col, (const double *)A->val, off, A->sell_col, (double *)A->sell_val);
+    else
+        hipLaunchKernelGGL(k_sell_fill<float>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr,
+                           A->col, (const float *)A->val, off, A->sell_col, (float *)A->sell_val);
+    UHIP_CHECK_LAUNCH();
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    A->slice_off = off;
+    A->nslices = ns;
+    A->format = UHIP_FMT_SELL64;
+    A->nparts = (ns + kSellWaves - 1) / kSellWaves;
+    return UHIP_OK;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ col, const T *__restrict__ val,
@@ -96,20 +171,33 @@ using namespace uhip;
 extern "C" {
 
 int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz) {
+    return uhip_csr_plan_format(ctx, A, block_nnz == 0 ? UHIP_FMT_AUTO : UHIP_FMT_ROWBLOCK, block_nnz);
+}
+
+int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz) {
     UHIP_REQUIRE(ctx && A && A->rowptr, "uhip_csr_plan: null argument");
     UHIP_REQUIRE(A->nrows >= 0 && A->ncols >= 0, "uhip_csr_plan: bad shape");
+    UHIP_REQUIRE(format == UHIP_FMT_AUTO || format == UHIP_FMT_ROWBLOCK || format == UHIP_FMT_SELL64,
+                 "uhip_csr_plan: format %d", format);
     if (block_nnz == 0) block_nnz = 2048;
     int capk;
     if (block_nnz == 1024) capk = 5;
     else if (block_nnz == 2048) capk = 10;
     else if (block_nnz == 4096) capk = 20;
     else return set_error(UHIP_ERR_INVALID, "uhip_csr_plan: block_nnz %d not in {1024,2048,4096}", block_nnz);
-    if (A->blocks) UHIP_TRY(uhip_csr_plan_destroy(ctx, A));
+    UHIP_TRY(uhip_csr_plan_destroy(ctx, A));
     const int32_t n = A->nrows;
     hipStream_t s = ctx->stream;
     A->block_capk = capk;
+    A->format = UHIP_FMT_ROWBLOCK;
+    if (n > 0 && A->nnz > 0 && format != UHIP_FMT_ROWBLOCK) {
+        const int st = plan_sell(ctx, A, format == UHIP_FMT_SELL64 ? 1e30 : 1.25);
+        if (st == UHIP_OK) return UHIP_OK;
+        if (st != UHIP_ERR_UNSUPPORTED) return st;
+    }
     if (n == 0) {
         A->nblocks = 0;
+        A->nparts = 0;
         UHIP_CHECK_HIP(hipMalloc(&A->blocks, sizeof(int32_t)));
         UHIP_CHECK_HIP(hipMemsetAsync(A->blocks, 0, sizeof(int32_t), s));
         return UHIP_OK;
@@ -141,17 +229,27 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz) {
     hipFree(count);
     hipFree(tmp);
     A->nblocks = nb;
+    A->nparts = nb;
     return UHIP_OK;
 }
 
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     UHIP_REQUIRE(A, "uhip_csr_plan_destroy: null matrix");
-    if (A->blocks) {
+    if (A->blocks || A->slice_off || A->sell_col || A->sell_val) {
         if (ctx) UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-        UHIP_CHECK_HIP(hipFree(A->blocks));
     }
+    if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
+    if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
+    if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));
+    if (A->sell_val) UHIP_CHECK_HIP(hipFree(A->sell_val));
     A->blocks = nullptr;
     A->nblocks = 0;
+    A->slice_off = nullptr;
+    A->sell_col = nullptr;
+    A->sell_val = nullptr;
+    A->nslices = 0;
+    A->nparts = 0;
+    A->format = UHIP_FMT_ROWBLOCK;
     return UHIP_OK;
 }
 
@@ -208,7 +306,7 @@ int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, doubl
         EpiStoreDot<float, float, float> e{(const float *)x, (float *)y};
         UHIP_TRY(launch_rowblock<float>(ctx, A, (const float *)x, e));
     }
-    return finish_sums(ctx, 1, A->nblocks, ctx->max_partials, dot_d);
+    return finish_sums(ctx, 1, A->nparts, ctx->max_partials, dot_d);
 }
 
 int uhip_spmv_masked(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *mask, void *y) {

@@ -81,11 +81,85 @@ __global__ __launch_bounds__(kSpmvWG) void k_spmv_rowblock(const int32_t *__rest
     }
 }
 
-// Dispatch over the LDS budget (block_nnz = 4/5 * CAPK * WG). Matrix values
-// have the accumulation type TAcc.
+// SELL-64: one wavefront per slice of 64 consecutive rows, lane = row. Entry j
+// of the slice's rows sits at slice_off[s] + 64*j + lane, so every val/col
+// load of the wave is one contiguous 256/512-byte segment. A lane stops at its
+// own row length (from rowptr), so padding is never read; lanes sum their row
+// in column order.
+constexpr int kSellWaves = 4;  // slices per workgroup
+
+template <typename TAcc, class Epi, typename TA, typename TX>
+__global__ __launch_bounds__(64 * kSellWaves) void k_spmv_sell(int32_t nrows, int32_t nslices,
+                                                              const int32_t *__restrict__ rowptr,
+                                                              const int32_t *__restrict__ slice_off,
+                                                              const int32_t *__restrict__ col,
+                                                              const TA *__restrict__ val, const TX *__restrict__ x,
+                                                              Epi epi, double *__restrict__ partials, int64_t stride) {
+    constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int s = b * kSellWaves + (threadIdx.x >> 6);
+    double part[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) part[d] = 0.0;
+    const int32_t row = s * 64 + lane;
+    if (s < nslices && row < nrows) {
+        const int32_t o0 = slice_off[s];
+        const int32_t len = rowptr[row + 1] - rowptr[row];
+        const int32_t *c = col + o0 + lane;
+        const TA *v = val + o0 + lane;
+        TAcc acc = TAcc(0);
+        int j = 0;
+        for (; j + 8 <= len; j += 8) {
+            int ci[8];
+            TA vi[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ci[u] = ld_stream(c + (j + u) * 64);
+                vi[u] = ld_stream(v + (j + u) * 64);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += (TAcc)vi[u] * (TAcc)x[ci[u]];
+        }
+        if (j < len) {
+            int ci[8];
+            TA vi[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j + u < len) {
+                    ci[u] = ld_stream(c + (j + u) * 64);
+                    vi[u] = ld_stream(v + (j + u) * 64);
+                }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j + u < len) acc += (TAcc)vi[u] * (TAcc)x[ci[u]];
+        }
+        epi(row, acc, part);
+    }
+    if constexpr (Epi::kDots > 0) {
+#pragma unroll
+        for (int d = 0; d < Epi::kDots; ++d) {
+            const double t = block_sum<64 * kSellWaves>(part[d]);
+            if (threadIdx.x == 0) partials[d * stride + b] = t;
+        }
+    }
+}
+
+// Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
+// Matrix values have the accumulation type TAcc.
 template <typename TAcc, typename TX, class Epi>
 int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi) {
     using TA = TAcc;
+    if (A && A->format == UHIP_FMT_SELL64) {
+        UHIP_REQUIRE(A->slice_off && A->sell_col && A->sell_val, "spmv: SELL-64 plan missing");
+        if (A->nslices == 0) return UHIP_OK;
+        if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
+        hipLaunchKernelGGL((k_spmv_sell<TAcc, Epi, TA, TX>), dim3(A->nparts), dim3(64 * kSellWaves), 0,
+                           stream_of(ctx), A->nrows, A->nslices, A->rowptr, A->slice_off, A->sell_col,
+                           (const TA *)A->sell_val, x, epi, ctx->partials, ctx->max_partials);
+        UHIP_CHECK_LAUNCH();
+        return UHIP_OK;
+    }
     UHIP_REQUIRE(A && A->blocks && A->nblocks >= 0, "spmv: matrix has no row-block plan");
     if (A->nblocks == 0) return UHIP_OK;
     if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nblocks));
