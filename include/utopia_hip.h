@@ -206,6 +206,10 @@ typedef struct uhip_cheb_step {
     void *pout;
     double a_prev, beta, a_cur;
     int32_t last;
+    /* optional (last step only): dot_d[0] = xout . dot_with (FP64 vector), the
+     * dot(z_new, r_new) of ConjugateGradient (utopia_ConjugateGradient_impl.hpp:270) */
+    const double *dot_with;
+    double *dot_d;
 } uhip_cheb_step;
 int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *step);
 

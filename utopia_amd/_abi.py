@@ -56,6 +56,8 @@ class uhip_cheb_step(ctypes.Structure):
         ("beta", ctypes.c_double),
         ("a_cur", ctypes.c_double),
         ("last", ctypes.c_int32),
+        ("dot_with", ctypes.c_void_p),
+        ("dot_d", ctypes.c_void_p),
     ]
 
 

@@ -171,13 +171,17 @@ public:
     // Multigrid::standard_cycle (utopia_Multigrid.hpp:252-367). r: level rhs
     // (dtype r_dt); the correction c_l is accumulated in levels[l].c and the
     // final post-smoothed value is written to out (dtype out_dt).
-    void cycle(int l, const void *r, int r_dt, void *out, int out_dt) {
+    // With dot_with, also dot_d[0] = out . dot_with when the last kernel can
+    // fuse it (returns whether it did).
+    bool cycle(int l, const void *r, int r_dt, void *out, int out_dt, const double *dot_with = nullptr,
+               double *dot_d = nullptr) {
         auto &L = levels[(size_t)l];
         const int64_t n = L.A->rows();
         uhip_ctx c = L.A->c();
+        bool dot_done = false;
         if (l == 0) {
             coarse->solve_raw(r, r_dt, out, out_dt);
-            return;
+            return false;
         }
         auto &Lc = levels[(size_t)l - 1];
         for (int k = 0; k < mg_type; ++k) {
@@ -195,24 +199,26 @@ public:
             hip_check(uhip_spmv_add(c, &L.P->raw(), Lc.c.data(), L.c.data()), "mg interpolate");
             void *o = final_pass ? out : (void *)L.c.data();
             const int odt = final_pass ? out_dt : dt;
-            smooth(L, r, r_dt, false, o, odt, post, L.alpha_post, L.beta_post);
+            dot_done = smooth(L, r, r_dt, false, o, odt, post, L.alpha_post, L.beta_post,
+                              final_pass ? dot_with : nullptr, dot_d);
         }
         (void)n;
+        return dot_done;
     }
 
 private:
-    void smooth(MGLevel<T> &L, const void *b, int b_dt, bool zero_guess, void *out, int out_dt, int nu,
-                const std::vector<double> &a, const std::vector<double> &be) {
+    bool smooth(MGLevel<T> &L, const void *b, int b_dt, bool zero_guess, void *out, int out_dt, int nu,
+                const std::vector<double> &a, const std::vector<double> &be, const double *dot_with = nullptr,
+                double *dot_d = nullptr) {
         if (nu <= 0) {
             if (out != (void *)L.c.data())
                 hip_check(uhip_vec_axpy_cast(L.A->c(), L.A->rows(), 0.0, L.c.data(), dt, L.c.data(), dt, out, out_dt),
                           "copy");
-            return;
+            return false;
         }
         if (smoother == MG_CHEBYSHEV) {
-            chebyshev_smooth_fused<T>(*L.A, b, b_dt, L.c.data(), zero_guess, out, out_dt, nu, a, be, L.rk.data(),
-                                      L.pa.data(), L.pb.data());
-            return;
+            return chebyshev_smooth_fused<T>(*L.A, b, b_dt, L.c.data(), zero_guess, out, out_dt, nu, a, be,
+                                             L.rk.data(), L.pa.data(), L.pb.data(), dot_with, dot_d);
         }
         // Jacobi sweeps: r = b - A x; x += d_inv .* r (x = 0 on a zero guess)
         uhip_ctx c = L.A->c();
@@ -226,8 +232,21 @@ private:
                                          last ? out_dt : dt),
                       "jacobi x");
         }
+        return false;
     }
 };
+
+// z = M rhs for a V-cycle engine, plus dot_d = z . rhs (fused into the last
+// kernel when possible).
+template <typename T>
+inline void vcycle_apply_dot(VCycle<T> &eng, const HipVector &rhs, HipVector &z, double *dot_d) {
+    const bool fused = eng.cycle((int)eng.levels.size() - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64, rhs.data(),
+                                 dot_d);
+    if (!fused) {
+        const double *a[1] = {z.data()}, *b[1] = {rhs.data()};
+        hip_check(uhip_vec_dots_f64(z.ctx(), z.size(), 1, a, b, dot_d), "dot(z, r)");
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Multigrid<HipCrsMatrix, HipVector>
@@ -390,6 +409,12 @@ public:
         eng_.cycle(n_levels_ - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64);
         return true;
     }
+    bool apply_zero_initial_guess_dot(const HipVector &rhs, HipVector &z, double *dot_d) override {
+        if (this->max_it_ != 1 || this->verbose_) return LinearSolver::apply_zero_initial_guess_dot(rhs, z, dot_d);
+        if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
+        vcycle_apply_dot(eng_, rhs, z, dot_d);
+        return true;
+    }
 
     const std::vector<std::shared_ptr<HipCrsMatrix>> &interpolations() const { return I_; }
     const std::vector<std::shared_ptr<HipCrsMatrix>> &restrictions() const { return R_; }
@@ -459,6 +484,11 @@ public:
     bool apply_zero_initial_guess(const HipVector &rhs, HipVector &z) override {
         if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
         eng_.cycle((int)eng_.levels.size() - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64);
+        return true;
+    }
+    bool apply_zero_initial_guess_dot(const HipVector &rhs, HipVector &z, double *dot_d) override {
+        if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
+        vcycle_apply_dot(eng_, rhs, z, dot_d);
         return true;
     }
     const Multigrid &fp64() const { return *mg_; }

@@ -50,6 +50,21 @@ public:
         else sol.set(0);
         return apply(rhs, sol);
     }
+    // The same, followed by dot_d[0] = sol . rhs (device scalar); solvers that
+    // produce sol in a fused kernel compute the dot while writing it.
+    virtual bool apply_zero_initial_guess_dot(const Vector &rhs, Vector &sol, double *dot_d) {
+        const bool ok = apply_zero_initial_guess(rhs, sol);
+        const typename Vector::Scalar *a[1] = {sol.data()}, *b[1] = {rhs.data()};
+        if (Vector::dtype == UHIP_F64)
+            hip_check(uhip_vec_dots_f64(sol.ctx(), sol.size(), 1, (const double *const *)a, (const double *const *)b,
+                                        dot_d),
+                      "dot(z, r)");
+        else
+            hip_check(uhip_vec_dots_f32(sol.ctx(), sol.size(), 1, (const float *const *)a, (const float *const *)b,
+                                        dot_d),
+                      "dot(z, r)");
+        return ok;
+    }
     bool solve(const Matrix &A, const Vector &b, Vector &x) {
         update(std::shared_ptr<const Matrix>(&A, [](const Matrix *) {}));
         return apply(b, x);
@@ -223,9 +238,8 @@ private:
                 stop = this->check_convergence(it, r_norm, 1, 1);
                 break;
             }
-            precond_->apply_zero_initial_guess(r_new_, z_new_);
-            const double *zn[1] = {z_new_.data()}, *rn[1] = {r_new_.data()};
-            hip_check(uhip_vec_dots_f64(c, b.size(), 1, zn, rn, s + S_RZNEW), "dot(z_new,r_new)");
+            // z_new.set(0); precond->apply(r_new, z_new); dot(z_new, r_new)
+            precond_->apply_zero_initial_guess_dot(r_new_, z_new_, s + S_RZNEW);
             hip_check(uhip_cg_pupdate_f64(c, b.size(), p_.data(), z_new_.data(), s, S_RZNEW, S_RZ), "p update");
             std::swap(r_, r_new_);
             std::swap(z_, z_new_);
@@ -354,10 +368,12 @@ inline void chebyshev_coefficients(double eig_max, double eig_min, int nu, std::
 // nu Chebyshev steps on level operator A (values T) smoothing A x = b.
 // b has dtype b_dt; x (level dtype) is the initial guess unless zero_guess; the
 // result goes to out (dtype out_dt; may be x itself). Work: rk, pa, pb (level dtype).
+// With dot_with (FP64, only when out is FP64) the last step also writes
+// dot_d[0] = out . dot_with; returns false when it could not (nu == 1).
 template <typename T>
-void chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, T *x, bool zero_guess, void *out,
+bool chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, T *x, bool zero_guess, void *out,
                             int out_dt, int nu, const std::vector<double> &alpha, const std::vector<double> &beta,
-                            T *rk, T *pa, T *pb) {
+                            T *rk, T *pa, T *pb, const double *dot_with = nullptr, double *dot_d = nullptr) {
     constexpr int dt = HipDtype<T>::value;
     uhip_ctx c = A.c();
     const int64_t n = A.rows();
@@ -374,7 +390,7 @@ void chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, 
     if (nu == 1) {  // x = x + alpha0 p0
         hip_check(uhip_vec_axpy_cast(c, n, alpha[0], pprev, p_dt, zero_guess ? nullptr : x, dt, out, out_dt),
                   "cheb nu=1");
-        return;
+        return false;
     }
     T *pbuf[2] = {pb, pa};
     for (int k = 1; k < nu; ++k) {
@@ -395,10 +411,15 @@ void chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, 
         st.beta = beta[(size_t)k];
         st.a_cur = alpha[(size_t)k];
         st.last = last ? 1 : 0;
+        if (last && dot_with && out_dt == UHIP_F64) {
+            st.dot_with = dot_with;
+            st.dot_d = dot_d;
+        }
         hip_check(uhip_cheb_step_apply(c, &A.raw(), &st), "cheb step");
         pprev = pnext;
         p_dt = dt;
     }
+    return dot_with != nullptr && out_dt == UHIP_F64;
 }
 
 }  // namespace utopia

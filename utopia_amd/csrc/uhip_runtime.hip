@@ -27,8 +27,15 @@ __global__ __launch_bounds__(1024) void k_finish_sums(const double *__restrict__
                                                       unsigned sqrt_mask) {
     const int k = blockIdx.x;
     const double *p = partials + (int64_t)k * stride;
-    double s = 0.0;
-    for (int64_t b = threadIdx.x; b < nparts; b += 1024) s += p[b];
+    // eight independent chains keep eight loads in flight; combined in a fixed order
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t b = threadIdx.x;
+    for (; b + 7 * 1024 < nparts; b += 8 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += p[b + u * 1024];
+    }
+    for (int u = 0; b < nparts; b += 1024, ++u) a[u] += p[b];
+    double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     s = block_sum<1024>(s);
     if (threadIdx.x == 0) result[k] = ((sqrt_mask >> k) & 1u) ? sqrt(s) : s;
 }

@@ -329,32 +329,45 @@ int uhip_spmv_add(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
     return launch_rowblock<float>(ctx, A, (const float *)x, e);
 }
 
+}  // extern "C"
+
+namespace uhip {
+
+template <typename TAcc, typename TP, typename TX, bool DOT>
+static int cheb_typed(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *st) {
+    EpiCheb<TAcc, TP, TAcc, TAcc, TX, DOT> e{(const TP *)st->pprev, (const TAcc *)st->rprev,
+                                             (const TAcc *)st->xin, (TX *)st->xout,
+                                             (TAcc *)st->rout,      (TAcc *)st->pout,
+                                             (TAcc)st->a_prev,      (TAcc)st->beta,
+                                             (TAcc)st->a_cur,       st->last,
+                                             st->dot_with};
+    UHIP_TRY(launch_rowblock<TAcc>(ctx, A, (const TP *)st->pprev, e));
+    if (DOT) return finish_sums(ctx, 1, A->nparts, ctx->max_partials, st->dot_d);
+    return UHIP_OK;
+}
+
+template <typename TAcc, typename TP, typename TX>
+static int cheb_dispatch(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *st) {
+    if (st->dot_with) return cheb_typed<TAcc, TP, TX, true>(ctx, A, st);
+    return cheb_typed<TAcc, TP, TX, false>(ctx, A, st);
+}
+
+}  // namespace uhip
+
+extern "C" {
+
 int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *st) {
     UHIP_REQUIRE(ctx && A && st && st->pprev && st->xout, "uhip_cheb_step_apply: null argument");
     UHIP_REQUIRE(st->last || (st->rout && st->pout), "uhip_cheb_step_apply: rout/pout needed when last == 0");
+    UHIP_REQUIRE(!st->dot_with || (st->last && st->dot_d), "uhip_cheb_step_apply: dot only on the last step");
     if (A->dtype == UHIP_F64) {
         UHIP_REQUIRE(st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F64, "uhip_cheb_step_apply: FP64 level");
-        EpiCheb<double, double, double, double, double> e{
-            (const double *)st->pprev, (const double *)st->rprev, (const double *)st->xin, (double *)st->xout,
-            (double *)st->rout,        (double *)st->pout,        st->a_prev,              st->beta,
-            st->a_cur,                 st->last};
-        return launch_rowblock<double>(ctx, A, (const double *)st->pprev, e);
+        return cheb_dispatch<double, double, double>(ctx, A, st);
     }
-    const float ap = (float)st->a_prev, be = (float)st->beta, ac = (float)st->a_cur;
-#define UHIP_CHEB_CASE(TP, TX)                                                                                     \
-    {                                                                                                              \
-        EpiCheb<float, TP, float, float, TX> e{(const TP *)st->pprev, (const float *)st->rprev,                   \
-                                               (const float *)st->xin, (TX *)st->xout,                            \
-                                               (float *)st->rout,      (float *)st->pout,                         \
-                                               ap,                     be,                                        \
-                                               ac,                     st->last};                                 \
-        return launch_rowblock<float>(ctx, A, (const TP *)st->pprev, e);                                           \
-    }
-    if (st->p_dtype == UHIP_F32 && st->x_dtype == UHIP_F32) UHIP_CHEB_CASE(float, float)
-    if (st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F32) UHIP_CHEB_CASE(double, float)
-    if (st->p_dtype == UHIP_F32 && st->x_dtype == UHIP_F64) UHIP_CHEB_CASE(float, double)
-    if (st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F64) UHIP_CHEB_CASE(double, double)
-#undef UHIP_CHEB_CASE
+    if (st->p_dtype == UHIP_F32 && st->x_dtype == UHIP_F32) return cheb_dispatch<float, float, float>(ctx, A, st);
+    if (st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F32) return cheb_dispatch<float, double, float>(ctx, A, st);
+    if (st->p_dtype == UHIP_F32 && st->x_dtype == UHIP_F64) return cheb_dispatch<float, float, double>(ctx, A, st);
+    if (st->p_dtype == UHIP_F64 && st->x_dtype == UHIP_F64) return cheb_dispatch<float, double, double>(ctx, A, st);
     return set_error(UHIP_ERR_INVALID, "uhip_cheb_step_apply: dtypes");
 }
 

@@ -265,9 +265,11 @@ struct EpiAddTo {
 //   r_k = r_{k-1} + a_prev*s ; p_k = -r_k + beta*p_{k-1} ; x += a_prev*p_{k-1}
 // and on the last step also x += a_cur*p_k. rprev == null means r_{k-1} = -p_{k-1}
 // (first step), xin == null means x = 0 (zero initial guess).
-template <typename TAcc, typename TP, typename TV, typename TXI, typename TXO>
+// With DOT, part += xout_r * dot_with_r (the z.r of the CG step that follows a
+// preconditioner application, computed while z is written).
+template <typename TAcc, typename TP, typename TV, typename TXI, typename TXO, bool DOT = false>
 struct EpiCheb {
-    static constexpr int kDots = 0;
+    static constexpr int kDots = DOT ? 1 : 0;
     const TP *pprev;
     const TV *rprev;
     const TXI *xin;
@@ -276,7 +278,8 @@ struct EpiCheb {
     TV *pout;
     TAcc a_prev, beta, a_cur;
     int last;
-    __device__ void operator()(int r, TAcc s, double *) const {
+    const double *dot_with;
+    __device__ void operator()(int r, TAcc s, double *part) const {
         const TAcc pp = (TAcc)pprev[r];
         const TAcc rp = rprev ? (TAcc)rprev[r] : -pp;
         const TAcc rk = rp + a_prev * s;
@@ -289,7 +292,9 @@ struct EpiCheb {
             rout[r] = (TV)rk;
             pout[r] = (TV)pk;
         }
-        xout[r] = (TXO)xv;
+        const TXO xo = (TXO)xv;
+        xout[r] = xo;
+        if constexpr (DOT) part[0] += (double)xo * dot_with[r];
     }
 };
 
