@@ -67,6 +67,16 @@ int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h);
 int uhip_write_scalars(uhip_ctx ctx, double *dst_d, int n, const double *src_h);
 int uhip_event_timer_start(uhip_ctx ctx, void **ev_pair);
 int uhip_event_timer_stop(uhip_ctx ctx, void *ev_pair, float *ms);
+/* Pre-size the reduction workspace (per_slot partial sums per dot) so that a
+ * later captured sequence allocates nothing. */
+int uhip_ctx_reserve(uhip_ctx ctx, int64_t per_slot);
+/* hipGraph capture of the work issued on the context stream between begin and
+ * end; launch replays it on the same stream. No reference analogue (the
+ * reference drives every iteration from the host). */
+int uhip_graph_begin(uhip_ctx ctx);
+int uhip_graph_end(uhip_ctx ctx, void **exec);
+int uhip_graph_launch(uhip_ctx ctx, void *exec);
+int uhip_graph_destroy(void *exec);
 
 /* ------------------------------------------------------------------------
  * BLAS1 on device vectors (n entries). Suffix _f64 / _f32.
@@ -352,6 +362,74 @@ int uhip_lsolve_set_stol(uhip_solver s, double v);
 int uhip_lsolve_set_verbosity(uhip_solver s, int level);
 
 /* ------------------------------------------------------------------------
+ * Multi-RHS and batched kernels (no reference analogue: utopia vectors are
+ * single-column and every solve is one system). Blocks of k right-hand sides
+ * are row-major [n x k]; k in {1, 2, 4, 8, 16}. A batch of independent systems
+ * is one block-diagonal CSR; rowseg[i] is the system of row i.
+ * ---------------------------------------------------------------------- */
+/* op 0: Y = A X; op 1: Y += A X; op 2: Y = B - A X (B dtype b_dtype).
+ * Each column is the single-vector SpMV of PetscMatrix::multiply
+ * (backend/petsc/utopia_petsc_Matrix.cpp:1137-1152) with per-row sums in entry order. */
+int uhip_spmm(uhip_ctx ctx, const uhip_csr *A, int k, const void *X, int x_dtype, void *Y, int y_dtype, int op,
+              const void *B, int b_dtype);
+/* One Chebyshev3level step (solvers/smoothers/utopia_Chebyshev3level.hpp:150-175)
+ * on k columns with per-system coefficients coef[3*s + {0,1,2}] = (alpha_{k-1}, beta_k, alpha_k).
+ * rprev NULL: r = -pprev; xin NULL: x = 0; last: x += alpha_k p_k, no rout/pout. */
+int uhip_spmm_cheb_step(uhip_ctx ctx, const uhip_csr *A, int k, const void *pprev, int p_dtype, const void *rprev,
+                        const void *xin, void *xout, int x_dtype, void *rout, void *pout, const int32_t *rowseg,
+                        const double *coef, int last);
+/* out = xin + coef[3*rowseg[i]] * p (one-step Chebyshev); xin may be NULL */
+int uhip_axpy_seg(uhip_ctx ctx, int32_t n, int k, const int32_t *rowseg, const double *coef, const void *p,
+                  int p_dtype, const void *xin, void *out, int out_dtype, int level_dtype);
+/* out[s*k + j] = sum over rows of system s of x[i,j]*y[i,j] (sqrt if take_sqrt).
+ * Row chunks [c_row0[c], c_row1[c]) never straddle systems; system s owns chunks
+ * seg_c0[s] .. seg_c0[s+1]-1. Fixed summation order (deterministic). */
+int uhip_segdot(uhip_ctx ctx, int k, int32_t nchunks, const int32_t *c_row0, const int32_t *c_row1, int32_t nseg,
+                const int32_t *seg_c0, const void *x, const void *y, int dtype, double *out, int take_sqrt);
+/* Batched CG stages for m = nseg*k (system, rhs) pairs, arithmetic of
+ * ConjugateGradient::preconditioned_solve (solvers/smoothers/utopia_ConjugateGradient_impl.hpp:195-294).
+ * state_d = {rz, pap, alpha, beta, rznew, rr, atol} (device, m each); flags 0 active,
+ * 1 converged, 2 stopped on a non-finite alpha. stage 0: alpha = rz/pap, x += alpha p,
+ * r -= alpha Ap; stage 1: rr (= |r|) <= atol converges, *iter += 1, conv_it = *iter,
+ * *active_count = active pairs; stage 2: beta = rznew/rz, rz = rznew, p = z + beta p. */
+int uhip_bcg_step(uhip_ctx ctx, int stage, int32_t n, int k, int32_t m, const int32_t *rowseg, double *x, double *p,
+                  double *r, const double *Ap, const double *z, double *const *state_d, int32_t *flags,
+                  int32_t *conv_it, int32_t *iter, int32_t *active_count);
+/* v = w / norm[rowseg[i]] for systems with done == NULL or done[s] == 0 (power method) */
+int uhip_seg_scale(uhip_ctx ctx, int32_t n, const int32_t *rowseg, const double *norm, const int32_t *done,
+                   const double *w, double *v);
+/* Value gradient on the pattern of A: grad[e] = -sum_j lambda[row(e), j] * x[col(e), j]
+ * for L(x(A)), with lambda = A^-T dL/dx (the adjoint solve). */
+int uhip_pattern_grad(uhip_ctx ctx, const uhip_csr *A, int k, const double *lambda, const double *x, double *grad);
+
+/* ------------------------------------------------------------------------
+ * Batched MG-preconditioned CG (no reference analogue): nseg independent SPD
+ * systems, k right-hand sides each, FP64 CG with an FP64 or FP32 V-cycle
+ * (Chebyshev smoothing, Galerkin coarse operators, dense coarse inverses per
+ * system). One CG iteration of the whole batch is a captured hipGraph.
+ * ---------------------------------------------------------------------- */
+typedef struct uhip_bsolver_s *uhip_bsolver;
+int uhip_bsolver_create(uhip_ctx ctx, int k, int mixed, uhip_bsolver *out);
+int uhip_bsolver_destroy(uhip_bsolver s);
+/* rtol max_it pre_smoothing_steps post_smoothing_steps scale_max_eig scale_min_eig
+ * power_method_tol power_method_max_it use_graph */
+int uhip_bsolver_set_param(uhip_bsolver s, const char *key, double value);
+/* Block-diagonal operator (device CSR, FP64) with seg[0..nseg] row offsets;
+ * nlev-1 interpolations coarse to fine (device CSR, finest last), coarse_seg
+ * holds the nseg+1 row offsets of each coarse level (coarsest first). */
+int uhip_bsolver_setup(uhip_bsolver s, int32_t nseg, const int32_t *seg, const uhip_csr *A, int nlev,
+                       const uhip_csr *const *interps, const int32_t *const *coarse_seg);
+/* Device arrays [n x k]: X holds the initial guess and receives the solution. */
+int uhip_bsolver_solve(uhip_bsolver s, const double *B_d, double *X_d, int64_t *iterations);
+/* per (system, rhs): iterations to convergence (-1: not converged) and flags */
+int uhip_bsolver_get_conv(uhip_bsolver s, int32_t *conv_it, int32_t *flags);
+/* dL/dA on the pattern of the finest operator from lambda and x (device, [n x k]) */
+int uhip_bsolver_pattern_grad(uhip_bsolver s, const double *lambda_d, const double *x_d, double *grad_d);
+/* setup_seconds solve_seconds iterations graph_replays n_levels eig_max_<l>_<s> */
+int uhip_bsolver_get_info(uhip_bsolver s, const char *key, double *value);
+int uhip_bsolver_set_level_eig(uhip_bsolver s, int l, int32_t sys, double eig_max);
+
+/* ------------------------------------------------------------------------
  * Problem generators (host, seeded): the inputs every path consumes.
  * splitmix64 stream; uniform in [0,1) = (u >> 11) * 2^-53; normals by
  * Box-Muller. Arrays are caller-allocated host memory.
@@ -367,7 +445,7 @@ int64_t uhip_gen_nnz(int kind, int32_t nx, int32_t ny, int32_t nz);
 int uhip_gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift,
                      int32_t *rowptr, int32_t *col, double *val);
 /* Trilinear (bilinear / linear for nz==1 / ny==1) prolongation from the coarse
- * grid (nx/2, ny/2, nz/2) to the fine grid (nx, ny, nz), nx, ny, nz even (or 1).
+ * grid (nx/2, ny/2, nz/2) (integer halves; a side of 1 stays 1) to the fine grid (nx, ny, nz).
  * Fine point f (0-based) takes coarse point (f-1)/2 with weight 1 when f is odd,
  * coarse points f/2-1 and f/2 with weight 1/2 each (those inside the grid) when
  * f is even. */

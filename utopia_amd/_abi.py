@@ -121,7 +121,7 @@ def _ctype(decl: str):
     toks = d.split()
     if not toks:
         return None
-    if toks[0] in ("uhip_ctx", "uhip_solver"):
+    if toks[0] in ("uhip_ctx", "uhip_solver", "uhip_bsolver"):
         return ctypes.c_void_p
     if toks[0] in _SCALARS:
         return _SCALARS[toks[0]]
@@ -143,6 +143,12 @@ def lib():
     if not LIB_PATH.exists():
         raise LibraryMissing(
             f"{LIB_PATH} is not built; run `python -m utopia_amd.build` (or __graft_entry__.build()).")
+    # torch ships its own HIP runtime; when it is loaded first the library binds
+    # to that one, so torch tensors and this library share one device runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(str(LIB_PATH))
     for name, (ret, args) in parse_header().items():
         fn = getattr(L, name, None)

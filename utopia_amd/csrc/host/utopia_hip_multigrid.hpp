@@ -492,6 +492,13 @@ public:
         return true;
     }
     const Multigrid &fp64() const { return *mg_; }
+    // Chebyshev bounds of level l on both the FP64 hierarchy and its FP32 copy
+    void set_level_eigs(int l, double emax, double emin) {
+        mg_->set_level_eigs(l, emax, emin);
+        eng_.levels[(size_t)l].eig_max = emax;
+        eng_.levels[(size_t)l].eig_min = emin;
+        eng_.allocate_work();
+    }
 
 private:
     std::shared_ptr<Multigrid> mg_;

@@ -1,0 +1,443 @@
+// Batched, multi-right-hand-side kernels: many independent SPD systems stored
+// as one block-diagonal CSR matrix (segments of rows), K right-hand sides per
+// system stored row-major (entry (i, j) at i*K + j).
+//
+//   SpMM      one K-lane group per row, lane j owns right-hand side j; the
+//             row's values/columns are read once for all K columns and the
+//             gathered x entries of a column index are K contiguous values.
+//   segdot    per (segment, column) dot products, two fixed-order stages.
+//   batched CG / Chebyshev steps with per-(segment, column) step lengths.
+#include "uhip_common.hpp"
+
+namespace uhip {
+
+template <int K, typename TAcc, class Epi, typename TA, typename TX>
+__global__ __launch_bounds__(256) void k_spmm_csr(int32_t n, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col, const TA *__restrict__ val,
+                                                  const TX *__restrict__ X, Epi epi) {
+    constexpr int RPB = 256 / K;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int32_t r = b * RPB + threadIdx.x / K;
+    const int j = threadIdx.x % K;
+    if (r >= n) return;
+    TAcc acc = TAcc(0);
+    const int32_t e1 = rowptr[r + 1];
+    for (int32_t e = rowptr[r]; e < e1; ++e) acc += (TAcc)val[e] * (TAcc)X[(int64_t)col[e] * K + j];
+    epi(r, j, acc);
+}
+
+template <int K, typename TAcc, typename TY>
+struct EpiKStore {
+    TY *y;
+    __device__ void operator()(int32_t r, int j, TAcc s) const { y[(int64_t)r * K + j] = (TY)s; }
+};
+
+template <int K, typename TAcc, typename TY>
+struct EpiKAdd {
+    TY *y;
+    __device__ void operator()(int32_t r, int j, TAcc s) const {
+        const int64_t i = (int64_t)r * K + j;
+        y[i] = (TY)((TAcc)y[i] + s);
+    }
+};
+
+template <int K, typename TAcc, typename TB, typename TY>
+struct EpiKResidual {
+    const TB *b;
+    TY *y;
+    __device__ void operator()(int32_t r, int j, TAcc s) const {
+        const int64_t i = (int64_t)r * K + j;
+        y[i] = (TY)((TAcc)b[i] - s);
+    }
+};
+
+// Chebyshev step (same algebra as the single-vector EpiCheb) with per-segment
+// coefficients coef[3*seg + {0,1,2}] = (a_prev, beta, a_cur).
+template <int K, typename TAcc, typename TP, typename TXO>
+struct EpiKCheb {
+    const TP *pprev;
+    const TAcc *rprev;
+    const TAcc *xin;
+    TXO *xout;
+    TAcc *rout;
+    TAcc *pout;
+    const int32_t *rowseg;
+    const double *coef;
+    int last;
+    __device__ void operator()(int32_t r, int j, TAcc s) const {
+        const int64_t i = (int64_t)r * K + j;
+        const int32_t g = rowseg[r];
+        const TAcc a_prev = (TAcc)coef[3 * g], beta = (TAcc)coef[3 * g + 1], a_cur = (TAcc)coef[3 * g + 2];
+        const TAcc pp = (TAcc)pprev[i];
+        const TAcc rp = rprev ? rprev[i] : -pp;
+        const TAcc rk = rp + a_prev * s;
+        const TAcc pk = -rk + beta * pp;
+        TAcc xv = xin ? xin[i] : TAcc(0);
+        xv = xv + a_prev * pp;
+        if (last) {
+            xv = xv + a_cur * pk;
+        } else {
+            rout[i] = rk;
+            pout[i] = pk;
+        }
+        xout[i] = (TXO)xv;
+    }
+};
+
+template <int K, typename TAcc, class Epi, typename TA, typename TX>
+int launch_spmm(uhip_ctx ctx, const uhip_csr *A, const TX *X, const Epi &e) {
+    const int32_t n = A->nrows;
+    if (n == 0) return UHIP_OK;
+    constexpr int RPB = 256 / K;
+    hipLaunchKernelGGL((k_spmm_csr<K, TAcc, Epi, TA, TX>), dim3((n + RPB - 1) / RPB), dim3(256), 0, ctx->stream, n,
+                       A->rowptr, A->col, (const TA *)A->val, X, e);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+// ---- segmented dots: out[s*K + j] = sum_{i in seg s} x[i,j] * y[i,j] ----
+// Chunks of rows never straddle a segment; chunk c covers [c_row0[c], c_row1[c]).
+template <int K, typename TX, typename TY>
+__global__ __launch_bounds__(256) void k_segdot_chunk(const int32_t *__restrict__ c_row0,
+                                                      const int32_t *__restrict__ c_row1, const TX *__restrict__ x,
+                                                      const TY *__restrict__ y, double *__restrict__ partials) {
+    constexpr int RPB = 256 / K;
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    const int32_t r0 = c_row0[c], r1 = c_row1[c];
+    const int j = threadIdx.x % K, ro = threadIdx.x / K;
+    double acc = 0.0;
+    for (int32_t r = r0 + ro; r < r1; r += RPB) {
+        const int64_t i = (int64_t)r * K + j;
+        acc += (double)x[i] * (double)y[i];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RPB / 2; s > 0; s >>= 1) {  // fixed tree over row offsets, per column
+        if (ro < s) red[threadIdx.x] += red[threadIdx.x + s * K];
+        __syncthreads();
+    }
+    if (ro == 0) partials[(int64_t)c * K + j] = red[j];
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_segdot_finish(int32_t nseg, const int32_t *__restrict__ seg_c0,
+                                                       const double *__restrict__ partials, double *__restrict__ out,
+                                                       int take_sqrt) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nseg * K) return;
+    const int s = t / K, j = t % K;
+    double acc = 0.0;
+    for (int32_t c = seg_c0[s]; c < seg_c0[s + 1]; ++c) acc += partials[(int64_t)c * K + j];
+    out[t] = take_sqrt ? sqrt(acc) : acc;
+}
+
+// ---- batched CG on per-(segment, column) scalars ----
+// state: rz, pap, alpha, beta, rznew, rr, atol, flags (0 active, 1 converged, 2 stopped on a
+// non-finite step), conv_it.
+struct BCGState {
+    double *rz, *pap, *alpha, *beta, *rznew, *rr, *atol;
+    int32_t *flags, *conv_it;
+};
+
+__global__ __launch_bounds__(256) void k_bcg_alpha(int32_t m, BCGState st) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= m) return;
+    double a = st.rz[t] / st.pap[t];
+    if (st.flags[t] != 0) {
+        a = 0.0;
+    } else if (!isfinite(a)) {  // utopia_ConjugateGradient_impl.hpp:241-244: stop without updating x
+        st.flags[t] = 2;
+        a = 0.0;
+    }
+    st.alpha[t] = a;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_bcg_update(int32_t n, const int32_t *__restrict__ rowseg,
+                                                    const double *__restrict__ alpha, double *__restrict__ x,
+                                                    const double *__restrict__ p, double *__restrict__ r,
+                                                    const double *__restrict__ Ap) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * K) return;
+    const int32_t row = (int32_t)(i / K);
+    const double a = alpha[rowseg[row] * K + (int)(i % K)];
+    if (a == 0.0) return;
+    x[i] = x[i] + a * p[i];
+    r[i] = r[i] - a * Ap[i];
+}
+
+// after |r| is known: converge (flag 1) where |r| <= atol; count the still active ones
+// iter (device) counts the iterations done; it is advanced here so that a
+// captured iteration replays with the right count.
+__global__ __launch_bounds__(256) void k_bcg_check(int32_t m, BCGState st, int32_t *__restrict__ iter,
+                                                   int32_t *__restrict__ active_count) {
+    __shared__ int32_t cnt[256];
+    const int32_t it = *iter + 1;
+    int32_t c = 0;
+    for (int t = threadIdx.x; t < m; t += 256) {
+        if (st.flags[t] == 0) {
+            if (st.rr[t] <= st.atol[t]) {
+                st.flags[t] = 1;
+                st.conv_it[t] = it;
+            } else {
+                ++c;
+            }
+        }
+    }
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) cnt[threadIdx.x] += cnt[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *active_count = cnt[0];
+        *iter = it;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bcg_beta(int32_t m, BCGState st) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= m) return;
+    st.beta[t] = (st.flags[t] == 0) ? st.rznew[t] / st.rz[t] : 0.0;
+    st.rz[t] = st.rznew[t];
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_bcg_pupdate(int32_t n, const int32_t *__restrict__ rowseg,
+                                                     const double *__restrict__ beta, double *__restrict__ p,
+                                                     const double *__restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * K) return;
+    const int32_t row = (int32_t)(i / K);
+    p[i] = z[i] + beta[rowseg[row] * K + (int)(i % K)] * p[i];
+}
+
+// x_i = xin_i + a[seg]*p_i (Chebyshev with one sweep), dtype conversions
+template <int K, typename TP, typename TX, typename TO, typename TAcc>
+__global__ __launch_bounds__(256) void k_axpy_seg(int32_t n, const int32_t *__restrict__ rowseg,
+                                                  const double *__restrict__ coef, const TP *__restrict__ p,
+                                                  const TX *__restrict__ xin, TO *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * K) return;
+    const TAcc a = (TAcc)coef[3 * rowseg[i / K]];
+    TAcc v = xin ? (TAcc)xin[i] : TAcc(0);
+    out[i] = (TO)(v + a * (TAcc)p[i]);
+}
+
+// per segment power-method helpers (single column)
+__global__ __launch_bounds__(256) void k_seg_scale(int32_t n, const int32_t *__restrict__ rowseg,
+                                                   const double *__restrict__ norm, const int32_t *__restrict__ done,
+                                                   const double *__restrict__ w, double *__restrict__ v) {
+    const int32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int32_t s = rowseg[i];
+    if (done && done[s]) return;
+    v[i] = (1. / norm[s]) * w[i];
+}
+
+// grad_e = -sum_j lambda[i, j] * x[col_e, j] over the pattern (dL/dA_ij = -lambda_i x_j)
+template <int K>
+__global__ __launch_bounds__(256) void k_pattern_grad(int32_t n, const int32_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ lam, const double *__restrict__ x,
+                                                      double *__restrict__ grad) {
+    const int32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    double l[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) l[j] = lam[(int64_t)r * K + j];
+    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) {
+        const int64_t c = (int64_t)col[e] * K;
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) s += l[j] * x[c + j];
+        grad[e] = -s;
+    }
+}
+
+}  // namespace uhip
+
+using namespace uhip;
+
+// ---- dispatch helpers (K in {1, 2, 4, 8, 16}) ----
+#define UHIP_K_SWITCH(k, ...)                                         \
+    switch (k) {                                                      \
+        case 1: { constexpr int KK = 1; __VA_ARGS__; } break;         \
+        case 2: { constexpr int KK = 2; __VA_ARGS__; } break;         \
+        case 4: { constexpr int KK = 4; __VA_ARGS__; } break;         \
+        case 8: { constexpr int KK = 8; __VA_ARGS__; } break;         \
+        case 16: { constexpr int KK = 16; __VA_ARGS__; } break;       \
+        default: return set_error(UHIP_ERR_INVALID, "K=%d not in {1,2,4,8,16}", k); \
+    }
+
+extern "C" {
+
+int uhip_spmm(uhip_ctx ctx, const uhip_csr *A, int k, const void *X, int x_dtype, void *Y, int y_dtype, int op,
+              const void *B, int b_dtype) {
+    UHIP_REQUIRE(ctx && A && X && Y, "uhip_spmm: null argument");
+    UHIP_REQUIRE(op >= 0 && op <= 2, "uhip_spmm: op %d", op);
+    UHIP_REQUIRE(op != 2 || B, "uhip_spmm: residual needs B");
+    // op 0: Y = A X; 1: Y += A X; 2: Y = B - A X. Accumulation in the matrix precision.
+#define UHIP_SPMM_T(TA, TX, TY, TB)                                                                          \
+    do {                                                                                                     \
+        if (op == 0) {                                                                                       \
+            UHIP_K_SWITCH(k, UHIP_TRY((launch_spmm<KK, TA, EpiKStore<KK, TA, TY>, TA, TX>(                 \
+                                 ctx, A, (const TX *)X, EpiKStore<KK, TA, TY>{(TY *)Y}))));                  \
+        } else if (op == 1) {                                                                                \
+            UHIP_K_SWITCH(k, UHIP_TRY((launch_spmm<KK, TA, EpiKAdd<KK, TA, TY>, TA, TX>(                   \
+                                 ctx, A, (const TX *)X, EpiKAdd<KK, TA, TY>{(TY *)Y}))));                    \
+        } else {                                                                                             \
+            UHIP_K_SWITCH(k, UHIP_TRY((launch_spmm<KK, TA, EpiKResidual<KK, TA, TB, TY>, TA, TX>(          \
+                                 ctx, A, (const TX *)X, EpiKResidual<KK, TA, TB, TY>{(const TB *)B, (TY *)Y})))); \
+        }                                                                                                    \
+        return UHIP_OK;                                                                                      \
+    } while (0)
+    if (A->dtype == UHIP_F64) {
+        if (x_dtype == UHIP_F64 && y_dtype == UHIP_F64) UHIP_SPMM_T(double, double, double, double);
+        if (x_dtype == UHIP_F32 && y_dtype == UHIP_F32) UHIP_SPMM_T(double, float, float, double);
+        if (x_dtype == UHIP_F32 && y_dtype == UHIP_F64) UHIP_SPMM_T(double, float, double, double);
+        if (x_dtype == UHIP_F64 && y_dtype == UHIP_F32) UHIP_SPMM_T(double, double, float, double);
+    } else {
+        UHIP_REQUIRE(y_dtype == UHIP_F32, "uhip_spmm: FP32 matrix writes FP32");
+        if (x_dtype == UHIP_F32) {
+            if (op != 2 || b_dtype == UHIP_F32) UHIP_SPMM_T(float, float, float, float);
+            UHIP_SPMM_T(float, float, float, double);
+        }
+    }
+#undef UHIP_SPMM_T
+    return set_error(UHIP_ERR_INVALID, "uhip_spmm: dtype combination");
+}
+
+int uhip_spmm_cheb_step(uhip_ctx ctx, const uhip_csr *A, int k, const void *pprev, int p_dtype, const void *rprev,
+                        const void *xin, void *xout, int x_dtype, void *rout, void *pout, const int32_t *rowseg,
+                        const double *coef, int last) {
+    UHIP_REQUIRE(ctx && A && pprev && xout && rowseg && coef, "uhip_spmm_cheb_step: null argument");
+    UHIP_REQUIRE(last || (rout && pout), "uhip_spmm_cheb_step: rout/pout needed");
+#define UHIP_SPMM_CHEB(TA, TP, TX)                                                                              \
+    do {                                                                                                        \
+        UHIP_K_SWITCH(k,                                                                                      \
+                          EpiKCheb<KK, TA, TP, TX> e{(const TP *)pprev, (const TA *)rprev, (const TA *)xin,     \
+                                                     (TX *)xout,         (TA *)rout,         (TA *)pout,         \
+                                                     rowseg,             coef,               last};              \
+                          UHIP_TRY((launch_spmm<KK, TA, EpiKCheb<KK, TA, TP, TX>, TA, TP>(ctx, A, (const TP *)pprev, \
+                                                                                           e)));                \
+                      );                                                                                      \
+        return UHIP_OK;                                                                                         \
+    } while (0)
+    if (A->dtype == UHIP_F64) {
+        UHIP_REQUIRE(p_dtype == UHIP_F64 && x_dtype == UHIP_F64, "uhip_spmm_cheb_step: FP64 level");
+        UHIP_SPMM_CHEB(double, double, double);
+    }
+    if (p_dtype == UHIP_F32 && x_dtype == UHIP_F32) UHIP_SPMM_CHEB(float, float, float);
+    if (p_dtype == UHIP_F64 && x_dtype == UHIP_F32) UHIP_SPMM_CHEB(float, double, float);
+    if (p_dtype == UHIP_F32 && x_dtype == UHIP_F64) UHIP_SPMM_CHEB(float, float, double);
+    if (p_dtype == UHIP_F64 && x_dtype == UHIP_F64) UHIP_SPMM_CHEB(float, double, double);
+#undef UHIP_SPMM_CHEB
+    return set_error(UHIP_ERR_INVALID, "uhip_spmm_cheb_step: dtypes");
+}
+
+int uhip_axpy_seg(uhip_ctx ctx, int32_t n, int k, const int32_t *rowseg, const double *coef, const void *p,
+                  int p_dtype, const void *xin, void *out, int out_dtype, int level_dtype) {
+    UHIP_REQUIRE(ctx && rowseg && coef && p && out, "uhip_axpy_seg: null argument");
+    if (n == 0) return UHIP_OK;
+    const int64_t m = (int64_t)n * k;
+    dim3 g((unsigned)((m + 255) / 256)), b(256);
+#define UHIP_AXS(TP, TX, TO, TA)                                                                              \
+    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_axpy_seg<KK, TP, TX, TO, TA>), g, b, 0, ctx->stream, n, rowseg, coef, \
+                                         (const TP *)p, (const TX *)xin, (TO *)out))
+    if (level_dtype == UHIP_F64) {
+        UHIP_AXS(double, double, double, double);
+    } else if (p_dtype == UHIP_F64 && out_dtype == UHIP_F32) {
+        UHIP_AXS(double, float, float, float);
+    } else if (p_dtype == UHIP_F32 && out_dtype == UHIP_F64) {
+        UHIP_AXS(float, float, double, float);
+    } else if (p_dtype == UHIP_F64 && out_dtype == UHIP_F64) {
+        UHIP_AXS(double, float, double, float);
+    } else {
+        UHIP_AXS(float, float, float, float);
+    }
+#undef UHIP_AXS
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_segdot(uhip_ctx ctx, int k, int32_t nchunks, const int32_t *c_row0, const int32_t *c_row1, int32_t nseg,
+                const int32_t *seg_c0, const void *x, const void *y, int dtype, double *out, int take_sqrt) {
+    UHIP_REQUIRE(ctx && c_row0 && c_row1 && seg_c0 && x && y && out, "uhip_segdot: null argument");
+    if (nchunks == 0) return UHIP_OK;
+    UHIP_TRY(ensure_partials(ctx, (int64_t)nchunks * k));
+    if (dtype == UHIP_F64) {
+        UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_chunk<KK, double, double>), dim3(nchunks), dim3(256), 0,
+                                             ctx->stream, c_row0, c_row1, (const double *)x, (const double *)y,
+                                             ctx->partials));
+    } else {
+        UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_chunk<KK, float, float>), dim3(nchunks), dim3(256), 0,
+                                             ctx->stream, c_row0, c_row1, (const float *)x, (const float *)y,
+                                             ctx->partials));
+    }
+    UHIP_CHECK_LAUNCH();
+    const int m = nseg * k;
+    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_finish<KK>), dim3((m + 255) / 256), dim3(256), 0, ctx->stream, nseg,
+                                         seg_c0, ctx->partials, out, take_sqrt));
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_bcg_step(uhip_ctx ctx, int stage, int32_t n, int k, int32_t m, const int32_t *rowseg, double *x, double *p,
+                  double *r, const double *Ap, const double *z, double *const *state_d, int32_t *flags,
+                  int32_t *conv_it, int32_t *iter, int32_t *active_count) {
+    // state_d: rz, pap, alpha, beta, rznew, rr, atol
+    UHIP_REQUIRE(ctx && state_d && flags && conv_it && iter && active_count, "uhip_bcg_step: null argument");
+    BCGState st{state_d[0], state_d[1], state_d[2], state_d[3], state_d[4], state_d[5], state_d[6], flags, conv_it};
+    const int64_t nk = (int64_t)n * k;
+    dim3 gm((m + 255) / 256), gv((unsigned)((nk + 255) / 256)), b(256);
+    switch (stage) {
+        case 0:  // alpha and x, r update
+            hipLaunchKernelGGL(k_bcg_alpha, gm, b, 0, ctx->stream, m, st);
+            UHIP_CHECK_LAUNCH();
+            if (nk > 0) {
+                UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_bcg_update<KK>), gv, b, 0, ctx->stream, n, rowseg, st.alpha, x,
+                                                     (const double *)p, r, Ap));
+                UHIP_CHECK_LAUNCH();
+            }
+            return UHIP_OK;
+        case 1:  // convergence check (rr holds |r|)
+            hipLaunchKernelGGL(k_bcg_check, dim3(1), b, 0, ctx->stream, m, st, iter, active_count);
+            UHIP_CHECK_LAUNCH();
+            return UHIP_OK;
+        case 2:  // beta and p update
+            hipLaunchKernelGGL(k_bcg_beta, gm, b, 0, ctx->stream, m, st);
+            UHIP_CHECK_LAUNCH();
+            if (nk > 0) {
+                UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_bcg_pupdate<KK>), gv, b, 0, ctx->stream, n, rowseg, st.beta, p,
+                                                     z));
+                UHIP_CHECK_LAUNCH();
+            }
+            return UHIP_OK;
+        default:
+            return set_error(UHIP_ERR_INVALID, "uhip_bcg_step: stage %d", stage);
+    }
+}
+
+int uhip_seg_scale(uhip_ctx ctx, int32_t n, const int32_t *rowseg, const double *norm, const int32_t *done,
+                   const double *w, double *v) {
+    UHIP_REQUIRE(ctx && rowseg && norm && w && v, "uhip_seg_scale: null argument");
+    if (n == 0) return UHIP_OK;
+    hipLaunchKernelGGL(k_seg_scale, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, rowseg, norm, done, w, v);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_pattern_grad(uhip_ctx ctx, const uhip_csr *A, int k, const double *lambda, const double *x, double *grad) {
+    UHIP_REQUIRE(ctx && A && lambda && x && grad, "uhip_pattern_grad: null argument");
+    const int32_t n = A->nrows;
+    if (n == 0) return UHIP_OK;
+    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_pattern_grad<KK>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                                         A->rowptr, A->col, lambda, x, grad));
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+}  // extern "C"

@@ -166,7 +166,6 @@ int64_t uhip_gen_prolongation_nnz(int32_t nx, int32_t ny, int32_t nz) {
 
 int uhip_gen_prolongation(int32_t nx, int32_t ny, int32_t nz, int32_t *rowptr, int32_t *col, double *val) {
     if (nx <= 0 || ny <= 0 || nz <= 0 || !rowptr || !col || !val) return UHIP_ERR_INVALID;
-    if ((nx > 1 && nx % 2) || (ny > 1 && ny % 2) || (nz > 1 && nz % 2)) return UHIP_ERR_INVALID;
     const int cx = nx > 1 ? nx / 2 : 1, cy = ny > 1 ? ny / 2 : 1, cz = nz > 1 ? nz / 2 : 1;
     int64_t e = 0;
     rowptr[0] = 0;

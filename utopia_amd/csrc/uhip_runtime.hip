@@ -229,4 +229,39 @@ int uhip_event_timer_stop(uhip_ctx ctx, void *ev_pair, float *ms) {
     return UHIP_OK;
 }
 
+int uhip_ctx_reserve(uhip_ctx ctx, int64_t per_slot) {
+    UHIP_REQUIRE(ctx && per_slot >= 0, "uhip_ctx_reserve: bad argument");
+    return ensure_partials(ctx, per_slot);
+}
+
+int uhip_graph_begin(uhip_ctx ctx) {
+    UHIP_REQUIRE(ctx && ctx->stream, "uhip_graph_begin: context needs its own (non-null) stream");
+    UHIP_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    return UHIP_OK;
+}
+
+int uhip_graph_end(uhip_ctx ctx, void **exec) {
+    UHIP_REQUIRE(ctx && exec, "uhip_graph_end: null argument");
+    *exec = nullptr;
+    hipGraph_t g = nullptr;
+    UHIP_CHECK_HIP(hipStreamEndCapture(ctx->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return set_error(UHIP_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    *exec = ge;
+    return UHIP_OK;
+}
+
+int uhip_graph_launch(uhip_ctx ctx, void *exec) {
+    UHIP_REQUIRE(ctx && exec, "uhip_graph_launch: null argument");
+    UHIP_CHECK_HIP(hipGraphLaunch((hipGraphExec_t)exec, ctx->stream));
+    return UHIP_OK;
+}
+
+int uhip_graph_destroy(void *exec) {
+    if (exec) UHIP_CHECK_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+    return UHIP_OK;
+}
+
 }  // extern "C"

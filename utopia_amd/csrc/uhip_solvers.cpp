@@ -21,6 +21,7 @@ struct uhip_solver_s {
     std::shared_ptr<LinearSolver<HipCrsMatrix, HipVector>> solver;
     std::shared_ptr<ConjugateGradient<HipCrsMatrix, HipVector>> cg;
     std::shared_ptr<Multigrid> mg;
+    std::shared_ptr<MixedPrecisionMultigrid> mixed;
     std::shared_ptr<MPRGP> mprgp;
     std::shared_ptr<ProjectedGaussSeidel> pgs;
     HipVector lb, ub;
@@ -99,7 +100,8 @@ void build(uhip_solver_s *s) {
                 mg->max_it(1);
                 s->mg = mg;
                 if (s->type == UHIP_SOLVER_CG_MG_MIXED) {
-                    cg->set_preconditioner(std::make_shared<MixedPrecisionMultigrid>(mg));
+                    s->mixed = std::make_shared<MixedPrecisionMultigrid>(mg);
+                    cg->set_preconditioner(s->mixed);
                     cg->refine(param(s, "refine", 1) != 0);
                 } else {
                     cg->set_preconditioner(mg);
@@ -368,7 +370,8 @@ int uhip_solver_get_mask(uhip_solver s, int l, double *mask) {
 int uhip_solver_set_level_eigs(uhip_solver s, int l, double eig_max, double eig_min) {
     if (!s || !s->mg || l < 1 || l >= s->mg->n_levels()) return UHIP_ERR_INVALID;
     try {
-        s->mg->set_level_eigs(l, eig_max, eig_min);
+        if (s->mixed) s->mixed->set_level_eigs(l, eig_max, eig_min);
+        else s->mg->set_level_eigs(l, eig_max, eig_min);
         return UHIP_OK;
     } catch (const std::exception &e) {
         return fail(e);
