@@ -67,7 +67,10 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
             y = ua.DeviceArray(ctx, n_rows, np.float32)
         ms = time_kernel(ua, ctx, lambda: D.spmv(x, y), reps)
         nbytes = spmv_bytes(n_rows, A.nnz, vb)
-        out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+        ty = "double" if dt is np.float64 else "float"
+        label = (f"k_spmv_sell<{ty}, EpiStore<{ty}>> (SELL-64)" if D.format == ua.FMT_SELL64
+                 else f"k_spmv_rowblock<{ty}, EpiStore<{ty}>> (row-block CSR)")
+        out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9, "label": label}
         del D
     # streaming probe: w = 2x - 3y on 3 x 1.6 GB vectors
     m = 200_000_000
@@ -78,7 +81,8 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
     ua.check(L.uhip_vec_set_f64(ctx.h, m, 2.0, ctypes.c_void_p(b.ptr)))
     ms = time_kernel(ua, ctx, lambda: L.uhip_vec_waxpby_f64(ctx.h, m, 2.0, ctypes.c_void_p(a.ptr), -3.0,
                                                            ctypes.c_void_p(b.ptr), ctypes.c_void_p(w.ptr)), reps)
-    out["stream_waxpby_f64"] = {"ms": ms, "bytes": 24 * m, "gbs": 24 * m / (ms * 1e-3) / 1e9}
+    out["stream_waxpby_f64"] = {"ms": ms, "bytes": 24 * m, "gbs": 24 * m / (ms * 1e-3) / 1e9,
+                                "label": "k_waxpby<double> (w = a x + b y, 200M entries)"}
     for v in (a, b, w):
         v.free()
     return out
@@ -143,6 +147,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=128)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the SpMV / streaming kernel timings (for counter collection runs)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design "
@@ -159,6 +165,12 @@ def main(argv=None):
     interps = ua.mg_hierarchy(n, n, n, a.levels)
     N, nnz = A.shape[0], A.nnz
     log(f"generated {n}^3 variable-coefficient operator: N={N} nnz={nnz} in {time.perf_counter() - t0:.1f} s")
+    if a.kernels_only:
+        kern = kernel_rooflines(ua, ctx, A, N)
+        for k, v in kern.items():
+            log(f"kernel {k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s, {v['bytes']} algorithmic bytes, {v.get('label', '')}")
+        print(json.dumps({"kernels": kern}))
+        return 0
     atol = 1e-10 * float(np.linalg.norm(b))
     bd = ua.DeviceArray.from_numpy(ctx, b)
     xd = ua.DeviceArray(ctx, N)
@@ -244,7 +256,7 @@ def main(argv=None):
                   "refinements": mixed["refinements"], "setup_s": mixed["setup_s"],
                   "rel_residual": mixed["rel_residual"], "rel_error_vs_fp64": err_mixed},
         "speedup_mixed_vs_fp64": fp64["s_per_solve"] / mixed["s_per_solve"],
-        "roofline": {"bound": "hbm", "kernel": "spmv_csr_f64 (k_spmv_rowblock, EpiStore<double>)",
+        "roofline": {"bound": "hbm", "kernel": f"spmv_csr_f64: {sp['label']}",
                      "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": sp["bytes"], "avg_launch_ms": sp["ms"]},
         "kernels": {k: {"gbs": v["gbs"], "ms": v["ms"], "frac": v["gbs"] / HBM_PEAK_GBS} for k, v in kern.items()},

@@ -20,7 +20,8 @@
 
 namespace utopia {
 
-// Backend tag next to PETSC=10, TRILINOS=10000, BLAS=1 (utopia_core/utopia_Traits.hpp:12-21).
+// Backend tag beside BLAS=1, PETSC=10, TRILINOS=10000, KOKKOS=100000, SERIAL_HOMEMADE=200000
+// (utopia_core/utopia_Traits.hpp:12-21).
 enum { HIP = 1000000 };
 
 class HipError : public std::runtime_error {
