@@ -38,6 +38,13 @@ def spmv_bytes(n, nnz, val_bytes):
     return (val_bytes + 4) * nnz + (4 + 2 * val_bytes) * n + 4
 
 
+def dia_bytes(n, ndiag, val_bytes):
+    # DIA: ndiag value slots per row (absent entries stored as zeros), the per-row
+    # mask (1/2/4 B for <= 8/16/32 offsets), x read once, y written; offsets table
+    mb = 1 if ndiag <= 8 else (2 if ndiag <= 16 else 4)
+    return val_bytes * ndiag * n + mb * n + 2 * val_bytes * n + 4 * ndiag
+
+
 def time_kernel(ua, ctx, launch, reps):
     L = ua.lib()
     for _ in range(3):
@@ -66,11 +73,17 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
             x = ua.DeviceArray.from_numpy(ctx, ua.gen_uniform(5, n_rows).astype(np.float32))
             y = ua.DeviceArray(ctx, n_rows, np.float32)
         ms = time_kernel(ua, ctx, lambda: D.spmv(x, y), reps)
-        nbytes = spmv_bytes(n_rows, A.nnz, vb)
+        csr_b = spmv_bytes(n_rows, A.nnz, vb)
         ty = "double" if dt is np.float64 else "float"
-        label = (f"k_spmv_sell<{ty}, EpiStore<{ty}>> (SELL-64)" if D.format == ua.FMT_SELL64
-                 else f"k_spmv_rowblock<{ty}, EpiStore<{ty}>> (row-block CSR)")
-        out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9, "label": label}
+        if D.format == ua.FMT_DIA:
+            nbytes = dia_bytes(n_rows, D.m.ndiag, vb)
+            label = f"k_spmv_dia<{ty}, EpiStore<{ty}>> (DIA, {D.m.ndiag} offsets, per-row mask)"
+        elif D.format == ua.FMT_SELL64:
+            nbytes, label = csr_b, f"k_spmv_sell<{ty}, EpiStore<{ty}>> (SELL-64)"
+        else:
+            nbytes, label = csr_b, f"k_spmv_rowblock<{ty}, EpiStore<{ty}>> (row-block CSR)"
+        out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9, "label": label,
+                     "csr_bytes": csr_b, "csr_equivalent_gbs": csr_b / (ms * 1e-3) / 1e9}
         del D
     # streaming probe: w = 2x - 3y on 3 x 1.6 GB vectors
     m = 200_000_000
@@ -224,11 +237,15 @@ def main(argv=None):
         cpu = cpu_baseline(n, results["mixed"]["iterations"], a.cpu_sample_n, a.seed)
         log(f"cpu baseline ({time.perf_counter() - t0:.1f} s): {cpu['value']:.1f} s estimated, {cpu['sample']}")
 
+    # HBM bytes per launch from the committed counter runs (FETCH_SIZE / WRITE_SIZE),
+    # used only when they were taken on the kernel measured here
     traffic = None
     tf = ROOT / "profiles" / "spmv_csr_f64_varcoef256_traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+            tj = json.loads(tf.read_text())
+            if tj.get("label") == kern["spmv_csr_f64"]["label"] and tj.get("n") == n:
+                traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
 
@@ -259,7 +276,9 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "kernel": f"spmv_csr_f64: {sp['label']}",
                      "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": sp["bytes"], "avg_launch_ms": sp["ms"]},
-        "kernels": {k: {"gbs": v["gbs"], "ms": v["ms"], "frac": v["gbs"] / HBM_PEAK_GBS} for k, v in kern.items()},
+        "kernels": {k: {"gbs": v["gbs"], "ms": v["ms"], "frac": v["gbs"] / HBM_PEAK_GBS, "label": v["label"],
+                        "bytes": v["bytes"], "csr_equivalent_gbs": v.get("csr_equivalent_gbs")}
+                    for k, v in kern.items()},
         "cpu_baseline": cpu,
         "host": host_info(),
     }

@@ -160,11 +160,17 @@ typedef struct uhip_csr {
     int32_t *sell_col;  /* SELL-64: column-major within a slice, padded with zeros */
     void *sell_val;
     int32_t nparts;     /* workgroups of one SpMV launch (partial sums of fused dots) */
+    int32_t ndiag;      /* DIA: distinct column offsets (col - row), <= 32 */
+    int32_t *dia_off;   /* DIA: ndiag offsets, ascending (device) */
+    void *dia_val;      /* DIA: ndiag x nrows values, diagonal-major, zero where absent */
+    void *dia_mask;     /* DIA: per-row bit d set iff the row stores offset d (uint8 / uint16 / uint32
+                         * for ndiag <= 8 / 16 / 32) */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)
 #define UHIP_FMT_ROWBLOCK 0
 #define UHIP_FMT_SELL64 1
+#define UHIP_FMT_DIA 2
 
 /* Build the row-block plan. block_nnz selects the LDS budget (1024, 2048 or 4096
  * nonzeros per block; 0 = automatic format choice, see uhip_csr_plan_format).
@@ -172,9 +178,13 @@ typedef struct uhip_csr {
 int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
 /* Plan with an explicit format. UHIP_FMT_SELL64 copies the matrix into slices
  * of 64 rows stored column-major (one wavefront per slice, every load
- * coalesced); UHIP_FMT_AUTO takes SELL-64 when its padding adds at most 25%
- * to nnz and the row-block CSR kernel otherwise. Each y_i is summed in column
- * order in either format. */
+ * coalesced). UHIP_FMT_DIA stores a square matrix with sorted rows and at most
+ * 32 distinct offsets col - row as one value array per offset plus a per-row
+ * bitmask (no column indices; fails with UHIP_ERR_UNSUPPORTED otherwise).
+ * UHIP_FMT_AUTO takes DIA when it applies with at most 25% padding, else
+ * SELL-64 when its padding adds at most 25% to nnz, else the row-block CSR
+ * kernel. Each y_i is the sum of exactly the row's CSR entries in column
+ * order in every format. */
 int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz);
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A);
 

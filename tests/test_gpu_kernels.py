@@ -94,13 +94,83 @@ def test_spmv_sell64_fp64_bit_exact(ctx, name):
 
 
 def test_auto_format_choice(ctx):
-    # regular stencils take SELL-64; a matrix with one long row per slice keeps row-block CSR
-    assert ua.DeviceCsr(ctx, ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 16, 16, seed=1)).format == ua.FMT_SELL64
+    # stencils (few distinct offsets col - row) take DIA; block-diagonal systems of
+    # different grid sizes (many offsets) take SELL-64; a matrix with one long row
+    # per slice and rectangular transfers keep row-block CSR
+    D = ua.DeviceCsr(ctx, ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 16, 16, seed=1))
+    assert D.format == ua.FMT_DIA and D.m.ndiag == 7
+    assert ua.DeviceCsr(ctx, ua.gen_stencil(ua.STENCIL_2D9, 30, 20, shift=1e-3)).m.ndiag == 9
+    from utopia_amd import batched as bt
+    assert ua.DeviceCsr(ctx, bt.make_batch([9, 13, 17, 21, 25, 29, 33, 37, 41, 45, 49, 53, 57, 61, 65, 69, 73],
+                                           levels=1).A).format == ua.FMT_SELL64
     assert ua.DeviceCsr(ctx, MATRICES["rows_up_to_900"]()).format == ua.FMT_ROWBLOCK
     assert ua.DeviceCsr(ctx, ua.gen_prolongation(16, 16, 16)).format == ua.FMT_ROWBLOCK
 
 
-@pytest.mark.parametrize("fmt", [ua.FMT_ROWBLOCK, ua.FMT_SELL64])
+def _galerkin(n):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=2)
+    return ua.Csr(*orc.ptap(A, ua.gen_prolongation(n, n, n)))
+
+
+DIA_MATRICES = {
+    "2d5": lambda: ua.gen_stencil(ua.STENCIL_2D5, 37, 29),
+    "3d7_varcoef": lambda: ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 21, 17, 12, seed=5),
+    "2d9_shift": lambda: ua.gen_stencil(ua.STENCIL_2D9, 33, 31, shift=1e-3),
+    "galerkin27": lambda: _galerkin(18),
+    "edgecoef_odd": lambda: ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 45, 13, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DIA_MATRICES))
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_spmv_dia_bit_exact(ctx, name, dt):
+    A = DIA_MATRICES[name]()
+    n = A.shape[0]
+    D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
+    assert D.format == ua.FMT_DIA
+    x = ua.gen_uniform(17, n)
+    if dt is np.float32:
+        x = x.astype(np.float32)
+        ref = orc.spmv_f32(A, x)
+    else:
+        ref = orc.spmv(A, x)
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    yd = ua.DeviceArray(ctx, n, dt)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), ref)
+
+
+def test_dia_skips_absent_entries_exactly(ctx):
+    # x holds inf at a column that a row's padded (absent) diagonal slot would touch:
+    # the masked sum must equal the CSR sum (finite wherever the CSR row avoids it)
+    A = ua.gen_stencil(ua.STENCIL_2D5, 8, 8)
+    x = ua.gen_uniform(3, 64)
+    x[8] = np.inf  # row 7 (end of grid line 0) has no entry at column 8, offset +1 is absent there
+    D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_DIA)
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    yd = ua.DeviceArray(ctx, 64)
+    D.spmv(xd, yd)
+    y = yd.numpy()
+    ref = orc.spmv(A, x)
+    assert np.isfinite(y[7]) and np.array_equal(np.isfinite(y), np.isfinite(ref))
+    assert np.array_equal(y[np.isfinite(ref)], ref[np.isfinite(ref)])
+
+
+def test_dia_rejects_unsorted_and_rectangular(ctx):
+    A = ua.gen_stencil(ua.STENCIL_2D5, 6, 6)
+    col = A.col.copy()
+    col[A.rowptr[10]], col[A.rowptr[10] + 1] = col[A.rowptr[10] + 1], col[A.rowptr[10]]
+    val = A.val.copy()
+    val[A.rowptr[10]], val[A.rowptr[10] + 1] = val[A.rowptr[10] + 1], val[A.rowptr[10]]
+    U = ua.Csr(A.rowptr, col, val, A.shape)
+    with pytest.raises(ua.HipError):
+        ua.DeviceCsr(ctx, U, fmt=ua.FMT_DIA)
+    assert ua.DeviceCsr(ctx, U).format != ua.FMT_DIA  # automatic choice falls back
+    with pytest.raises(ua.HipError):
+        ua.DeviceCsr(ctx, ua.gen_prolongation(8, 8), fmt=ua.FMT_DIA)
+
+
+@pytest.mark.parametrize("fmt", [ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA])
 def test_fused_epilogues_both_formats_with_dot(ctx, fmt):
     A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 20, 18, 13, seed=9)
     n = A.shape[0]

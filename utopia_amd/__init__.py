@@ -22,7 +22,7 @@ import numpy as np
 from ._abi import LibraryMissing, lib, uhip_cheb_step, uhip_csr  # noqa: F401
 
 F64, F32 = 0, 1
-FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64 = -1, 0, 1
+FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64, FMT_DIA = -1, 0, 1, 2
 _NP = {F64: np.float64, F32: np.float32}
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
@@ -212,7 +212,7 @@ class DeviceCsr:
     """uhip_csr on the device with its row-block plan."""
 
     def __init__(self, ctx: Context, A: Csr, dtype=np.float64, block_nnz: int = 0, fmt: Optional[int] = None):
-        """fmt: None (block_nnz > 0: row-block CSR, 0: automatic), or FMT_AUTO / FMT_ROWBLOCK / FMT_SELL64."""
+        """fmt: None (block_nnz > 0: row-block CSR, 0: automatic), or FMT_AUTO / FMT_ROWBLOCK / FMT_SELL64 / FMT_DIA."""
         self.ctx = ctx
         self.shape = A.shape
         self._rp = DeviceArray.from_numpy(ctx, A.rowptr.astype(np.int32))

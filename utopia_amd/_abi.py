@@ -37,6 +37,10 @@ class uhip_csr(ctypes.Structure):
         ("sell_col", ctypes.c_void_p),
         ("sell_val", ctypes.c_void_p),
         ("nparts", ctypes.c_int32),
+        ("ndiag", ctypes.c_int32),
+        ("dia_off", ctypes.c_void_p),
+        ("dia_val", ctypes.c_void_p),
+        ("dia_mask", ctypes.c_void_p),
     ]
 
 

@@ -374,7 +374,10 @@ public:
         m_.slice_off = nullptr;
         m_.sell_col = nullptr;
         m_.sell_val = nullptr;
-        m_.nblocks = m_.nslices = m_.nparts = 0;
+        m_.dia_off = nullptr;
+        m_.dia_val = nullptr;
+        m_.dia_mask = nullptr;
+        m_.nblocks = m_.nslices = m_.nparts = m_.ndiag = 0;
         m_.format = UHIP_FMT_ROWBLOCK;
         plan(block_nnz);
     }
@@ -464,7 +467,7 @@ inline HipCrsMatrixT<float> HipCrsMatrixT<double>::cast_f32() const {
     hip_check(uhip_memcpy_dtod(c(), m.rowptr, m_.rowptr, sizeof(int32_t) * ((size_t)m_.nrows + 1)), "copy rowptr");
     hip_check(uhip_memcpy_dtod(c(), m.col, m_.col, sizeof(int32_t) * (size_t)m_.nnz), "copy col");
     hip_check(uhip_csr_cast_values_f64_f32(c(), m_.nnz, (const double *)m_.val, (float *)m.val), "cast vals");
-    o.adopt(ctx_, m, m_.format == UHIP_FMT_SELL64 ? 0 : (m_.block_capk == 5 ? 1024 : (m_.block_capk == 20 ? 4096 : 2048)));
+    o.adopt(ctx_, m, m_.format != UHIP_FMT_ROWBLOCK ? 0 : (m_.block_capk == 5 ? 1024 : (m_.block_capk == 20 ? 4096 : 2048)));
     return o;
 }
 

@@ -1,5 +1,8 @@
 // Row-block plan, SpMV family (MatMult and its fused callers), and the
 // per-row matrix utilities (diagonal, masks, semidefinite fix).
+#include <algorithm>
+#include <vector>
+
 #include <hipcub/hipcub.hpp>
 
 #include "uhip_spmv.hpp"
@@ -104,6 +107,153 @@ static int plan_sell(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     return UHIP_OK;
 }
 
+// ---- DIA plan ---------------------------------------------------------------
+// Distinct offsets col - row are collected in a 64-slot open-addressing set per
+// workgroup (LDS), then merged into one global 64-slot set. flags bit 0: a row
+// is not strictly increasing in column; bit 1: more than 64 distinct offsets.
+constexpr int kDiaSlots = 64;
+constexpr int32_t kDiaEmpty = INT32_MIN;
+
+__device__ __forceinline__ int dia_hash(int32_t key) { return (int)(((uint32_t)key * 2654435761u) >> 26); }
+
+__global__ __launch_bounds__(kBlock) void k_dia_init(int32_t *__restrict__ table, int32_t *__restrict__ flags) {
+    if (threadIdx.x < kDiaSlots) table[threadIdx.x] = kDiaEmpty;
+    if (threadIdx.x == 0) *flags = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dia_scan(int32_t n, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col, int32_t *__restrict__ gtable,
+                                                     int32_t *__restrict__ gflags) {
+    __shared__ int32_t t[kDiaSlots];
+    __shared__ int32_t bad;
+    if (threadIdx.x < kDiaSlots) t[threadIdx.x] = kDiaEmpty;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r < n) {
+        const int32_t a = rowptr[r], e1 = rowptr[r + 1];
+        int32_t prev = INT32_MIN;
+        for (int32_t e = a; e < e1; ++e) {
+            const int32_t c = col[e];
+            if (e > a && c <= prev) atomicOr(&bad, 1);
+            prev = c;
+            const int32_t key = c - r;
+            const int h = dia_hash(key);
+            bool placed = false;
+            for (int p = 0; p < kDiaSlots && !placed; ++p) {
+                const int slot = (h + p) & (kDiaSlots - 1);
+                const int32_t cur = t[slot];
+                if (cur == key) {
+                    placed = true;
+                } else if (cur == kDiaEmpty) {
+                    const int32_t old = atomicCAS(&t[slot], kDiaEmpty, key);
+                    placed = (old == kDiaEmpty || old == key);
+                }
+            }
+            if (!placed) atomicOr(&bad, 2);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kDiaSlots && t[threadIdx.x] != kDiaEmpty) {
+        const int32_t key = t[threadIdx.x];
+        const int h = dia_hash(key);
+        bool placed = false;
+        for (int p = 0; p < kDiaSlots && !placed; ++p) {
+            const int slot = (h + p) & (kDiaSlots - 1);
+            const int32_t cur = __hip_atomic_load(&gtable[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key) {
+                placed = true;
+            } else if (cur == kDiaEmpty) {
+                const int32_t old = atomicCAS(&gtable[slot], kDiaEmpty, key);
+                placed = (old == kDiaEmpty || old == key);
+            }
+        }
+        if (!placed) atomicOr(gflags, 2);
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(gflags, bad);
+}
+
+// Row r writes all ndiag slots of its diagonals (value or zero) and its mask.
+template <typename T, typename TM>
+__global__ __launch_bounds__(kBlock) void k_dia_fill(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
+                                                     const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                     T *__restrict__ dval, TM *__restrict__ mask) {
+    __shared__ int32_t so[32];
+    if (threadIdx.x < ndiag) so[threadIdx.x] = off[threadIdx.x];
+    __syncthreads();
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    int32_t e = rowptr[r];
+    const int32_t e1 = rowptr[r + 1];
+    uint32_t m = 0;
+    for (int d = 0; d < ndiag; ++d) {
+        T v = T(0);
+        if (e < e1 && col[e] - r == so[d]) {
+            v = val[e];
+            m |= 1u << d;
+            ++e;
+        }
+        dval[(int64_t)d * n + r] = v;
+    }
+    mask[r] = (TM)m;
+}
+
+// Plan DIA arrays; UHIP_ERR_UNSUPPORTED (nothing allocated) when the matrix is
+// not square, has unsorted or duplicate columns in a row, more than 32 distinct
+// offsets, or ndiag * n > max_ratio * nnz.
+static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
+    hipStream_t s = ctx->stream;
+    const int32_t n = A->nrows;
+    if (n != A->ncols || n == 0) return UHIP_ERR_UNSUPPORTED;
+    int32_t *table = nullptr;
+    UHIP_CHECK_HIP(hipMalloc(&table, sizeof(int32_t) * (kDiaSlots + 1)));
+    int32_t *flags = table + kDiaSlots;
+    hipLaunchKernelGGL(k_dia_init, dim3(1), dim3(kBlock), 0, s, table, flags);
+    UHIP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_dia_scan, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr, A->col, table,
+                       flags);
+    UHIP_CHECK_LAUNCH();
+    int32_t h[kDiaSlots + 1];
+    UHIP_CHECK_HIP(hipMemcpyAsync(h, table, sizeof(h), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> offs;
+    for (int i = 0; i < kDiaSlots; ++i)
+        if (h[i] != kDiaEmpty) offs.push_back(h[i]);
+    std::sort(offs.begin(), offs.end());
+    const int32_t nd = (int32_t)offs.size();
+    if (h[kDiaSlots] != 0 || nd < 1 || nd > 32 || (double)nd * (double)n > max_ratio * (double)A->nnz) {
+        hipFree(table);
+        return UHIP_ERR_UNSUPPORTED;
+    }
+    const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
+    const size_t mb = nd <= 8 ? 1 : (nd <= 16 ? 2 : 4);
+    UHIP_CHECK_HIP(hipMemcpyAsync(table, offs.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
+    UHIP_CHECK_HIP(hipMalloc(&A->dia_val, vb * (size_t)nd * (size_t)n));
+    UHIP_CHECK_HIP(hipMalloc(&A->dia_mask, mb * (size_t)n));
+    const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+#define UHIP_DIA_FILL(T, TM)                                                                                      \
+    hipLaunchKernelGGL((k_dia_fill<T, TM>), g, b, 0, s, n, nd, table, A->rowptr, A->col, (const T *)A->val,      \
+                       (T *)A->dia_val, (TM *)A->dia_mask)
+    if (A->dtype == UHIP_F64) {
+        if (mb == 1) UHIP_DIA_FILL(double, uint8_t);
+        else if (mb == 2) UHIP_DIA_FILL(double, uint16_t);
+        else UHIP_DIA_FILL(double, uint32_t);
+    } else {
+        if (mb == 1) UHIP_DIA_FILL(float, uint8_t);
+        else if (mb == 2) UHIP_DIA_FILL(float, uint16_t);
+        else UHIP_DIA_FILL(float, uint32_t);
+    }
+#undef UHIP_DIA_FILL
+    UHIP_CHECK_LAUNCH();
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    A->dia_off = table;  // first nd slots hold the sorted offsets
+    A->ndiag = nd;
+    A->format = UHIP_FMT_DIA;
+    A->nparts = (n + kDiaWG - 1) / kDiaWG;
+    return UHIP_OK;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ col, const T *__restrict__ val,
@@ -177,7 +327,8 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz) {
 int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz) {
     UHIP_REQUIRE(ctx && A && A->rowptr, "uhip_csr_plan: null argument");
     UHIP_REQUIRE(A->nrows >= 0 && A->ncols >= 0, "uhip_csr_plan: bad shape");
-    UHIP_REQUIRE(format == UHIP_FMT_AUTO || format == UHIP_FMT_ROWBLOCK || format == UHIP_FMT_SELL64,
+    UHIP_REQUIRE(format == UHIP_FMT_AUTO || format == UHIP_FMT_ROWBLOCK || format == UHIP_FMT_SELL64 ||
+                     format == UHIP_FMT_DIA,
                  "uhip_csr_plan: format %d", format);
     if (block_nnz == 0) block_nnz = 2048;
     int capk;
@@ -190,6 +341,14 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
     hipStream_t s = ctx->stream;
     A->block_capk = capk;
     A->format = UHIP_FMT_ROWBLOCK;
+    if (format == UHIP_FMT_DIA || (format == UHIP_FMT_AUTO && n > 0 && A->nnz > 0)) {
+        const int st = plan_dia(ctx, A, format == UHIP_FMT_DIA ? 1e30 : 1.25);
+        if (st == UHIP_OK) return UHIP_OK;
+        if (st != UHIP_ERR_UNSUPPORTED) return st;
+        if (format == UHIP_FMT_DIA)
+            return set_error(UHIP_ERR_UNSUPPORTED,
+                             "uhip_csr_plan: DIA needs a square matrix with sorted rows and <= 32 distinct offsets");
+    }
     if (n > 0 && A->nnz > 0 && format != UHIP_FMT_ROWBLOCK) {
         const int st = plan_sell(ctx, A, format == UHIP_FMT_SELL64 ? 1e30 : 1.25);
         if (st == UHIP_OK) return UHIP_OK;
@@ -235,9 +394,16 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
 
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     UHIP_REQUIRE(A, "uhip_csr_plan_destroy: null matrix");
-    if (A->blocks || A->slice_off || A->sell_col || A->sell_val) {
+    if (A->blocks || A->slice_off || A->sell_col || A->sell_val || A->dia_off || A->dia_val || A->dia_mask) {
         if (ctx) UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     }
+    if (A->dia_off) UHIP_CHECK_HIP(hipFree(A->dia_off));
+    if (A->dia_val) UHIP_CHECK_HIP(hipFree(A->dia_val));
+    if (A->dia_mask) UHIP_CHECK_HIP(hipFree(A->dia_mask));
+    A->dia_off = nullptr;
+    A->dia_val = nullptr;
+    A->dia_mask = nullptr;
+    A->ndiag = 0;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));

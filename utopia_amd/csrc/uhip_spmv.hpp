@@ -145,11 +145,79 @@ __global__ __launch_bounds__(64 * kSellWaves) void k_spmv_sell(int32_t nrows, in
     }
 }
 
+// DIA: one thread per row; diagonal d of row r holds A[r, r + off[d]] at
+// val[d * n + r] (coalesced across the wavefront, no column indices), and bit d
+// of mask[r] says whether the row stores that entry. The row sum visits the
+// offsets in ascending order, i.e. the row's CSR entries in column order, and
+// skips absent ones, so it equals the sequential CSR sum bit for bit.
+constexpr int kDiaWG = 256;
+
+template <typename TAcc, class Epi, typename TA, typename TX, typename TM>
+__global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
+                                                     const TA *__restrict__ val, const TM *__restrict__ mask,
+                                                     const TX *__restrict__ x, Epi epi, double *__restrict__ partials,
+                                                     int64_t stride) {
+    constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int32_t row = b * kDiaWG + threadIdx.x;
+    double part[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) part[d] = 0.0;
+    if (row < nrows) {
+        const uint32_t m = (uint32_t)mask[row];
+        const TA *v = val + row;
+        TAcc acc = TAcc(0);
+        int d = 0;
+        for (; d + 4 <= ndiag; d += 4) {
+            TA vi[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) vi[u] = ld_stream(v + (int64_t)(d + u) * nrows);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if ((m >> (d + u)) & 1u) acc += (TAcc)vi[u] * (TAcc)x[row + off[d + u]];
+        }
+        for (; d < ndiag; ++d) {
+            const TA vd = ld_stream(v + (int64_t)d * nrows);
+            if ((m >> d) & 1u) acc += (TAcc)vd * (TAcc)x[row + off[d]];
+        }
+        epi(row, acc, part);
+    }
+    if constexpr (Epi::kDots > 0) {
+#pragma unroll
+        for (int d = 0; d < Epi::kDots; ++d) {
+            const double t = block_sum<kDiaWG>(part[d]);
+            if (threadIdx.x == 0) partials[d * stride + b] = t;
+        }
+    }
+}
+
 // Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
 // Matrix values have the accumulation type TAcc.
 template <typename TAcc, typename TX, class Epi>
 int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi) {
     using TA = TAcc;
+    if (A && A->format == UHIP_FMT_DIA) {
+        UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->ndiag >= 1 && A->ndiag <= 32,
+                     "spmv: DIA plan missing");
+        if (A->nrows == 0) return UHIP_OK;
+        if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
+        const dim3 g(A->nparts), blk(kDiaWG);
+        const TA *v = (const TA *)A->dia_val;
+        if (A->ndiag <= 8)
+            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint8_t>), g, blk, 0, stream_of(ctx), A->nrows,
+                               A->ndiag, A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, ctx->partials,
+                               ctx->max_partials);
+        else if (A->ndiag <= 16)
+            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint16_t>), g, blk, 0, stream_of(ctx), A->nrows,
+                               A->ndiag, A->dia_off, v, (const uint16_t *)A->dia_mask, x, epi, ctx->partials,
+                               ctx->max_partials);
+        else
+            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint32_t>), g, blk, 0, stream_of(ctx), A->nrows,
+                               A->ndiag, A->dia_off, v, (const uint32_t *)A->dia_mask, x, epi, ctx->partials,
+                               ctx->max_partials);
+        UHIP_CHECK_LAUNCH();
+        return UHIP_OK;
+    }
     if (A && A->format == UHIP_FMT_SELL64) {
         UHIP_REQUIRE(A->slice_off && A->sell_col && A->sell_val, "spmv: SELL-64 plan missing");
         if (A->nslices == 0) return UHIP_OK;
