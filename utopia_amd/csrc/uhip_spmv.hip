@@ -187,6 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int32_t n, int32_t ndiag, c
     int32_t e = rowptr[r];
     const int32_t e1 = rowptr[r + 1];
     uint32_t m = 0;
+    const int64_t ld = dia_ld(n);
     for (int d = 0; d < ndiag; ++d) {
         T v = T(0);
         if (e < e1 && col[e] - r == so[d]) {
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int32_t n, int32_t ndiag, c
             m |= 1u << d;
             ++e;
         }
-        dval[(int64_t)d * n + r] = v;
+        dval[(int64_t)d * ld + r] = v;
     }
     mask[r] = (TM)m;
 }
@@ -229,7 +230,7 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
     const size_t mb = nd <= 8 ? 1 : (nd <= 16 ? 2 : 4);
     UHIP_CHECK_HIP(hipMemcpyAsync(table, offs.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
-    UHIP_CHECK_HIP(hipMalloc(&A->dia_val, vb * (size_t)nd * (size_t)n));
+    UHIP_CHECK_HIP(hipMalloc(&A->dia_val, vb * (size_t)nd * (size_t)dia_ld(n)));
     UHIP_CHECK_HIP(hipMalloc(&A->dia_mask, mb * (size_t)n));
     const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
 #define UHIP_DIA_FILL(T, TM)                                                                                      \
@@ -250,7 +251,7 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->format = UHIP_FMT_DIA;
-    A->nparts = (n + kDiaWG - 1) / kDiaWG;
+    A->nparts = (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
     return UHIP_OK;
 }
 

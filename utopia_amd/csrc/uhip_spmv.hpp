@@ -145,12 +145,24 @@ __global__ __launch_bounds__(64 * kSellWaves) void k_spmv_sell(int32_t nrows, in
     }
 }
 
-// DIA: one thread per row; diagonal d of row r holds A[r, r + off[d]] at
-// val[d * n + r] (coalesced across the wavefront, no column indices), and bit d
-// of mask[r] says whether the row stores that entry. The row sum visits the
-// offsets in ascending order, i.e. the row's CSR entries in column order, and
-// skips absent ones, so it equals the sequential CSR sum bit for bit.
+// DIA: diagonal d of row r holds A[r, r + off[d]] at val[d * ld + r]
+// (ld = nrows rounded up to 64; no column indices), and bit d of mask[r] says
+// whether the row stores that entry. A wavefront owns kDiaRows * 64
+// consecutive rows; lane l takes rows base + 64 i + l, so every value, mask and
+// x load instruction of the wave reads one contiguous segment. All loads of a
+// group of diagonals are issued before the products (absent entries read x[0])
+// and absent products are discarded by a select, so each row sum adds exactly
+// the row's CSR entries in column order: bit-identical to the sequential CSR sum.
 constexpr int kDiaWG = 256;
+// rows per thread
+template <typename TA>
+struct DiaRows {
+    static constexpr int R = sizeof(TA) == 8 ? 2 : 4;  // rows per thread
+    static constexpr int G = sizeof(TA) == 8 ? 8 : 4;  // diagonals whose loads are in flight together
+};
+inline int32_t dia_rows_per_wg(int dtype) { return kDiaWG * (dtype == UHIP_F64 ? DiaRows<double>::R : DiaRows<float>::R); }
+
+__host__ __device__ constexpr int32_t dia_ld(int32_t n) { return (n + 63) & ~63; }
 
 template <typename TAcc, class Epi, typename TA, typename TX, typename TM>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
@@ -158,29 +170,54 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
                                                      const TX *__restrict__ x, Epi epi, double *__restrict__ partials,
                                                      int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
+    constexpr int R = DiaRows<TA>::R;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int32_t row = b * kDiaWG + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int32_t base = (b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6)) * (64 * R);
+    const int64_t ld = dia_ld(nrows);
     double part[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) part[d] = 0.0;
-    if (row < nrows) {
-        const uint32_t m = (uint32_t)mask[row];
-        const TA *v = val + row;
-        TAcc acc = TAcc(0);
-        int d = 0;
-        for (; d + 4 <= ndiag; d += 4) {
-            TA vi[4];
+    if (base < nrows) {
+        int32_t row[R];
+        uint32_t m[R];
+        TAcc acc[R];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) vi[u] = ld_stream(v + (int64_t)(d + u) * nrows);
+        for (int i = 0; i < R; ++i) {
+            row[i] = base + 64 * i + lane;
+            m[i] = (row[i] < nrows) ? (uint32_t)mask[row[i]] : 0u;
+            acc[i] = TAcc(0);
+        }
+        constexpr int G = DiaRows<TA>::G;
+        for (int d = 0; d < ndiag; d += G) {
+            TA vv[G][R];
+            TX xv[G][R];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if ((m >> (d + u)) & 1u) acc += (TAcc)vi[u] * (TAcc)x[row + off[d + u]];
+            for (int u = 0; u < G; ++u) {
+                if (d + u < ndiag) {
+                    const TA *vd = val + (int64_t)(d + u) * ld;
+                    const int32_t o = off[d + u];
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        vv[u][i] = ld_stream(vd + min(row[i], (int32_t)ld - 1));
+                        xv[u][i] = x[((m[i] >> (d + u)) & 1u) ? row[i] + o : 0];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (d + u < ndiag) {
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const TAcc p = (TAcc)vv[u][i] * (TAcc)xv[u][i];
+                        acc[i] = ((m[i] >> (d + u)) & 1u) ? acc[i] + p : acc[i];
+                    }
+                }
+            }
         }
-        for (; d < ndiag; ++d) {
-            const TA vd = ld_stream(v + (int64_t)d * nrows);
-            if ((m >> d) & 1u) acc += (TAcc)vd * (TAcc)x[row + off[d]];
-        }
-        epi(row, acc, part);
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (row[i] < nrows) epi(row[i], acc[i], part);
     }
     if constexpr (Epi::kDots > 0) {
 #pragma unroll
