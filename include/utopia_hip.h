@@ -135,6 +135,16 @@ int uhip_vec_axpy_cast(uhip_ctx ctx, int64_t n, double a, const void *p, int p_d
 int uhip_cg_update_f64(uhip_ctx ctx, int64_t n, double *x, const double *p, const double *r, const double *Ap,
                        double *r_new, double *scalars, int i_num, int i_den, int i_alpha, int i_rnorm);
 int uhip_cg_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *z, double *scalars, int i_num, int i_den);
+/* The same iteration with the x update moved into the p update's pass
+ * (ConjugateGradient::preconditioned_solve, solvers/smoothers/utopia_ConjugateGradient_impl.hpp:237-266;
+ * identical arithmetic, x_{k+1} = x_k + alpha_k p_k is formed before p_k is replaced):
+ * cg_residual:  alpha = s[i_num]/s[i_den]; r_new = r - alpha Ap (r_new32 = (float)r_new if given);
+ *               s[i_alpha] = alpha; s[i_rnorm] = |r_new|_2  (non-finite alpha: nothing written but s[i_alpha])
+ * cg_xpupdate:  x += s[i_alpha] p; if z: p = z + (s[i_num]/s[i_den]) p, then s[i_den] = s[i_num] */
+int uhip_cg_residual_f64(uhip_ctx ctx, int64_t n, const double *r, const double *Ap, double *r_new, float *r_new32,
+                         double *scalars, int i_num, int i_den, int i_alpha, int i_rnorm);
+int uhip_cg_xpupdate_f64(uhip_ctx ctx, int64_t n, double *x, double *p, const double *z, double *scalars, int i_alpha,
+                         int i_num, int i_den);
 
 /* ------------------------------------------------------------------------
  * CSR matrices. Replaces PetscMatrix (AIJ) (backend/petsc/utopia_petsc_Matrix.hpp:103-1028).

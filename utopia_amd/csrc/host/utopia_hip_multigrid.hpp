@@ -239,9 +239,13 @@ private:
 // z = M rhs for a V-cycle engine, plus dot_d = z . rhs (fused into the last
 // kernel when possible).
 template <typename T>
-inline void vcycle_apply_dot(VCycle<T> &eng, const HipVector &rhs, HipVector &z, double *dot_d) {
-    const bool fused = eng.cycle((int)eng.levels.size() - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64, rhs.data(),
-                                 dot_d);
+inline void vcycle_apply_dot(VCycle<T> &eng, const HipVector &rhs, HipVector &z, double *dot_d,
+                             const float *rhs32 = nullptr) {
+    // rhs32 = (float)rhs: an FP32 cycle reads it in place of rhs (same values after
+    // the conversion every FP32 kernel applies), the dot still uses the FP64 rhs
+    const void *r = rhs32 ? (const void *)rhs32 : (const void *)rhs.data();
+    const bool fused = eng.cycle((int)eng.levels.size() - 1, r, rhs32 ? UHIP_F32 : UHIP_F64, z.data(), UHIP_F64,
+                                 rhs.data(), dot_d);
     if (!fused) {
         const double *a[1] = {z.data()}, *b[1] = {rhs.data()};
         hip_check(uhip_vec_dots_f64(z.ctx(), z.size(), 1, a, b, dot_d), "dot(z, r)");
@@ -491,6 +495,16 @@ public:
         vcycle_apply_dot(eng_, rhs, z, dot_d);
         return true;
     }
+    float *rhs_f32_buffer(int64_t n) override {
+        if (r32_.size() != n || r32_.empty()) r32_ = HipVector32(this->op_->context(), n);
+        return r32_.data();
+    }
+    bool apply_zero_initial_guess_dot_f32(const HipVector &rhs, const float *rhs32, HipVector &z,
+                                          double *dot_d) override {
+        if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
+        vcycle_apply_dot(eng_, rhs, z, dot_d, rhs32);
+        return true;
+    }
     const Multigrid &fp64() const { return *mg_; }
     // Chebyshev bounds of level l on both the FP64 hierarchy and its FP32 copy
     void set_level_eigs(int l, double emax, double emin) {
@@ -503,6 +517,7 @@ public:
 private:
     std::shared_ptr<Multigrid> mg_;
     VCycle<float> eng_;
+    HipVector32 r32_;
 };
 
 }  // namespace utopia

@@ -65,6 +65,13 @@ public:
                       "dot(z, r)");
         return ok;
     }
+    // FP32 copy of the right-hand side that a reduced-precision solver reads
+    // instead of rhs (nullptr: not wanted). A caller that fills it with
+    // (float)rhs passes it to apply_zero_initial_guess_dot_f32.
+    virtual float *rhs_f32_buffer(int64_t) { return nullptr; }
+    virtual bool apply_zero_initial_guess_dot_f32(const Vector &rhs, const float *, Vector &sol, double *dot_d) {
+        return apply_zero_initial_guess_dot(rhs, sol, dot_d);
+    }
     bool solve(const Matrix &A, const Vector &b, Vector &x) {
         update(std::shared_ptr<const Matrix>(&A, [](const Matrix *) {}));
         return apply(b, x);
@@ -220,27 +227,34 @@ private:
         const double *rz[1] = {r_.data()}, *zz[1] = {z_.data()};
         hip_check(uhip_vec_dots_f64(c, b.size(), 1, rz, zz, s + S_RZ), "dot(r,z)");
         bool stop = false;
+        float *r32 = precond_->rhs_f32_buffer(b.size());
         while (!stop) {
             hip_check(uhip_spmv_dot(c, &A.raw(), p_.data(), Ap_.data(), s + S_PAP), "Ap, dot(p,Ap)");
-            hip_check(uhip_cg_update_f64(c, b.size(), x.data(), p_.data(), r_.data(), Ap_.data(), r_new_.data(), s,
-                                         S_RZ, S_PAP, S_ALPHA, S_RNORM),
-                      "cg update");
+            // alpha, r_new = r - alpha Ap, |r_new|; x += alpha p is applied with the p update
+            hip_check(uhip_cg_residual_f64(c, b.size(), r_.data(), Ap_.data(), r_new_.data(), r32, s, S_RZ, S_PAP,
+                                           S_ALPHA, S_RNORM),
+                      "cg residual");
             double h[2];
             A.context()->read_scalars(s + S_ALPHA, 2, h);
             const double alpha = h[0];
-            if (std::isinf(alpha) || std::isnan(alpha)) {
+            if (std::isinf(alpha) || std::isnan(alpha)) {  // stops before touching x
                 stop = this->check_convergence(it, r_norm, 1, 1);
                 break;
             }
             r_norm = h[1];
             if (r_norm <= this->atol_) {
+                hip_check(uhip_cg_xpupdate_f64(c, b.size(), x.data(), p_.data(), nullptr, s, S_ALPHA, 0, 0),
+                          "x update");
                 this->print_iter(it, r_norm);
                 stop = this->check_convergence(it, r_norm, 1, 1);
                 break;
             }
             // z_new.set(0); precond->apply(r_new, z_new); dot(z_new, r_new)
-            precond_->apply_zero_initial_guess_dot(r_new_, z_new_, s + S_RZNEW);
-            hip_check(uhip_cg_pupdate_f64(c, b.size(), p_.data(), z_new_.data(), s, S_RZNEW, S_RZ), "p update");
+            if (r32) precond_->apply_zero_initial_guess_dot_f32(r_new_, r32, z_new_, s + S_RZNEW);
+            else precond_->apply_zero_initial_guess_dot(r_new_, z_new_, s + S_RZNEW);
+            // x += alpha p; p = z_new + beta p
+            hip_check(uhip_cg_xpupdate_f64(c, b.size(), x.data(), p_.data(), z_new_.data(), s, S_ALPHA, S_RZNEW, S_RZ),
+                      "x, p update");
             std::swap(r_, r_new_);
             std::swap(z_, z_new_);
             this->print_iter(it, r_norm);

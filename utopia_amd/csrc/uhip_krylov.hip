@@ -51,6 +51,45 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, double *__rest
         p[i] = z[i] + beta * p[i];
 }
 
+// alpha = s[i_num]/s[i_den]; rn = r - alpha*Ap (and rn32 = (float)rn when
+// given); partial |rn|^2. x is left to k_cg_xpupdate. Block 0 stores alpha.
+__global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double *__restrict__ r,
+                                                        const double *__restrict__ Ap, double *__restrict__ rn,
+                                                        float *__restrict__ rn32, double *__restrict__ s, int i_num,
+                                                        int i_den, int i_alpha, double *__restrict__ partials) {
+    const double alpha = s[i_num] / s[i_den];
+    if (!isfinite(alpha)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) s[i_alpha] = alpha;
+        return;
+    }
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double v = r[i] - alpha * Ap[i];
+        rn[i] = v;
+        if (rn32) rn32[i] = (float)v;
+        acc += v * v;
+    }
+    acc = block_sum<kBlock>(acc);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = acc;
+        if (blockIdx.x == 0) s[i_alpha] = alpha;
+    }
+}
+
+// x += s[i_alpha]*p, then (z given) p = z + beta*p with beta = s[i_num]/s[i_den]:
+// the x update of iteration k uses p_k before it is replaced by p_{k+1}.
+__global__ __launch_bounds__(kBlock) void k_cg_xpupdate(int64_t n, double *__restrict__ x, double *__restrict__ p,
+                                                        const double *__restrict__ z, const double *__restrict__ s,
+                                                        int i_alpha, int i_num, int i_den) {
+    const double alpha = s[i_alpha];
+    const double beta = z ? s[i_num] / s[i_den] : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double pi = p[i];
+        x[i] = x[i] + alpha * pi;
+        if (z) p[i] = z[i] + beta * pi;
+    }
+}
+
 __global__ void k_scalar_copy(double *s, int dst, int src) { s[dst] = s[src]; }
 
 }  // namespace uhip
@@ -108,6 +147,32 @@ int uhip_cg_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *z, dou
     }
     hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, ctx->stream, scalars, i_den, i_num);
     UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_cg_residual_f64(uhip_ctx ctx, int64_t n, const double *r, const double *Ap, double *r_new, float *r_new32,
+                         double *scalars, int i_num, int i_den, int i_alpha, int i_rnorm) {
+    UHIP_REQUIRE(ctx && r && Ap && r_new && scalars, "uhip_cg_residual_f64: null argument");
+    const int g = grid_for(n, 4, kMaxRedBlocks);
+    UHIP_TRY(ensure_partials(ctx, g));
+    hipLaunchKernelGGL(k_cg_residual, dim3(g), dim3(kBlock), 0, ctx->stream, n, r, Ap, r_new, r_new32, scalars, i_num,
+                       i_den, i_alpha, ctx->partials);
+    UHIP_CHECK_LAUNCH();
+    return finish_sums(ctx, 1, g, ctx->max_partials, scalars + i_rnorm, 1u);
+}
+
+int uhip_cg_xpupdate_f64(uhip_ctx ctx, int64_t n, double *x, double *p, const double *z, double *scalars, int i_alpha,
+                         int i_num, int i_den) {
+    UHIP_REQUIRE(ctx && x && p && scalars, "uhip_cg_xpupdate_f64: null argument");
+    if (n > 0) {
+        hipLaunchKernelGGL(k_cg_xpupdate, dim3(grid_for(n, 4, 1 << 16)), dim3(kBlock), 0, ctx->stream, n, x, p, z,
+                           scalars, i_alpha, i_num, i_den);
+        UHIP_CHECK_LAUNCH();
+    }
+    if (z) {
+        hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, ctx->stream, scalars, i_den, i_num);
+        UHIP_CHECK_LAUNCH();
+    }
     return UHIP_OK;
 }
 
