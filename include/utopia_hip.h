@@ -389,7 +389,9 @@ int uhip_lsolve_set_verbosity(uhip_solver s, int level);
  * ---------------------------------------------------------------------- */
 /* op 0: Y = A X; op 1: Y += A X; op 2: Y = B - A X (B dtype b_dtype).
  * Each column is the single-vector SpMV of PetscMatrix::multiply
- * (backend/petsc/utopia_petsc_Matrix.cpp:1137-1152) with per-row sums in entry order. */
+ * (backend/petsc/utopia_petsc_Matrix.cpp:1137-1152) with per-row sums in entry order,
+ * except for matrices averaging >= 32 entries per row (dense blocks), whose rows
+ * are summed by one wavefront each in a fixed tree order. */
 int uhip_spmm(uhip_ctx ctx, const uhip_csr *A, int k, const void *X, int x_dtype, void *Y, int y_dtype, int op,
               const void *B, int b_dtype);
 /* One Chebyshev3level step (solvers/smoothers/utopia_Chebyshev3level.hpp:150-175)

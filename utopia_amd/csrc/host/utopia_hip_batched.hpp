@@ -70,7 +70,7 @@ struct SegLayout {
     std::vector<int32_t> seg;  // nseg + 1 row offsets
     DeviceInts rowseg, c_row0, c_row1, seg_c0;
     int32_t nchunks = 0;
-    void build(const HipContextPtr &ctx, const std::vector<int32_t> &offsets, int32_t chunk_rows = 8192) {
+    void build(const HipContextPtr &ctx, const std::vector<int32_t> &offsets, int32_t chunk_rows = 1024) {
         seg = offsets;
         const int32_t ns = (int32_t)seg.size() - 1;
         std::vector<int32_t> rs((size_t)seg.back()), r0, r1, c0((size_t)ns + 1, 0);

@@ -2,42 +2,96 @@
 // as one block-diagonal CSR matrix (segments of rows), K right-hand sides per
 // system stored row-major (entry (i, j) at i*K + j).
 //
-//   SpMM      one K-lane group per row, lane j owns right-hand side j; the
-//             row's values/columns are read once for all K columns and the
-//             gathered x entries of a column index are K contiguous values.
+//   SpMM      one thread per row owns all K right-hand sides; the row's
+//             values/columns are read once and the gathered x entries of a
+//             column index are K contiguous values.
 //   segdot    per (segment, column) dot products, two fixed-order stages.
 //   batched CG / Chebyshev steps with per-(segment, column) step lengths.
 #include "uhip_common.hpp"
 
 namespace uhip {
 
+// SpMM: one thread per row holds all K sums; each stored entry's value and
+// column are read once and the K x values of that column are one contiguous
+// K*sizeof(TX)-byte segment. EB entries' loads are issued before their
+// products are added (in entry order, so each column's sum is the sequential
+// CSR sum).
+template <int K, typename TX>
+__device__ __forceinline__ const TX *xrow(const TX *X, int32_t c) {
+    const TX *p = X + (int64_t)c * K;
+    if constexpr (K * sizeof(TX) >= 16) return (const TX *)__builtin_assume_aligned(p, 16);
+    return p;
+}
+
 template <int K, typename TAcc, class Epi, typename TA, typename TX>
 __global__ __launch_bounds__(256) void k_spmm_csr(int32_t n, const int32_t *__restrict__ rowptr,
                                                   const int32_t *__restrict__ col, const TA *__restrict__ val,
                                                   const TX *__restrict__ X, Epi epi) {
-    constexpr int RPB = 256 / K;
+    constexpr int EB = K >= 16 ? 2 : 4;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int32_t r = b * RPB + threadIdx.x / K;
-    const int j = threadIdx.x % K;
+    const int32_t r = b * 256 + (int32_t)threadIdx.x;
     if (r >= n) return;
-    TAcc acc = TAcc(0);
+    TAcc acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = TAcc(0);
+    int32_t e = rowptr[r];
     const int32_t e1 = rowptr[r + 1];
-    for (int32_t e = rowptr[r]; e < e1; ++e) acc += (TAcc)val[e] * (TAcc)X[(int64_t)col[e] * K + j];
-    epi(r, j, acc);
+    for (; e < e1; e += EB) {
+        int32_t c[EB];
+        TA v[EB];
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
+            c[u] = e + u < e1 ? col[e + u] : 0;
+            v[u] = e + u < e1 ? val[e + u] : TA(0);
+        }
+        TX xv[EB][K];
+#pragma unroll
+        for (int u = 0; u < EB; ++u) {
+            const TX *xr = xrow<K>(X, c[u]);
+#pragma unroll
+            for (int j = 0; j < K; ++j) xv[u][j] = xr[j];
+        }
+#pragma unroll
+        for (int u = 0; u < EB; ++u)
+            if (e + u < e1) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[j] += (TAcc)v[u] * (TAcc)xv[u][j];
+            }
+    }
+    epi(r, acc);
 }
 
+// Row r's K entries of a row-major [n x K] array, 16-byte aligned when the row
+// spans at least 16 bytes (hipMalloc'd base).
+template <int K, typename T>
+__device__ __forceinline__ T *krow(T *base, int32_t r) {
+    T *p = base + (int64_t)r * K;
+    if constexpr (K * sizeof(T) >= 16) return (T *)__builtin_assume_aligned(p, 16);
+    return p;
+}
+
+// Epilogues read all K inputs of the row before writing any output, so an
+// output may alias an input of the same row (in-place updates).
 template <int K, typename TAcc, typename TY>
 struct EpiKStore {
     TY *y;
-    __device__ void operator()(int32_t r, int j, TAcc s) const { y[(int64_t)r * K + j] = (TY)s; }
+    __device__ void operator()(int32_t r, const TAcc *s) const {
+        TY *yr = krow<K>(y, r);
+#pragma unroll
+        for (int j = 0; j < K; ++j) yr[j] = (TY)s[j];
+    }
 };
 
 template <int K, typename TAcc, typename TY>
 struct EpiKAdd {
     TY *y;
-    __device__ void operator()(int32_t r, int j, TAcc s) const {
-        const int64_t i = (int64_t)r * K + j;
-        y[i] = (TY)((TAcc)y[i] + s);
+    __device__ void operator()(int32_t r, const TAcc *s) const {
+        TY *yr = krow<K>(y, r);
+        TY t[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) t[j] = yr[j];
+#pragma unroll
+        for (int j = 0; j < K; ++j) yr[j] = (TY)((TAcc)t[j] + s[j]);
     }
 };
 
@@ -45,9 +99,14 @@ template <int K, typename TAcc, typename TB, typename TY>
 struct EpiKResidual {
     const TB *b;
     TY *y;
-    __device__ void operator()(int32_t r, int j, TAcc s) const {
-        const int64_t i = (int64_t)r * K + j;
-        y[i] = (TY)((TAcc)b[i] - s);
+    __device__ void operator()(int32_t r, const TAcc *s) const {
+        const TB *br = krow<K>(b, r);
+        TB t[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) t[j] = br[j];
+        TY *yr = krow<K>(y, r);
+#pragma unroll
+        for (int j = 0; j < K; ++j) yr[j] = (TY)((TAcc)t[j] - s[j]);
     }
 };
 
@@ -64,33 +123,89 @@ struct EpiKCheb {
     const int32_t *rowseg;
     const double *coef;
     int last;
-    __device__ void operator()(int32_t r, int j, TAcc s) const {
-        const int64_t i = (int64_t)r * K + j;
+    __device__ void operator()(int32_t r, const TAcc *s) const {
         const int32_t g = rowseg[r];
         const TAcc a_prev = (TAcc)coef[3 * g], beta = (TAcc)coef[3 * g + 1], a_cur = (TAcc)coef[3 * g + 2];
-        const TAcc pp = (TAcc)pprev[i];
-        const TAcc rp = rprev ? rprev[i] : -pp;
-        const TAcc rk = rp + a_prev * s;
-        const TAcc pk = -rk + beta * pp;
-        TAcc xv = xin ? xin[i] : TAcc(0);
-        xv = xv + a_prev * pp;
-        if (last) {
-            xv = xv + a_cur * pk;
+        TAcc pp[K], rp[K], xv[K];
+        const TP *ppr = krow<K>(pprev, r);
+#pragma unroll
+        for (int j = 0; j < K; ++j) pp[j] = (TAcc)ppr[j];
+        if (rprev) {
+            const TAcc *rpr = krow<K>(rprev, r);
+#pragma unroll
+            for (int j = 0; j < K; ++j) rp[j] = rpr[j];
         } else {
-            rout[i] = rk;
-            pout[i] = pk;
+#pragma unroll
+            for (int j = 0; j < K; ++j) rp[j] = -pp[j];
         }
-        xout[i] = (TXO)xv;
+        if (xin) {
+            const TAcc *xr = krow<K>(xin, r);
+#pragma unroll
+            for (int j = 0; j < K; ++j) xv[j] = xr[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; ++j) xv[j] = TAcc(0);
+        }
+        TAcc rk[K], pk[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            rk[j] = rp[j] + a_prev * s[j];
+            pk[j] = -rk[j] + beta * pp[j];
+            xv[j] = xv[j] + a_prev * pp[j];
+        }
+        if (last) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) xv[j] = xv[j] + a_cur * pk[j];
+        } else {
+            TAcc *ro = krow<K>(rout, r), *po = krow<K>(pout, r);
+#pragma unroll
+            for (int j = 0; j < K; ++j) ro[j] = rk[j];
+#pragma unroll
+            for (int j = 0; j < K; ++j) po[j] = pk[j];
+        }
+        TXO *xo = krow<K>(xout, r);
+#pragma unroll
+        for (int j = 0; j < K; ++j) xo[j] = (TXO)xv[j];
     }
 };
 
+// Long rows (dense coarse blocks): one wavefront per row, lane l takes entries
+// l, l+64, ... in order, and a fixed butterfly combines the lanes.
+template <int K, typename TAcc, class Epi, typename TA, typename TX>
+__global__ __launch_bounds__(256) void k_spmm_wave(int32_t n, const int32_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ col, const TA *__restrict__ val,
+                                                   const TX *__restrict__ X, Epi epi) {
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int32_t r = b * 4 + (int32_t)(threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (r >= n) return;
+    TAcc acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = TAcc(0);
+    const int32_t e1 = rowptr[r + 1];
+    for (int32_t e = rowptr[r] + l; e < e1; e += 64) {
+        const TAcc v = (TAcc)val[e];
+        const TX *xr = xrow<K>(X, col[e]);
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] += v * (TAcc)xr[j];
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = wave_sum(acc[j]);
+    if (l == 0) epi(r, acc);
+}
+
+// Rows averaging >= 32 entries take the wavefront-per-row kernel (its sums are
+// tree-ordered); all others the row-per-thread kernel (sequential CSR order).
 template <int K, typename TAcc, class Epi, typename TA, typename TX>
 int launch_spmm(uhip_ctx ctx, const uhip_csr *A, const TX *X, const Epi &e) {
     const int32_t n = A->nrows;
     if (n == 0) return UHIP_OK;
-    constexpr int RPB = 256 / K;
-    hipLaunchKernelGGL((k_spmm_csr<K, TAcc, Epi, TA, TX>), dim3((n + RPB - 1) / RPB), dim3(256), 0, ctx->stream, n,
-                       A->rowptr, A->col, (const TA *)A->val, X, e);
+    if (A->nnz >= 32LL * n)
+        hipLaunchKernelGGL((k_spmm_wave<K, TAcc, Epi, TA, TX>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n,
+                           A->rowptr, A->col, (const TA *)A->val, X, e);
+    else
+        hipLaunchKernelGGL((k_spmm_csr<K, TAcc, Epi, TA, TX>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                           A->rowptr, A->col, (const TA *)A->val, X, e);
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
 }
@@ -121,15 +236,26 @@ __global__ __launch_bounds__(256) void k_segdot_chunk(const int32_t *__restrict_
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_segdot_finish(int32_t nseg, const int32_t *__restrict__ seg_c0,
-                                                       const double *__restrict__ partials, double *__restrict__ out,
-                                                       int take_sqrt) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= nseg * K) return;
-    const int s = t / K, j = t % K;
-    double acc = 0.0;
-    for (int32_t c = seg_c0[s]; c < seg_c0[s + 1]; ++c) acc += partials[(int64_t)c * K + j];
-    out[t] = take_sqrt ? sqrt(acc) : acc;
+// One wavefront per segment: lane l sums the segment's chunks l, l+64, ... in
+// order, then a fixed butterfly combines the lanes (deterministic).
+__global__ __launch_bounds__(64) void k_segdot_finish(int32_t nseg, const int32_t *__restrict__ seg_c0,
+                                                      const double *__restrict__ partials, double *__restrict__ out,
+                                                      int take_sqrt) {
+    const int s = blockIdx.x;
+    const int l = threadIdx.x;
+    const int32_t c0 = seg_c0[s], c1 = seg_c0[s + 1];
+    double acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = 0.0;
+    for (int32_t c = c0 + l; c < c1; c += 64) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] += partials[(int64_t)c * K + j];
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const double t = wave_sum(acc[j]);
+        if (l == 0) out[(int64_t)s * K + j] = take_sqrt ? sqrt(t) : t;
+    }
 }
 
 // ---- batched CG on per-(segment, column) scalars ----
@@ -379,8 +505,8 @@ int uhip_segdot(uhip_ctx ctx, int k, int32_t nchunks, const int32_t *c_row0, con
     }
     UHIP_CHECK_LAUNCH();
     const int m = nseg * k;
-    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_finish<KK>), dim3((m + 255) / 256), dim3(256), 0, ctx->stream, nseg,
-                                         seg_c0, ctx->partials, out, take_sqrt));
+    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_finish<KK>), dim3(nseg), dim3(64), 0, ctx->stream, nseg, seg_c0,
+                                         ctx->partials, out, take_sqrt));
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
 }
