@@ -197,6 +197,14 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
  * order in every format. */
 int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz);
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A);
+/* Autotuner: times y = A x for every applicable plan (DIA, SELL-64, row-block CSR
+ * with 1024 / 2048 / 4096 nonzeros per workgroup), reps launches each, and keeps
+ * the fastest (format and best average ms returned). No reference analogue
+ * (PETSc's MatMult has one fixed AIJ kernel). */
+int uhip_csr_autotune(uhip_ctx ctx, uhip_csr *A, int reps, int32_t *format, float *ms);
+/* reps > 0: every UHIP_FMT_AUTO plan made through this context is chosen by the
+ * autotuner; 0 (default): by the padding rules of uhip_csr_plan_format. */
+int uhip_ctx_set_autotune(uhip_ctx ctx, int reps);
 
 /* y = A x                         MatMult (utopia_petsc_Matrix.cpp:1326-1358) */
 int uhip_spmv(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y);
@@ -348,8 +356,9 @@ int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out);
 int uhip_solver_destroy(uhip_solver s);
 /* read(Input&)-style parameters: atol rtol stol max_it verbose reset_initial_guess refine
  * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
- * smoother(0 Chebyshev3level, 1 Jacobi) scale_max_eig scale_min_eig power_method_tol power_method_max_it
- * block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each. Read at the next update. */
+ * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour) scale_max_eig scale_min_eig power_method_tol
+ * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each. Read at the next update.
+ * autotune (reps; 0 = off) applies to the SpMV plans of matrices built after it is set. */
 int uhip_solver_set_param(uhip_solver s, const char *key, double value);
 /* Multigrid::set_transfer_operators (utopia_LinearMultiLevel.hpp:65-79), host CSR, coarse to fine */
 int uhip_solver_set_transfer_operators(uhip_solver s, int n, const int32_t *nrows, const int32_t *ncols,
@@ -375,6 +384,8 @@ int uhip_solver_get_info(uhip_solver s, const char *key, double *value);
 int uhip_solver_get_level(uhip_solver s, int l, int32_t *rowptr, int32_t *col, double *val);
 int uhip_solver_get_mask(uhip_solver s, int l, double *mask);
 int uhip_solver_set_level_eigs(uhip_solver s, int l, double eig_max, double eig_min);
+/* colour of every row of level l (GaussSeidel smoother) and the number of colours */
+int uhip_solver_get_colors(uhip_solver s, int l, int32_t *color, int32_t *ncolors);
 int uhip_lsolve_set_max_iterations(uhip_solver s, int max_it);
 int uhip_lsolve_set_atol(uhip_solver s, double v);
 int uhip_lsolve_set_rtol(uhip_solver s, double v);

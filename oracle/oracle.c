@@ -473,6 +473,8 @@ struct or_mg {
     double *chol;
     int smoother, pre, post, mg_type, masks, fix_semi;
     double *r[OR_MAXL], *c[OR_MAXL], *cI[OR_MAXL], *rR[OR_MAXL];
+    int32_t *color[OR_MAXL]; /* multicolour Gauss-Seidel: colour of each row (smoother 2) */
+    int32_t ncolors[OR_MAXL];
 };
 
 or_mg *or_mg_create(int nlevels, const or_csr *const *interp, int smoother, int pre_steps, int post_steps,
@@ -503,6 +505,45 @@ void or_mg_eigs(const or_mg *mg, int l, double *emax, double *emin) {
 void or_mg_set_eigs(or_mg *mg, int l, double emax, double emin) {
     mg->eig_max[l] = emax;
     mg->eig_min[l] = emin;
+}
+
+void or_mg_set_colors(or_mg *mg, int l, const int32_t *color, int32_t ncolors) {
+    const int64_t n = mg->A[l].nrows;
+    free(mg->color[l]);
+    mg->color[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    memcpy(mg->color[l], color, sizeof(int32_t) * (size_t)n);
+    mg->ncolors[l] = ncolors;
+}
+
+/* GaussSeidel::smooth with local_sweeps (solvers/smoothers/utopia_GaussSeidel.hpp:45-57,133-183):
+ * per sweep r = rhs - A x; c = 0; one forward then (symmetric) one backward pass,
+ * c_i = d_inv_i (r_i - sum_{j != i} a_ij c_j); x += c. The row order of a pass is
+ * the colour order (colour 0 first going forward, last colour first going
+ * backward); with color[i] = i it is the reference's lexicographic order. */
+void or_gs_smooth(const or_csr *A, const double *dinv, const int32_t *color, int32_t ncolors, const double *rhs,
+                  double *x, int sweeps, int symmetric) {
+    const int64_t n = A->nrows;
+    double *r = vnew(n), *c = vnew(n);
+    for (int s = 0; s < sweeps; ++s) {
+        or_spmv(A, x, r);
+        for (int64_t i = 0; i < n; ++i) r[i] = rhs[i] - r[i];
+        for (int64_t i = 0; i < n; ++i) c[i] = 0.0;
+        for (int pass = 0; pass < (symmetric ? 2 : 1); ++pass) {
+            for (int32_t t = 0; t < ncolors; ++t) {
+                const int32_t k = pass == 0 ? t : ncolors - 1 - t;
+                for (int64_t i = 0; i < n; ++i) {
+                    if (color[i] != k) continue;
+                    double v = r[i];
+                    for (int32_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e)
+                        if (A->col[e] != i) v -= A->val[e] * c[A->col[e]];
+                    c[i] = dinv[i] * v;
+                }
+            }
+        }
+        for (int64_t i = 0; i < n; ++i) x[i] = x[i] + c[i];
+    }
+    free(r);
+    free(c);
 }
 
 /* LinearMultiLevel::fix_semidefinite_operator (solvers/multilevel/utopia_LinearMultiLevel.hpp:89-113) */
@@ -565,6 +606,12 @@ int or_mg_update(or_mg *mg, const or_csr *A) {
             const double lam = or_power_method(&mg->A[l], 1e-2, 30);
             mg->eig_max[l] = 1.2 * lam;
             mg->eig_min[l] = 0.06 * mg->eig_max[l];
+        } else if (mg->smoother == 2) {
+            /* GaussSeidel::init (utopia_GaussSeidel.hpp:203-214): d_inv = 1/d (0 kept for d = 0) */
+            free(mg->dinv[l]);
+            mg->dinv[l] = vnew(n);
+            diag_of(&mg->A[l], mg->dinv[l]);
+            for (int64_t i = 0; i < n; ++i) mg->dinv[l][i] = mg->dinv[l][i] != 0.0 ? 1.0 / mg->dinv[l][i] : 0.0;
         } else {
             /* Jacobi::update (solvers/smoothers/utopia_Jacobi.hpp:99-110) */
             free(mg->dinv[l]);
@@ -609,6 +656,8 @@ static void jacobi_smooth(const or_csr *A, const double *dinv, const double *rhs
 static void smoothing(or_mg *mg, int l, const double *rhs, double *x, int nu) {
     if (mg->smoother == 0)
         or_chebyshev_smooth(&mg->A[l], rhs, x, nu, mg->eig_max[l], mg->eig_min[l]);
+    else if (mg->smoother == 2)
+        or_gs_smooth(&mg->A[l], mg->dinv[l], mg->color[l], mg->ncolors[l], rhs, x, nu, 1);
     else
         jacobi_smooth(&mg->A[l], mg->dinv[l], rhs, x, nu);
 }
@@ -689,6 +738,7 @@ void or_mg_destroy(or_mg *mg) {
         free(mg->c[l]);
         free(mg->cI[l]);
         free(mg->rR[l]);
+        free(mg->color[l]);
     }
     free(mg->chol);
     free(mg);

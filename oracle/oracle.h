@@ -69,6 +69,10 @@ const double *or_mg_mask(const or_mg *mg, int l);
 void or_mg_eigs(const or_mg *mg, int l, double *eig_max, double *eig_min);
 /* Override the Chebyshev bounds of level l (tests comparing cycles bit-for-bit). */
 void or_mg_set_eigs(or_mg *mg, int l, double eig_max, double eig_min);
+/* smoother 2 (multicolour Gauss-Seidel): the colouring of level l, after or_mg_update */
+void or_mg_set_colors(or_mg *mg, int l, const int32_t *color, int32_t ncolors);
+void or_gs_smooth(const or_csr *A, const double *dinv, const int32_t *color, int32_t ncolors, const double *rhs,
+                  double *x, int sweeps, int symmetric);
 void or_mg_destroy(or_mg *mg);
 
 /* PowerMethod::get_max_eig (rand() initial vector) */

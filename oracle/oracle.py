@@ -60,6 +60,8 @@ def lib():
             "or_mg_mask": (P, [P, I]),
             "or_mg_eigs": (None, [P, I, P, P]),
             "or_mg_set_eigs": (None, [P, I, D, D]),
+            "or_mg_set_colors": (None, [P, I, P, ctypes.c_int32]),
+            "or_gs_smooth": (None, [P, P, P, ctypes.c_int32, P, P, I, I]),
             "or_mg_destroy": (None, [P]),
             "or_power_method": (D, [P, D, I64]),
             "or_chebyshev_smooth": (None, [P, P, P, I, D, D]),
@@ -255,6 +257,21 @@ class MG:
 
     def set_eigs(self, l, emax, emin):
         lib().or_mg_set_eigs(self.h, l, emax, emin)
+
+    def set_colors(self, l, color, ncolors):
+        color = np.ascontiguousarray(color, np.int32)
+        lib().or_mg_set_colors(self.h, l, _p(color), int(ncolors))
+
+
+def gs_smooth(A, dinv, color, ncolors, b, x0=None, sweeps=1, symmetric=True):
+    """Multicolour symmetric Gauss-Seidel sweeps (GaussSeidel::smooth)."""
+    M = Mat.of(A)
+    dinv = np.ascontiguousarray(dinv, np.float64)
+    color = np.ascontiguousarray(color, np.int32)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    lib().or_gs_smooth(M.ref, _p(dinv), _p(color), int(ncolors), _p(b), _p(x), sweeps, 1 if symmetric else 0)
+    return x
 
 
 def mprgp_solve(A, b, lb=None, ub=None, x0=None, atol=1e-9, rtol=0.0, max_it=100000, eps_eig_est=1e-1,

@@ -412,3 +412,37 @@ def test_graph_coloring_is_proper(ctx):
     off = rows != A.col
     assert not np.any(c[rows[off]] == c[A.col[off]])
     assert k.value <= 9  # at most max degree + 1 for the 9-point stencil
+
+
+def test_autotuner_picks_a_plan_and_keeps_results(ctx):
+    """The autotuner times every applicable layout and keeps one; row sums stay
+    the sequential CSR sums whichever it picks."""
+    for A in (ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 40, 32, 24, seed=3), MATRICES["rows_up_to_900"](),
+              ua.gen_prolongation(24, 24, 24)):
+        D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_ROWBLOCK, block_nnz=2048)
+        fmt = ctypes.c_int32()
+        ms = ctypes.c_float()
+        ua.check(L().uhip_csr_autotune(ctx.h, D.ref, 3, ctypes.byref(fmt), ctypes.byref(ms)))
+        assert fmt.value in (ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA) and fmt.value == D.format
+        assert 0 < ms.value < 100
+        x = ua.gen_uniform(5, A.shape[1])
+        xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, A.shape[0])
+        D.spmv(xd, yd)
+        y, ref = yd.numpy(), orc.spmv(A, x)
+        short = np.diff(A.rowptr) <= 256  # long rows may take the block reduction (tree order)
+        assert np.array_equal(y[short], ref[short])
+        assert np.allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_context_autotune_applies_to_automatic_plans(ctx):
+    c2 = ua.Context(0)
+    c2.set_autotune(2)
+    A = ua.gen_stencil(ua.STENCIL_2D9, 64, 48, shift=1e-3)
+    D = ua.DeviceCsr(c2, A)
+    assert D.format in (ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA)
+    x = ua.gen_uniform(2, A.shape[0])
+    xd, yd = ua.DeviceArray.from_numpy(c2, x), ua.DeviceArray(c2, A.shape[0])
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv(A, x))
+    del D, xd, yd
+    c2.close()

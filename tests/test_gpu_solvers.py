@@ -123,6 +123,71 @@ def test_mg_cycles_match_oracle_with_pinned_eigs(ctx, pre, post, mg_type):
     assert rel(x, xo) <= 1e-12
 
 
+@pytest.mark.parametrize("pre,post", [(1, 1), (2, 2)])
+def test_mg_gauss_seidel_smoother_matches_oracle(ctx, pre, post):
+    """Multicolour symmetric GaussSeidel smoothing in the V-cycle: GPU cycles equal
+    the oracle's with the same colouring (the GPU's, read back per level)."""
+    n = 16
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=6)
+    b = ua.gen_uniform(7, A.shape[0])
+    interps = ua.mg_hierarchy(n, n, n, 3)
+    s = ua.Multigrid(ctx, smoother=2, pre_smoothing_steps=pre, post_smoothing_steps=post, max_it=3, atol=0, rtol=0,
+                     stol=0)
+    s.set_transfer_operators(interps)
+    s.update(A)
+    mg = orc.MG(interps, smoother=2, pre=pre, post=post)
+    mg.update(A)
+    for l in (1, 2):
+        color, nc = s.colors(l)
+        assert 2 <= nc <= 27 and color.min() == 0 and color.max() == nc - 1
+        G = s.level(l).to_scipy().tocoo()
+        off = G.row != G.col
+        assert not np.any(color[G.row[off]] == color[G.col[off]])  # proper colouring
+        mg.set_colors(l, color, nc)
+    x = s.apply(b)
+    xo = mg.apply(b, max_it=3, atol=0, rtol=0)
+    assert rel(x, xo) <= 1e-12
+
+
+def test_cg_mg_gauss_seidel_converges_like_oracle(ctx):
+    n = 24
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=2)
+    b = ua.gen_uniform(102, A.shape[0])
+    interps = ua.mg_hierarchy(n, n, n, 3)
+    atol = 1e-10 * np.linalg.norm(b)
+    cg = ua.ConjugateGradient(ctx, "multigrid", atol=atol, rtol=0, stol=0, max_it=300, smoother=2,
+                              pre_smoothing_steps=1, post_smoothing_steps=1)
+    cg.set_transfer_operators(interps)
+    x = cg.solve(A, b)
+    mg = orc.MG(interps, smoother=2, pre=1, post=1)
+    mg.update(A)
+    for l in (1, 2):
+        mg.set_colors(l, *cg.colors(l))
+    xo, so = mg.cg_solve(b, atol=atol, max_it=300)
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-10
+    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.0001 * atol
+
+
+def test_autotuned_mg_pcg_matches_default_plans(ctx):
+    """Plans change the partial-sum layout of the fused dots, not the row sums:
+    the autotuned solve agrees with the default one to rounding."""
+    n = 24
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=4)
+    b = ua.gen_uniform(104, A.shape[0])
+    interps = ua.mg_hierarchy(n, n, n, 3)
+    atol = 1e-10 * np.linalg.norm(b)
+    xs = []
+    for tune in (0, 3):
+        ua.srand(1)
+        cg = ua.ConjugateGradient(ctx, "multigrid_mixed", atol=atol, rtol=0, stol=0, max_it=300, autotune=tune,
+                                  pre_smoothing_steps=2, post_smoothing_steps=2)
+        cg.set_transfer_operators(interps)
+        xs.append((cg.solve(A, b), cg.status["iterates"]))
+    assert abs(xs[0][1] - xs[1][1]) <= 1
+    assert rel(xs[1][0], xs[0][0]) <= 1e-10
+
+
 def test_cg_mg_reference_fixture(ctx, golden):
     """PetscLinearSolverTest::petsc_cg_mg on utopia/data/laplace/matrices_for_petsc."""
     d = golden / "utopia_data"

@@ -275,3 +275,62 @@ def test_mprgp_kkt_certificate():
     assert np.all(x >= lb)
     assert np.all(g[act] >= -1e-9) and np.abs(g[~act]).max() < 1e-8
     assert act.sum() > 0 and (~act).sum() > 0
+
+
+def _reference_gs_local_sweeps(A, r, dinv, order_fwd, symmetric=True):
+    """GaussSeidel::local_sweeps (solvers/smoothers/utopia_GaussSeidel.hpp:133-183) in
+    plain Python floats, visiting rows in order_fwd forward and reversed backward."""
+    n = A.shape[0]
+    c = [0.0] * n
+    passes = [order_fwd, list(reversed(order_fwd))] if symmetric else [order_fwd]
+    for order in passes:
+        for i in order:
+            s_ = float(r[i])
+            for e in range(A.rowptr[i], A.rowptr[i + 1]):
+                j = int(A.col[e])
+                if j != i:
+                    s_ -= float(A.val[e]) * c[j]
+            c[i] = float(dinv[i]) * s_
+    return np.array(c)
+
+
+def test_gauss_seidel_lexicographic_equals_reference_sweep():
+    """With one colour per row (color[i] = i) the multicolour sweep is the
+    reference's lexicographic symmetric Gauss-Seidel, bit for bit."""
+    A = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 7, 5, seed=4)
+    n = A.shape[0]
+    b = ua.gen_uniform(3, n)
+    d = A.to_scipy().diagonal()
+    dinv = 1.0 / d
+    x = orc.gs_smooth(A, dinv, np.arange(n), n, b, sweeps=1)
+    ref = _reference_gs_local_sweeps(A, b, dinv, list(range(n)))  # x0 = 0: r = b, x = c
+    assert np.array_equal(x, ref)
+
+
+def test_gauss_seidel_red_black_equals_colour_ordered_sweep():
+    """Red-black colouring of the 5-point grid: the sweep equals the reference
+    sweep over the rows listed colour by colour (order inside a colour is free)."""
+    nx, ny = 6, 7
+    A = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, nx, ny, seed=8)
+    n = A.shape[0]
+    color = np.array([(i % nx + i // nx) % 2 for i in range(n)], np.int32)
+    b = ua.gen_uniform(5, n)
+    dinv = 1.0 / A.to_scipy().diagonal()
+    x = orc.gs_smooth(A, dinv, color, 2, b, sweeps=1)
+    order = [i for i in range(n) if color[i] == 0] + [i for i in range(n) if color[i] == 1]
+    # backward pass: colour 1 then colour 0 (rows of one colour are independent)
+    ref = _reference_gs_local_sweeps(A, b, dinv, order)
+    assert np.array_equal(x, ref)
+    # two sweeps reduce the residual of the SPD system
+    x2 = orc.gs_smooth(A, dinv, color, 2, b, sweeps=2)
+    Asp = A.to_scipy()
+    assert np.linalg.norm(b - Asp @ x2) < np.linalg.norm(b - Asp @ x) < np.linalg.norm(b)
+
+
+def test_gauss_seidel_on_diagonal_matrix_is_jacobi():
+    n = 9
+    d = 1.0 + np.arange(n)
+    A = ua.Csr(np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), d, (n, n))
+    b = ua.gen_uniform(1, n)
+    x = orc.gs_smooth(A, 1.0 / d, np.zeros(n, np.int32), 1, b, sweeps=1)
+    assert np.array_equal(x, (1.0 / d) * b)

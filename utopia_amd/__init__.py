@@ -13,6 +13,7 @@ back to a CPU implementation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 from dataclasses import dataclass
 from typing import Optional, Sequence
@@ -20,6 +21,15 @@ from typing import Optional, Sequence
 import numpy as np
 
 from ._abi import LibraryMissing, lib, uhip_cheb_step, uhip_csr  # noqa: F401
+
+# Device objects still alive when the interpreter exits are left to the process
+# teardown: no HIP call is made from a finaliser after exit has begun.
+_EXITING = [False]
+atexit.register(lambda: _EXITING.__setitem__(0, True))
+
+
+def _alive() -> bool:
+    return not _EXITING[0]
 
 F64, F32 = 0, 1
 FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64, FMT_DIA = -1, 0, 1, 2
@@ -146,7 +156,8 @@ class Context:
 
     def __del__(self):
         try:
-            self.close()
+            if _alive():
+                self.close()
         except Exception:
             pass
 
@@ -156,6 +167,10 @@ class Context:
     @property
     def scalars(self) -> int:
         return lib().uhip_ctx_scalars(self.h)
+
+    def set_autotune(self, reps: int):
+        """reps > 0: automatic SpMV plans made through this context are chosen by timing."""
+        check(lib().uhip_ctx_set_autotune(self.h, int(reps)), "uhip_ctx_set_autotune")
 
     def read_scalars(self, addr: int, n: int) -> np.ndarray:
         out = np.zeros(n, np.float64)
@@ -202,7 +217,7 @@ class DeviceArray:
 
     def __del__(self):
         try:
-            if self.ctx.h:
+            if _alive() and self.ctx.h:
                 self.free()
         except Exception:
             pass
@@ -242,7 +257,7 @@ class DeviceCsr:
 
     def __del__(self):
         try:
-            if self.ctx.h:
+            if _alive() and self.ctx.h:
                 lib().uhip_csr_plan_destroy(self.ctx.h, ctypes.byref(self.m))
         except Exception:
             pass
@@ -293,7 +308,7 @@ class Solver:
 
     def __del__(self):
         try:
-            if self.h:
+            if _alive() and self.h:
                 lib().uhip_solver_destroy(self.h)
                 self.h = None
         except Exception:
@@ -393,6 +408,14 @@ class Solver:
 
     def set_level_eigs(self, l: int, eig_max: float, eig_min: float):
         check(lib().uhip_solver_set_level_eigs(self.h, l, eig_max, eig_min), "set_level_eigs")
+
+    def colors(self, l: int):
+        """(colour of every row, number of colours) of level l (GaussSeidel smoother)."""
+        n = int(self.info(f"level_rows_{l}"))
+        out = np.zeros(max(n, 1), np.int32)
+        nc = ctypes.c_int32()
+        check(lib().uhip_solver_get_colors(self.h, l, _ptr(out), ctypes.byref(nc)), "get_colors")
+        return out[:n], nc.value
 
 
 class ConjugateGradient(Solver):

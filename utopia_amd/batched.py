@@ -20,7 +20,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from . import (Context, Csr, DeviceArray, DeviceCsr, STENCIL_2D5_EDGECOEF, _ptr, check, gen_prolongation,
+from . import (Context, Csr, DeviceArray, DeviceCsr, STENCIL_2D5_EDGECOEF, _alive, _ptr, check, gen_prolongation,
                gen_stencil, gen_uniform)
 from ._abi import lib, uhip_csr
 
@@ -132,7 +132,7 @@ class BatchedSolver:
 
     def __del__(self):
         try:
-            if self.h:
+            if _alive() and self.h:
                 lib().uhip_bsolver_destroy(self.h)
                 self.h = None
         except Exception:

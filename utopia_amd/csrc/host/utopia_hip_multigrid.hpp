@@ -2,6 +2,8 @@
 // coarse solve, and the fused V-cycle engine (FP64 or FP32 levels).
 #pragma once
 
+#include <type_traits>
+
 #include <chrono>
 
 #include "utopia_hip_solvers.hpp"
@@ -130,6 +132,96 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// Rows of a matrix grouped by colour (Jones-Plassmann colouring with a fixed
+// seed): rows of one colour are not coupled, so a Gauss-Seidel sweep updates
+// a whole colour in parallel.
+// ---------------------------------------------------------------------------
+class ColorGroups {
+public:
+    static constexpr uint64_t kSeed = 0x5eedull;
+    ColorGroups() = default;
+    ColorGroups(const ColorGroups &) = delete;
+    ColorGroups &operator=(const ColorGroups &) = delete;
+    ~ColorGroups() { release(); }
+    void build(const HipCrsMatrix &A) {
+        release();
+        ctx_ = A.context();
+        const int64_t n = A.rows();
+        void *p = nullptr;
+        hip_check(uhip_malloc(A.c(), &p, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)), "alloc colors");
+        color_ = (int32_t *)p;
+        hip_check(uhip_malloc(A.c(), &p, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)), "alloc rows");
+        rows_ = (int32_t *)p;
+        hip_check(uhip_graph_color(A.c(), &A.raw(), kSeed, color_, &ncolors_), "coloring");
+        offsets_.assign((size_t)ncolors_ + 1, 0);
+        hip_check(uhip_color_groups(A.c(), (int32_t)n, color_, ncolors_, rows_, offsets_.data()), "colour groups");
+    }
+    int32_t ncolors() const { return ncolors_; }
+    const int32_t *color() const { return color_; }
+    // c = 0 on entry; forward over the colours, then (symmetric) backward:
+    // c_i = d_inv_i (r_i - sum_{j != i} a_ij c_j)
+    void sweep(const HipCrsMatrix &A, const double *dinv, const double *r, double *c, bool symmetric) const {
+        for (int pass = 0; pass < (symmetric ? 2 : 1); ++pass)
+            for (int32_t t = 0; t < ncolors_; ++t) {
+                const int32_t k = pass == 0 ? t : ncolors_ - 1 - t;
+                const int32_t o = offsets_[(size_t)k], cnt = offsets_[(size_t)k + 1] - o;
+                hip_check(uhip_pgs_color_sweep(A.c(), &A.raw(), cnt, rows_ + o, dinv, r, nullptr, nullptr, nullptr,
+                                               c, 0),
+                          "gauss-seidel colour sweep");
+            }
+    }
+
+private:
+    void release() {
+        if (ctx_) {
+            if (color_) uhip_free(ctx_->get(), color_);
+            if (rows_) uhip_free(ctx_->get(), rows_);
+        }
+        color_ = rows_ = nullptr;
+    }
+    HipContextPtr ctx_;
+    int32_t ncolors_ = 0;
+    int32_t *color_ = nullptr, *rows_ = nullptr;
+    std::vector<int32_t> offsets_;
+};
+
+// GaussSeidel (solvers/smoothers/utopia_GaussSeidel.hpp:45-57, defaults :25 and
+// :232-234: one local sweep, symmetric, no l1, no line search): per sweep
+// r = rhs - A x; c = 0; symmetric Gauss-Seidel on D c = r - (L+U) c; x += c;
+// d_inv = 1/diag. The lexicographic row order of the reference sweep becomes a
+// colour order.
+class GaussSeidel final : public IterativeSolver<HipCrsMatrix, HipVector> {
+public:
+    GaussSeidel() { this->max_it_ = 1; }
+    void use_symmetric_sweep(bool v) { symmetric_ = v; }
+    bool symmetric() const { return symmetric_; }
+    void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
+        IterativeSolver::update(op);
+        d_inv_ = HipVector(op->context(), op->rows(), 0.0);
+        hip_check(uhip_pgs_dinv(op->c(), &op->raw(), d_inv_.data()), "gauss-seidel d_inv");
+        groups_ = std::make_shared<ColorGroups>();
+        groups_->build(*op);
+        r_ = HipVector(op->context(), op->rows(), 0.0);
+        c_ = HipVector(op->context(), op->rows(), 0.0);
+    }
+    bool apply(const HipVector &b, HipVector &x) override {
+        if (x.empty() || x.size() != b.size()) x.zeros_like(b);
+        for (int64_t s = 0; s < this->max_it_; ++s) {
+            this->op_->residual(x, b, r_);
+            c_.set(0.0);
+            groups_->sweep(*this->op_, d_inv_.data(), r_.data(), c_.data(), symmetric_);
+            x.axpy(1.0, c_);
+        }
+        return true;
+    }
+
+private:
+    bool symmetric_ = true;
+    HipVector d_inv_, r_, c_;
+    std::shared_ptr<ColorGroups> groups_;
+};
+
+// ---------------------------------------------------------------------------
 // The V-cycle engine on levels with values T. Level 0 is the coarsest.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -139,12 +231,13 @@ struct MGLevel {
     std::shared_ptr<const HipCrsMatrixT<T>> P;  // interpolation from level l-1 (l >= 1)
     HipVectorT<T> mask;                        // mask of this level (empty: masks off)
     HipVectorT<T> r, c, rR, rk, pa, pb, dinv;
+    std::shared_ptr<ColorGroups> groups;        // Gauss-Seidel colour groups (FP64 levels)
     std::vector<double> alpha, beta;            // Chebyshev coefficients for pre / post
     std::vector<double> alpha_post, beta_post;
     double eig_max = 0, eig_min = 0;
 };
 
-enum MGSmoother { MG_CHEBYSHEV = 0, MG_JACOBI = 1 };
+enum MGSmoother { MG_CHEBYSHEV = 0, MG_JACOBI = 1, MG_GAUSS_SEIDEL = 2 };
 
 template <typename T>
 class VCycle {
@@ -153,7 +246,7 @@ public:
     std::vector<MGLevel<T>> levels;
     const Factorization *coarse = nullptr;
     int pre = 3, post = 3, mg_type = 1, smoother = MG_CHEBYSHEV;
-    bool masks = true;
+    bool masks = true, gs_symmetric = true;
 
     void allocate_work() {
         for (size_t l = 0; l < levels.size(); ++l) {
@@ -220,9 +313,33 @@ private:
             return chebyshev_smooth_fused<T>(*L.A, b, b_dt, L.c.data(), zero_guess, out, out_dt, nu, a, be,
                                              L.rk.data(), L.pa.data(), L.pb.data(), dot_with, dot_d);
         }
-        // Jacobi sweeps: r = b - A x; x += d_inv .* r (x = 0 on a zero guess)
         uhip_ctx c = L.A->c();
         const int64_t n = L.A->rows();
+        if (smoother == MG_GAUSS_SEIDEL) {
+            if constexpr (std::is_same<T, double>::value) {
+                // per sweep: r = b - A x (= b on a zero guess); c = 0; colour sweeps; x += c
+                if (b_dt != UHIP_F64 || out_dt != UHIP_F64)
+                    throw HipError(UHIP_ERR_UNSUPPORTED, "Gauss-Seidel smoothing runs on FP64 levels");
+                if (zero_guess) L.c.set(0.0);
+                for (int s = 0; s < nu; ++s) {
+                    const double *r = (const double *)b;
+                    if (!(zero_guess && s == 0)) {
+                        hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), b, b_dt, L.rk.data()), "gs r");
+                        r = L.rk.data();
+                    }
+                    L.pa.set(0.0);
+                    L.groups->sweep(*L.A, L.dinv.data(), r, L.pa.data(), gs_symmetric);
+                    const bool last = (s == nu - 1);
+                    hip_check(uhip_vec_axpy_cast(c, n, 1.0, L.pa.data(), dt, L.c.data(), dt,
+                                                 last ? out : (void *)L.c.data(), last ? out_dt : dt),
+                              "gs x");
+                }
+                return false;
+            } else {
+                throw HipError(UHIP_ERR_UNSUPPORTED, "Gauss-Seidel smoothing runs on FP64 levels");
+            }
+        }
+        // Jacobi sweeps: r = b - A x; x += d_inv .* r (x = 0 on a zero guess)
         if (zero_guess) L.c.set(T(0));
         for (int s = 0; s < nu; ++s) {
             hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), b, b_dt, L.rk.data()), "jacobi r");
@@ -341,14 +458,17 @@ public:
         // smoothers, levels 1..L-1 in this order (rand() draws of the power method)
         auto cheb = std::dynamic_pointer_cast<Chebyshev3level<HipCrsMatrix, HipVector>>(smoother_);
         auto jac = std::dynamic_pointer_cast<Jacobi<HipCrsMatrix, HipVector>>(smoother_);
-        if (!cheb && !jac) throw HipError(UHIP_ERR_UNSUPPORTED, "Multigrid: smoother must be Chebyshev3level or Jacobi");
+        auto gs = std::dynamic_pointer_cast<GaussSeidel>(smoother_);
+        if (!cheb && !jac && !gs)
+            throw HipError(UHIP_ERR_UNSUPPORTED, "Multigrid: smoother must be Chebyshev3level, Jacobi or GaussSeidel");
         eng_.levels.clear();
         eng_.levels.resize((size_t)L);
         eng_.pre = pre_;
         eng_.post = post_;
         eng_.mg_type = mg_type_;
         eng_.masks = masks_;
-        eng_.smoother = cheb ? MG_CHEBYSHEV : MG_JACOBI;
+        eng_.smoother = cheb ? MG_CHEBYSHEV : (gs ? MG_GAUSS_SEIDEL : MG_JACOBI);
+        eng_.gs_symmetric = gs ? gs->symmetric() : true;
         for (int l = 0; l < L; ++l) {
             auto &lv = eng_.levels[(size_t)l];
             lv.A = A_[(size_t)l];
@@ -366,6 +486,11 @@ public:
                 pm.max_it(cheb->power_method().max_it());
                 lv.eig_max = cheb->scale_max_eig() * pm.get_max_eig(*A_[(size_t)l]);
                 lv.eig_min = cheb->scale_min_eig() * lv.eig_max;
+            } else if (gs) {
+                lv.dinv = HipVector(ctx, A_[(size_t)l]->rows(), 0.0);
+                hip_check(uhip_pgs_dinv(A_[(size_t)l]->c(), &A_[(size_t)l]->raw(), lv.dinv.data()), "gs d_inv");
+                lv.groups = std::make_shared<ColorGroups>();
+                lv.groups->build(*A_[(size_t)l]);
             } else {
                 A_[(size_t)l]->build_diag(lv.dinv);
                 InvDiagPreconditioner<HipCrsMatrix, HipVector>::pseudo_inv(lv.dinv, 0.0);

@@ -56,6 +56,7 @@ struct uhip_context_s {
     double *dev_scalars = nullptr;  // UHIP_NUM_SCALARS device scalar slots
     double *host_scalars = nullptr; // pinned mirror used for read-back
     unsigned int *counters = nullptr;
+    int autotune_reps = 0;          // > 0: automatic SpMV plans are chosen by timing
 };
 
 namespace uhip {

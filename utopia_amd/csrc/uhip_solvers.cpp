@@ -61,6 +61,10 @@ std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s) {
     std::shared_ptr<IterativeSolver<HipCrsMatrix, HipVector>> sm;
     if ((int)param(s, "smoother", 0) == 1) {
         sm = std::make_shared<Jacobi<HipCrsMatrix, HipVector>>();
+    } else if ((int)param(s, "smoother", 0) == 2) {
+        auto gs = std::make_shared<GaussSeidel>();
+        gs->use_symmetric_sweep(param(s, "use_symmetric_sweep", 1) != 0);
+        sm = gs;
     } else {
         auto ch = std::make_shared<Chebyshev3level<HipCrsMatrix, HipVector>>();
         ch->scale_max_eig(param(s, "scale_max_eig", 1.2));
@@ -186,6 +190,8 @@ int uhip_solver_destroy(uhip_solver s) {
 int uhip_solver_set_param(uhip_solver s, const char *key, double value) {
     if (!s || !key) return UHIP_ERR_INVALID;
     s->params[key] = value;
+    // SpMV plans of matrices built from now on are chosen by timing (reps > 0)
+    if (std::strcmp(key, "autotune") == 0) return uhip_ctx_set_autotune(s->ctx->get(), (int)value);
     // tolerances / iteration limits also act on an already built solver
     if (s->cg) apply_iterative_params(s, *s->cg);
     else if (s->mg && s->type == UHIP_SOLVER_MG) apply_iterative_params(s, *s->mg);
@@ -367,6 +373,20 @@ int uhip_solver_get_mask(uhip_solver s, int l, double *mask) {
 }
 
 // Override the Chebyshev bounds of level l after update (tests pinning the cycle).
+int uhip_solver_get_colors(uhip_solver s, int l, int32_t *color, int32_t *ncolors) {
+    if (!s || !s->mg || !color || !ncolors || l < 1 || l >= s->mg->n_levels()) return UHIP_ERR_INVALID;
+    try {
+        const auto &lv = s->mg->engine().levels[(size_t)l];
+        if (!lv.groups) return UHIP_ERR_INVALID;
+        *ncolors = lv.groups->ncolors();
+        hip_check(uhip_memcpy_dtoh(s->ctx->get(), color, lv.groups->color(), sizeof(int32_t) * (size_t)lv.A->rows()),
+                  "colors");
+        return UHIP_OK;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
 int uhip_solver_set_level_eigs(uhip_solver s, int l, double eig_max, double eig_min) {
     if (!s || !s->mg || l < 1 || l >= s->mg->n_levels()) return UHIP_ERR_INVALID;
     try {

@@ -315,6 +315,66 @@ static int spmv_typed(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
     return launch_rowblock<TAcc>(ctx, A, (const TV *)x, e);
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_one(int64_t n, T *__restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) x[i] = T(1);
+}
+
+// Autotuner: time y = A x (x = 1) for every applicable layout - DIA, SELL-64,
+// row-block CSR with 1024 / 2048 / 4096 nonzeros per workgroup - and keep the
+// fastest plan. Each candidate runs reps times after one untimed launch.
+static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out) {
+    struct Cand {
+        int32_t format, block_nnz;
+    };
+    const Cand cands[] = {{UHIP_FMT_DIA, 2048},
+                          {UHIP_FMT_SELL64, 2048},
+                          {UHIP_FMT_ROWBLOCK, 1024},
+                          {UHIP_FMT_ROWBLOCK, 2048},
+                          {UHIP_FMT_ROWBLOCK, 4096}};
+    const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
+    void *x = nullptr, *y = nullptr;
+    UHIP_CHECK_HIP(hipMalloc(&x, vb * (size_t)std::max(A->ncols, 1)));
+    UHIP_CHECK_HIP(hipMalloc(&y, vb * (size_t)std::max(A->nrows, 1)));
+    const dim3 g(grid_for(std::max(A->ncols, 1), 1, 1 << 14)), b(kBlock);
+    if (A->dtype == UHIP_F64) hipLaunchKernelGGL(k_fill_one<double>, g, b, 0, ctx->stream, A->ncols, (double *)x);
+    else hipLaunchKernelGGL(k_fill_one<float>, g, b, 0, ctx->stream, A->ncols, (float *)x);
+    hipEvent_t e0, e1;
+    UHIP_CHECK_HIP(hipEventCreate(&e0));
+    UHIP_CHECK_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand win{UHIP_FMT_ROWBLOCK, 2048};
+    int st = UHIP_OK;
+    for (const Cand &c : cands) {
+        st = uhip_csr_plan_format(ctx, A, c.format, c.block_nnz);
+        if (st == UHIP_ERR_UNSUPPORTED) continue;  // DIA does not apply
+        if (st != UHIP_OK) break;
+        auto run = [&]() {
+            return A->dtype == UHIP_F64 ? spmv_typed<double, double>(ctx, A, x, y) : spmv_typed<float, float>(ctx, A, x, y);
+        };
+        if ((st = run()) != UHIP_OK) break;
+        hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && st == UHIP_OK; ++r) st = run();
+        hipEventRecord(e1, ctx->stream);
+        hipEventSynchronize(e1);
+        if (st != UHIP_OK) break;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (ms < best) {
+            best = ms;
+            win = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+    if (st != UHIP_OK && st != UHIP_ERR_UNSUPPORTED) return st;
+    if (best_ms_out) *best_ms_out = best;
+    return uhip_csr_plan_format(ctx, A, win.format, win.block_nnz);
+}
+
 }  // namespace uhip
 
 using namespace uhip;
@@ -325,12 +385,28 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz) {
     return uhip_csr_plan_format(ctx, A, block_nnz == 0 ? UHIP_FMT_AUTO : UHIP_FMT_ROWBLOCK, block_nnz);
 }
 
+int uhip_ctx_set_autotune(uhip_ctx ctx, int reps) {
+    UHIP_REQUIRE(ctx && reps >= 0, "uhip_ctx_set_autotune: bad argument");
+    ctx->autotune_reps = reps;
+    return UHIP_OK;
+}
+
+int uhip_csr_autotune(uhip_ctx ctx, uhip_csr *A, int reps, int32_t *format, float *ms) {
+    UHIP_REQUIRE(ctx && A && A->rowptr && reps > 0, "uhip_csr_autotune: bad argument");
+    if (A->nrows == 0 || A->nnz == 0) return uhip_csr_plan_format(ctx, A, UHIP_FMT_ROWBLOCK, 2048);
+    UHIP_TRY(autotune_plan(ctx, A, reps, ms));
+    if (format) *format = A->format;
+    return UHIP_OK;
+}
+
 int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz) {
     UHIP_REQUIRE(ctx && A && A->rowptr, "uhip_csr_plan: null argument");
     UHIP_REQUIRE(A->nrows >= 0 && A->ncols >= 0, "uhip_csr_plan: bad shape");
     UHIP_REQUIRE(format == UHIP_FMT_AUTO || format == UHIP_FMT_ROWBLOCK || format == UHIP_FMT_SELL64 ||
                      format == UHIP_FMT_DIA,
                  "uhip_csr_plan: format %d", format);
+    if (format == UHIP_FMT_AUTO && ctx->autotune_reps > 0 && A->nrows > 0 && A->nnz > 0)
+        return autotune_plan(ctx, A, ctx->autotune_reps, nullptr);
     if (block_nnz == 0) block_nnz = 2048;
     int capk;
     if (block_nnz == 1024) capk = 5;
