@@ -214,6 +214,62 @@ def bench_batched(a):
     return 0
 
 
+def bench_mprgp(a, n=1024):
+    """min 1/2 x'Ax - b'x, l <= x, A = 9-point Laplacian + 1e-3 I on n^2, l = -0.05 - 0.15 u,
+    MPRGP to projected-gradient norm 1e-9; active set compared with the oracle's."""
+    import utopia_amd as ua
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    ctx = ua.Context(0)
+    seed = a.seed
+    A = ua.gen_stencil(ua.STENCIL_2D9, n, n, shift=1e-3)
+    b = ua.gen_uniform(seed, n * n)
+    lb = -0.05 - 0.15 * ua.gen_uniform(seed + 1, n * n, 0, 1)
+    q = ua.MPRGP(ctx, atol=1e-9, rtol=0, max_it=1000000)
+    q.update(A)
+    q.set_box_constraints(lb, None)
+    bd = ua.DeviceArray.from_numpy(ctx, b)
+    xd = ua.DeviceArray(ctx, n * n)
+    L = ua.lib()
+    times = []
+    for i in range(a.warmup + a.steps):
+        ua.check(L.uhip_vec_set_f64(ctx.h, n * n, 0.0, ctypes.c_void_p(xd.ptr)))
+        ctx.sync()
+        ua.srand(1)  # power-method start vector (rand(), as in the reference)
+        t = time.perf_counter()
+        q.apply_device(bd, xd)
+        ctx.sync()
+        if i >= a.warmup:
+            times.append(time.perf_counter() - t)
+        log(f"mprgp {n}^2: solve {time.perf_counter() - t:.3f} s, iterations {q.status['iterates']}")
+    x = xd.numpy()
+    st = q.status
+    line = {"metric": f"MPRGP box-constrained QP {n}^2 9-point + 1e-3 I, projected gradient 1e-9",
+            "value": float(np.mean(times)), "unit": "s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": float(np.mean(times)) * 1e3, "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded b, bounds)",
+            "config": {"workload": f"MPRGP {n}^2 9-point + 1e-3 I, l = -0.05 - 0.15 u", "rows": n * n,
+                       "nnz": A.nnz, "parallelism": "single GPU"},
+            "iterations": st["iterates"], "reason": st["reason"]}
+    if not a.no_cpu_baseline:
+        orc.set_threads(min(16, os.cpu_count() or 1))
+        ua.srand(1)
+        t = time.perf_counter()
+        xo, so = orc.mprgp_solve(A, b, lb=lb, atol=1e-9, rtol=0, max_it=1000000)
+        tc = time.perf_counter() - t
+        act, act_o = np.abs(x - lb) < 1e-12, np.abs(xo - lb) < 1e-12
+        ties = np.abs(x - lb) < 1e-9
+        line["cpu_baseline"] = {"value": tc, "unit": "s", "cores": orc.get_threads(), "kind": "port",
+                                "sample": f"oracle MPRGP on the same {n}^2 problem, {so['iterates']} iterations"}
+        line["oracle_iterations"] = so["iterates"]
+        line["active_set"] = {"gpu": int(act.sum()), "oracle": int(act_o.sum()),
+                              "disagree_beyond_ties": int(np.sum((act != act_o) & ~ties))}
+        line["rel_error_vs_oracle"] = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+        log(f"oracle: {tc:.1f} s, {so['iterates']} iterations; active {act.sum()} vs {act_o.sum()}")
+    print(json.dumps(line))
+    return 0
+
+
 def host_info():
     info = {"nproc": os.cpu_count()}
     try:
@@ -239,8 +295,9 @@ def main(argv=None):
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV / streaming kernel timings (for counter collection runs)")
-    ap.add_argument("--workload", choices=["mgpcg256", "batched"], default="mgpcg256",
-                    help="mgpcg256: the 256^3 MG-PCG line (default); batched: 64 systems x 8 RHS hipGraph MG-PCG")
+    ap.add_argument("--workload", choices=["mgpcg256", "batched", "mprgp1024"], default="mgpcg256",
+                    help="mgpcg256: the 256^3 MG-PCG line (default); batched: 64 systems x 8 RHS hipGraph MG-PCG; "
+                         "mprgp1024: box-constrained QP on the 1024^2 9-point operator")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design "
@@ -249,6 +306,8 @@ def main(argv=None):
 
     if a.workload == "batched":
         return bench_batched(a)
+    if a.workload == "mprgp1024":
+        return bench_mprgp(a)
     import utopia_amd as ua
     L = ua.lib()
     n = a.n

@@ -334,3 +334,26 @@ def test_gauss_seidel_on_diagonal_matrix_is_jacobi():
     b = ua.gen_uniform(1, n)
     x = orc.gs_smooth(A, 1.0 / d, np.zeros(n, np.int32), 1, b, sweeps=1)
     assert np.array_equal(x, (1.0 / d) * b)
+
+
+def test_mprgp_iterations_are_rounding_sensitive():
+    """At 256^2 the MPRGP iteration count is not a rounding-stable quantity:
+    perturbing b by one ulp changes the oracle's count by more than 5 %, while
+    the solution (and its active set) stays put. Iteration counts of MPRGP runs
+    with different summation orders are therefore compared with a band, the
+    solution and the active set exactly (up to ties)."""
+    n = 256
+    A = ua.gen_stencil(ua.STENCIL_2D9, n, n, shift=1e-3)
+    b = ua.gen_uniform(4, n * n)
+    lb = -0.05 - 0.15 * ua.gen_uniform(5, n * n, 0, 1)
+    its, xs = [], []
+    for eps in (0.0, 1e-16):
+        bb = b * (1 + eps * np.sign(np.sin(np.arange(n * n))))
+        ua.srand(1)
+        x, st = orc.mprgp_solve(A, bb, lb=lb, atol=1e-9, rtol=0, max_it=100000)
+        its.append(st["iterates"])
+        xs.append(x)
+    assert abs(its[1] - its[0]) > 0.05 * its[0]
+    assert np.linalg.norm(xs[1] - xs[0]) <= 1e-7 * np.linalg.norm(xs[0])
+    tie = np.abs(xs[0] - lb) < 1e-9
+    assert np.all(((np.abs(xs[0] - lb) < 1e-12) == (np.abs(xs[1] - lb) < 1e-12)) | tie)
