@@ -6,7 +6,8 @@
 
 namespace uhip {
 
-// Grid-stride loop with four independent elements in flight per lane.
+// Grid-stride loop with four independent elements in flight per lane. Results
+// are written with streaming (non-temporal) stores: each is written once.
 #define UHIP_FOR4(i, n)                                                                    \
     for (int64_t base_ = (int64_t)blockIdx.x * kBlock + threadIdx.x,                      \
                  stride_ = (int64_t)gridDim.x * kBlock;                                    \
@@ -15,41 +16,41 @@ namespace uhip {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(int64_t n, T v, T *__restrict__ x) {
-    UHIP_FOR4(i, n) { x[i] = v; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store(v, x + i); }
 }
 
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const TI *__restrict__ x, TO *__restrict__ y) {
-    UHIP_FOR4(i, n) { y[i] = (TO)x[i]; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store((TO)x[i], y + i); }
 }
 
 // y = y + a*x
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy(int64_t n, T a, const T *__restrict__ x, T *__restrict__ y) {
-    UHIP_FOR4(i, n) { y[i] = y[i] + a * x[i]; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store(y[i] + a * x[i], y + i); }
 }
 
 // y = a*x + b*y
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, T a, const T *__restrict__ x, T b, T *__restrict__ y) {
-    UHIP_FOR4(i, n) { y[i] = a * x[i] + b * y[i]; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store(a * x[i] + b * y[i], y + i); }
 }
 
 // w = a*x + b*y
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_waxpby(int64_t n, T a, const T *__restrict__ x, T b,
                                                    const T *__restrict__ y, T *__restrict__ w) {
-    UHIP_FOR4(i, n) { w[i] = a * x[i] + b * y[i]; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store(a * x[i] + b * y[i], w + i); }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale(int64_t n, T a, T *__restrict__ x) {
-    UHIP_FOR4(i, n) { x[i] = a * x[i]; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store(a * x[i], x + i); }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_shift(int64_t n, T a, T *__restrict__ x) {
-    UHIP_FOR4(i, n) { x[i] = x[i] + a; }
+    UHIP_FOR4(i, n) { __builtin_nontemporal_store(x[i] + a, x + i); }
 }
 
 template <typename T>

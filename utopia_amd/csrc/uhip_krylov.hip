@@ -65,8 +65,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double 
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double v = r[i] - alpha * Ap[i];
-        rn[i] = v;
-        if (rn32) rn32[i] = (float)v;
+        __builtin_nontemporal_store(v, rn + i);
+        if (rn32) __builtin_nontemporal_store((float)v, rn32 + i);
         acc += v * v;
     }
     acc = block_sum<kBlock>(acc);
@@ -85,8 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpupdate(int64_t n, double *__res
     const double beta = z ? s[i_num] / s[i_den] : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double pi = p[i];
-        x[i] = x[i] + alpha * pi;
-        if (z) p[i] = z[i] + beta * pi;
+        __builtin_nontemporal_store(x[i] + alpha * pi, x + i);
+        if (z) __builtin_nontemporal_store(z[i] + beta * pi, p + i);
     }
 }
 

@@ -23,6 +23,12 @@ __device__ __forceinline__ T ld_stream(const T *p) {
     return __builtin_nontemporal_load(p);
 }
 
+// Streaming (non-temporal) store of a vector entry that this kernel writes once.
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+
 // Epilogues: operator()(row, sum, part) with part[kDots] per-thread partial sums
 // that the kernel reduces per block into partials[d*stride + block].
 template <typename TAcc, int CAPK, class Epi, typename TA, typename TX>
@@ -301,7 +307,7 @@ template <typename TAcc, typename TY>
 struct EpiStore {
     static constexpr int kDots = 0;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const { y[r] = (TY)s; }
+    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)s); }
 };
 
 // y = alpha*s + beta*y   (beta == 0 never reads y)
@@ -311,7 +317,7 @@ struct EpiAxpby {
     TY *y;
     TAcc alpha, beta;
     __device__ void operator()(int r, TAcc s, double *) const {
-        y[r] = (beta == TAcc(0)) ? (TY)(alpha * s) : (TY)(alpha * s + beta * (TAcc)y[r]);
+        st_stream(y + r, (beta == TAcc(0)) ? (TY)(alpha * s) : (TY)(alpha * s + beta * (TAcc)y[r]));
     }
 };
 
@@ -322,7 +328,7 @@ struct EpiAxpbz {
     const TY *z;
     TY *y;
     TAcc alpha, beta;
-    __device__ void operator()(int r, TAcc s, double *) const { y[r] = (TY)(alpha * s + beta * (TAcc)z[r]); }
+    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)(alpha * s + beta * (TAcc)z[r])); }
 };
 
 // r = b - s
@@ -331,7 +337,7 @@ struct EpiResidual {
     static constexpr int kDots = 0;
     const TB *b;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const { y[r] = (TY)((TAcc)b[r] - s); }
+    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)((TAcc)b[r] - s)); }
 };
 
 // y = s, part += x_r * s  (p.Ap of CG)
@@ -341,7 +347,7 @@ struct EpiStoreDot {
     const TX *x;
     TY *y;
     __device__ void operator()(int r, TAcc s, double *part) const {
-        y[r] = (TY)s;
+        st_stream(y + r, (TY)s);
         part[0] += (double)x[r] * (double)s;
     }
 };
@@ -353,7 +359,7 @@ struct EpiMaskStore {
     const TY *mask;
     TY *y;
     __device__ void operator()(int r, TAcc s, double *) const {
-        y[r] = mask ? (TY)((TAcc)mask[r] * s) : (TY)s;
+        st_stream(y + r, mask ? (TY)((TAcc)mask[r] * s) : (TY)s);
     }
 };
 
@@ -362,7 +368,7 @@ template <typename TAcc, typename TY>
 struct EpiAddTo {
     static constexpr int kDots = 0;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const { y[r] = (TY)((TAcc)y[r] + s); }
+    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)((TAcc)y[r] + s)); }
 };
 
 // One step of Chebyshev3level::solve (utopia_Chebyshev3level.hpp:138-175) fused
@@ -394,11 +400,11 @@ struct EpiCheb {
         if (last) {
             xv = xv + a_cur * pk;
         } else {
-            rout[r] = (TV)rk;
-            pout[r] = (TV)pk;
+            st_stream(rout + r, (TV)rk);
+            st_stream(pout + r, (TV)pk);
         }
         const TXO xo = (TXO)xv;
-        xout[r] = xo;
+        st_stream(xout + r, xo);
         if constexpr (DOT) part[0] += (double)xo * dot_with[r];
     }
 };
