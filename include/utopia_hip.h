@@ -322,6 +322,15 @@ int uhip_mprgp_expand_f64(uhip_ctx ctx, int64_t n, double *x, const double *g, c
 /* proportioning step (utopia_MPRGP.hpp:223-235): x -= a beta; g -= a Abeta; p = get_fi(x, g) */
 int uhip_mprgp_proportion_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *beta, const double *Abeta,
                               const double *lb, const double *ub, double *p, double alpha);
+/* ProjectedConjugateGradient element step (solvers/nonlinear/constrained/quadratic_programming/
+ * utopia_ProjectedConjugateGradient.hpp:71-97): with elem = z_i (the reference's view reads zk),
+ * w_i = val = (|elem-ub_i| <= eps or |elem-lb_i| <= eps) ? max(u_i,0) : u_i; z_i = val == 0 ? max(p_i,0) : p_i */
+int uhip_pcg_wz_f64(uhip_ctx ctx, int64_t n, const double *u, const double *lb, const double *ub, const double *p,
+                    double *z, double *w);
+/* ProjectedGradient direction (utopia_ProjectedGradient.hpp:157-170): p_i = 0 where x_i is within
+ * 10 eps of lb_i or ub_i, else u_i */
+int uhip_pg_direction_f64(uhip_ctx ctx, int64_t n, const double *x, const double *u, const double *lb,
+                          const double *ub, double *p);
 /* Jones-Plassmann colouring of a structurally symmetric pattern: color[i] in [0, ncolors),
  * no two coupled rows share a colour. */
 int uhip_graph_color(uhip_ctx ctx, const uhip_csr *A, uint64_t seed, int32_t *color, int32_t *ncolors);
@@ -349,6 +358,8 @@ typedef struct uhip_solver_s *uhip_solver;
 #define UHIP_SOLVER_MG 4          /* Multigrid as a solver */
 #define UHIP_SOLVER_MPRGP 5       /* MPRGP box-constrained QP */
 #define UHIP_SOLVER_PGS 6         /* ProjectedGaussSeidel (multicolour) */
+#define UHIP_SOLVER_PCG 7         /* ProjectedConjugateGradient */
+#define UHIP_SOLVER_PG 8          /* ProjectedGradient */
 
 /* isolver_lsolve_init */
 int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out);

@@ -890,6 +890,119 @@ done:
 
 /* ------------------------------------------------------ ProjectedGaussSeidel */
 
+/* ProjectedConjugateGradient::apply
+ * (solvers/nonlinear/constrained/quadratic_programming/utopia_ProjectedConjugateGradient.hpp:33-123).
+ * The element loop reads z_k where the reference's x_view points (a view of zk). */
+int or_pcg_solve(const or_csr *A, const double *b, const double *lb_in, const double *ub_in, double *x, double atol,
+                 double rtol, double stol, int64_t max_it, or_status *st) {
+    const int64_t n = A->nrows;
+    double *lb = vnew(n), *ub = vnew(n), *x_old = vnew(n), *x_half = vnew(n), *uk = vnew(n), *wk = vnew(n),
+           *zk = vnew(n), *pk = vnew(n), *Apk = vnew(n), *d = vnew(n);
+    for (int64_t i = 0; i < n; ++i) {
+        lb[i] = lb_in ? lb_in[i] : -DBL_MAX;
+        ub[i] = ub_in ? ub_in[i] : DBL_MAX;
+        zk[i] = 0.0;
+    }
+    memset(st, 0, sizeof(*st));
+    memcpy(x_old, x, sizeof(double) * (size_t)n);
+    or_spmv(A, x, uk);
+    for (int64_t i = 0; i < n; ++i) uk[i] = b[i] - uk[i];
+    for (int64_t i = 0; i < n; ++i) pk[i] = -uk[i];
+    int converged = 0;
+    int64_t it = 0;
+    const double eps = DBL_EPSILON;
+    while (!converged) {
+        or_spmv(A, pk, Apk);
+        const double pAp = or_dot(n, pk, Apk);
+        const double alpha = or_dot(n, uk, pk) / pAp;
+        if (alpha == 0. || isinf(alpha) || isnan(alpha)) break;
+        for (int64_t i = 0; i < n; ++i) x_half[i] = x_old[i] + alpha * pk[i];
+        for (int64_t i = 0; i < n; ++i) {
+            const double v = x_half[i] < ub[i] ? x_half[i] : ub[i];
+            x[i] = v > lb[i] ? v : lb[i];
+        }
+        or_spmv(A, x, uk);
+        for (int64_t i = 0; i < n; ++i) uk[i] = b[i] - uk[i];
+        for (int64_t i = 0; i < n; ++i) {
+            const double elem = zk[i];
+            double val = (fabs(elem - ub[i]) <= eps || fabs(elem - lb[i]) <= eps) ? fmax(uk[i], 0.0) : uk[i];
+            zk[i] = (val == 0) ? fmax(pk[i], 0.0) : pk[i];
+            wk[i] = val;
+        }
+        /* Apk = A * pk again in the reference: pk is unchanged, so Apk is */
+        const double beta = or_dot(n, wk, Apk) / pAp;
+        for (int64_t i = 0; i < n; ++i) pk[i] = wk[i] + beta * zk[i];
+        if (it % 10 == 0 || isinf(beta) || isnan(beta)) {
+            for (int64_t i = 0; i < n; ++i) d[i] = x_old[i] - x[i];
+            converged = check_convergence(it, 1, 1, or_norm2(n, d), atol, rtol, stol, max_it, st);
+        }
+        ++it;
+        if (converged) break;
+        memcpy(x_old, x, sizeof(double) * (size_t)n);
+    }
+    st->converged = converged;
+    free(lb), free(ub), free(x_old), free(x_half), free(uk), free(wk), free(zk), free(pk), free(Apk), free(d);
+    return converged;
+}
+
+/* ProjectedGradient::solve / solve_unconstrained
+ * (solvers/nonlinear/constrained/quadratic_programming/utopia_ProjectedGradient.hpp:54-213).
+ * Unconstrained when both bounds are NULL. */
+int or_pg_solve(const or_csr *A, const double *b, const double *lb_in, const double *ub_in, double *x, double atol,
+                double rtol, double stol, int64_t max_it, or_status *st) {
+    const int64_t n = A->nrows;
+    const int bounded = lb_in || ub_in;
+    double *lb = vnew(n), *ub = vnew(n), *x_old = vnew(n), *x_half = vnew(n), *u = vnew(n), *p = vnew(n),
+           *Ap = vnew(n), *d = vnew(n);
+    for (int64_t i = 0; i < n; ++i) {
+        lb[i] = lb_in ? lb_in[i] : -DBL_MAX;
+        ub[i] = ub_in ? ub_in[i] : DBL_MAX;
+    }
+    memset(st, 0, sizeof(*st));
+    memcpy(x_old, x, sizeof(double) * (size_t)n);
+    or_spmv(A, x, u);
+    for (int64_t i = 0; i < n; ++i) u[i] = b[i] - u[i];
+    memcpy(p, u, sizeof(double) * (size_t)n);
+    double alpha = 1.;
+    int converged = 0;
+    int64_t it = 0;
+    const double tol = 10. * DBL_EPSILON;
+    while (!converged) {
+        if (bounded) {
+            for (int64_t i = 0; i < n; ++i) x_half[i] = x[i] + alpha * p[i];
+            for (int64_t i = 0; i < n; ++i) {
+                const double v = lb[i] > x_half[i] ? lb[i] : x_half[i];
+                x[i] = ub[i] < v ? ub[i] : v;
+            }
+            or_spmv(A, x, u);
+            for (int64_t i = 0; i < n; ++i) u[i] = b[i] - u[i];
+            for (int64_t i = 0; i < n; ++i)
+                p[i] = (fabs(x[i] - ub[i]) <= tol || fabs(x[i] - lb[i]) <= tol) ? 0.0 : u[i];
+        } else {
+            for (int64_t i = 0; i < n; ++i) x[i] = x[i] + alpha * p[i];
+            or_spmv(A, x, u);
+            for (int64_t i = 0; i < n; ++i) p[i] = b[i] - u[i];
+        }
+        if (it % 20 == 0) {
+            for (int64_t i = 0; i < n; ++i) d[i] = x_old[i] - x[i];
+            converged = check_convergence(it, 1, 1, or_norm2(n, d), atol, rtol, stol, max_it, st);
+        }
+        ++it;
+        if (converged) break;
+        memcpy(x_old, x, sizeof(double) * (size_t)n);
+        or_spmv(A, p, Ap);
+        alpha = (bounded ? or_dot(n, u, p) : or_dot(n, p, p)) / or_dot(n, p, Ap);
+        if (isinf(alpha) || alpha == 0. || isnan(alpha)) {
+            for (int64_t i = 0; i < n; ++i) d[i] = x_old[i] - x[i];
+            converged = check_convergence(it, 1, 1, or_norm2(n, d), atol, rtol, stol, max_it, st);
+            break;
+        }
+    }
+    st->converged = converged;
+    free(lb), free(ub), free(x_old), free(x_half), free(u), free(p), free(Ap), free(d);
+    return converged;
+}
+
 /* ProjectedGaussSeidel<PetscMatrix,PetscVector,PETSC>::apply/step
  * (backend/petsc/solvers/nonlinear/constrained/quadratic_programming/utopia_petsc_ProjectedGaussSeidel.cpp:134-178,369-455)
  * with the scalar sweeper ProjectedScalarGaussSeidelSweep

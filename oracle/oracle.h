@@ -83,6 +83,11 @@ void or_chebyshev_smooth(const or_csr *A, const double *b, double *x, int nu, do
 /* MPRGP (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:57-387) */
 int or_mprgp_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double atol,
                    double rtol, int64_t max_it, double eps_eig_est, int64_t power_method_max_it, or_status *st);
+/* ProjectedConjugateGradient / ProjectedGradient (quadratic_programming/); NULL bound = none */
+int or_pcg_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double atol,
+                 double rtol, double stol, int64_t max_it, or_status *st);
+int or_pg_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double atol,
+                double rtol, double stol, int64_t max_it, or_status *st);
 /* ProjectedGaussSeidel<PetscMatrix,PetscVector,PETSC>::apply with the scalar sweeper */
 int or_pgs_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double stol,
                  int64_t max_it, int symmetric, int64_t check_each, or_status *st);

@@ -67,6 +67,8 @@ def lib():
             "or_chebyshev_smooth": (None, [P, P, P, I, D, D]),
             "or_mprgp_solve": (I, [P, P, P, P, P, D, D, I64, D, I64, P]),
             "or_pgs_solve": (I, [P, P, P, P, P, D, I64, I, I64, P]),
+            "or_pcg_solve": (I, [P, P, P, P, P, D, D, D, I64, P]),
+            "or_pg_solve": (I, [P, P, P, P, P, D, D, D, I64, P]),
             "or_set_threads": (None, [I]),
             "or_get_threads": (I, []),
         }
@@ -285,6 +287,27 @@ def mprgp_solve(A, b, lb=None, ub=None, x0=None, atol=1e-9, rtol=0.0, max_it=100
     lib().or_mprgp_solve(M.ref, _p(b), _p(lb), _p(ub), _p(x), atol, rtol, max_it, eps_eig_est, power_method_max_it,
                          ctypes.byref(st))
     return x, status_dict(st)
+
+
+def _qp_call(fn, A, b, lb, ub, x0, atol, rtol, stol, max_it):
+    M = Mat.of(A)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    lb = None if lb is None else np.ascontiguousarray(lb, np.float64)
+    ub = None if ub is None else np.ascontiguousarray(ub, np.float64)
+    st = or_status()
+    fn(M.ref, _p(b), _p(lb), _p(ub), _p(x), atol, rtol, stol, max_it, ctypes.byref(st))
+    return x, status_dict(st)
+
+
+def pcg_solve(A, b, lb=None, ub=None, x0=None, atol=1e-9, rtol=1e-9, stol=1e-11, max_it=1000):
+    """ProjectedConjugateGradient."""
+    return _qp_call(lib().or_pcg_solve, A, b, lb, ub, x0, atol, rtol, stol, max_it)
+
+
+def pg_solve(A, b, lb=None, ub=None, x0=None, atol=1e-9, rtol=1e-9, stol=1e-11, max_it=1000):
+    """ProjectedGradient."""
+    return _qp_call(lib().or_pg_solve, A, b, lb, ub, x0, atol, rtol, stol, max_it)
 
 
 def pgs_solve(A, b, lb=None, ub=None, x0=None, stol=1e-11, max_it=1000, symmetric=True, check_each=10):

@@ -291,6 +291,48 @@ def test_mprgp_256_active_set_matches_oracle(ctx):
     assert rel(x, xo) <= 1e-7
 
 
+def _reference_qp_problem(n=20):
+    from test_oracle_pins import _reference_qp_problem as mk
+    return mk(n)
+
+
+def test_projected_gradient_matches_oracle(ctx):
+    """ProjectedGradient on the reference's QP test problem (max_it 40 n) and on a
+    2D problem with lower bounds: same iterations and solution as the oracle."""
+    A, b, ub = _reference_qp_problem()
+    q = ua.ProjectedGradient(ctx, max_it=800)
+    q.update(A)
+    q.set_box_constraints(None, ub)
+    x = q.apply(b)
+    xo, so = orc.pg_solve(A, b, ub=ub, max_it=800)
+    assert q.status["iterates"] == so["iterates"] and q.status["reason"] == so["reason"] == 4
+    assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.norm(xo)
+    A2, b2, lb2 = _qp(24, 5)
+    q2 = ua.ProjectedGradient(ctx, max_it=20000, stol=1e-10)
+    q2.update(A2)
+    q2.set_box_constraints(lb2, None)
+    x2 = q2.apply(b2)
+    xo2, so2 = orc.pg_solve(A2, b2, lb=lb2, max_it=20000, stol=1e-10)
+    assert abs(q2.status["iterates"] - so2["iterates"]) <= 20
+    assert rel(x2, xo2) <= 1e-8
+
+
+def test_projected_cg_matches_oracle(ctx):
+    """ProjectedConjugateGradient: its iterates are rounding-chaotic on the reference
+    QP problem (test_oracle_pins::test_projected_cg_is_rounding_sensitive), so early
+    iterates are compared tightly and the max_it outcome within that sensitivity."""
+    A, b, ub = _reference_qp_problem()
+    for max_it, tol in ((10, 1e-11), (800, 5e-2)):
+        q = ua.ProjectedConjugateGradient(ctx, max_it=max_it, stol=0 if max_it == 10 else 1e-11)
+        q.update(A)
+        q.set_box_constraints(None, ub)
+        x = q.apply(b)
+        xo, so = orc.pcg_solve(A, b, ub=ub, max_it=max_it, stol=0 if max_it == 10 else 1e-11)
+        assert q.status["iterates"] == so["iterates"] and q.status["reason"] == so["reason"]
+        assert np.linalg.norm(x - xo) <= tol * np.linalg.norm(xo), max_it
+        assert np.all(x <= ub)
+
+
 def test_pgs_converges_to_the_qp_solution(ctx):
     A, b, lb = _qp(24, 21)
     q = ua.ProjectedGaussSeidel(ctx, stol=1e-13, max_it=20000)

@@ -357,3 +357,75 @@ def test_mprgp_iterations_are_rounding_sensitive():
     assert np.linalg.norm(xs[1] - xs[0]) <= 1e-7 * np.linalg.norm(xs[0])
     tie = np.abs(xs[0] - lb) < 1e-9
     assert np.all(((np.abs(xs[0] - lb) < 1e-12) == (np.abs(xs[1] - lb) < 1e-12)) | tie)
+
+
+def _reference_qp_problem(n=20):
+    """QPSolverTestProblem (tests/test_problems/utopia_QPSolverTestProblem.hpp:28-100):
+    1D Laplacian with identity boundary rows (explicit zeros kept), rhs 1 with
+    zero ends, upper bound 30, max_it 40 n."""
+    rp, col, val = [0], [], []
+    for i in range(n):
+        if i == 0:
+            col += [0, 1]
+            val += [1.0, 0.0]
+        elif i == n - 1:
+            col += [n - 2, n - 1]
+            val += [0.0, 1.0]
+        else:
+            col += [i - 1, i, i + 1]
+            val += [-1.0, 2.0, -1.0]
+        rp.append(len(col))
+    A = ua.Csr(np.array(rp, np.int32), np.array(col, np.int32), np.array(val), (n, n))
+    b = np.ones(n)
+    b[0] = b[-1] = 0.0
+    return A, b, np.full(n, 30.0)
+
+
+def test_projected_gradient_solves_reference_qp_problem():
+    A, b, ub = _reference_qp_problem()
+    x, st = orc.pg_solve(A, b, ub=ub, max_it=40 * 20)
+    assert st["reason"] == 4  # converged on the step norm before max_it
+    r = b - A.to_scipy() @ x
+    act = np.abs(x - ub) < 1e-12
+    assert act.sum() == 4 and np.all(x <= ub)
+    assert np.abs(r[~act]).max() < 1e-10 and r[act].min() > 0  # variational inequality
+
+
+def test_projected_gradient_first_step_and_steepest_descent_limit():
+    A = ua.gen_stencil(ua.STENCIL_2D5, 9, 7)
+    b = ua.gen_uniform(4, 63)
+    x, _ = orc.pg_solve(A, b, max_it=0, stol=0)  # one step with alpha = 1 (utopia_ProjectedGradient.hpp:69)
+    assert np.array_equal(x, b - A.to_scipy() @ np.zeros(63))
+    x, _ = orc.pg_solve(A, b, max_it=4000, stol=0)
+    xs = np.linalg.solve(A.to_scipy().toarray(), b)
+    assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def test_projected_cg_first_step_is_steepest_descent_and_far_bounds_are_inert():
+    A = ua.gen_stencil(ua.STENCIL_2D5, 12, 10)
+    n = A.shape[0]
+    b = ua.gen_uniform(3, n)
+    x1, _ = orc.pcg_solve(A, b, max_it=0, stol=0)
+    Asp = A.to_scipy()
+    a = (b @ b) / (b @ (Asp @ b))
+    assert np.allclose(x1, a * b, rtol=1e-14, atol=0)
+    xu, _ = orc.pcg_solve(A, b, max_it=200, stol=0)
+    xb, _ = orc.pcg_solve(A, b, lb=np.full(n, -1e10), ub=np.full(n, 1e10), max_it=200, stol=0)
+    assert np.array_equal(xu, xb)
+    xs = np.linalg.solve(Asp.toarray(), b)
+    assert np.linalg.norm(xu - xs) <= 1e-7 * np.linalg.norm(xs)
+
+
+def test_projected_cg_is_rounding_sensitive():
+    """On the reference's QP test problem the ProjectedConjugateGradient iteration
+    runs to max_it without meeting stol, and one ulp of b moves its result by
+    more than 0.1 % after 800 iterations but by < 1e-12 after 10."""
+    A, b, ub = _reference_qp_problem()
+    bb = b * (1 + 1e-16 * np.sign(np.sin(np.arange(20) + 1)))
+    x0, s0 = orc.pcg_solve(A, b, ub=ub, max_it=800)
+    x1, _ = orc.pcg_solve(A, bb, ub=ub, max_it=800)
+    assert s0["reason"] == -5
+    assert np.linalg.norm(x1 - x0) > 1e-3 * np.linalg.norm(x0)
+    x0, _ = orc.pcg_solve(A, b, ub=ub, max_it=10, stol=0)
+    x1, _ = orc.pcg_solve(A, bb, ub=ub, max_it=10, stol=0)
+    assert np.linalg.norm(x1 - x0) < 1e-12 * np.linalg.norm(x0)

@@ -36,7 +36,8 @@ FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64, FMT_DIA = -1, 0, 1, 2
 _NP = {F64: np.float64, F32: np.float32}
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
-SOLVER_CG, SOLVER_CG_JACOBI, SOLVER_CG_MG, SOLVER_CG_MG_MIXED, SOLVER_MG, SOLVER_MPRGP, SOLVER_PGS = range(7)
+(SOLVER_CG, SOLVER_CG_JACOBI, SOLVER_CG_MG, SOLVER_CG_MG_MIXED, SOLVER_MG, SOLVER_MPRGP, SOLVER_PGS,
+ SOLVER_PCG, SOLVER_PG) = range(9)
 
 # ConvergenceReason (solvers/utopia_ConvergenceReason.hpp:84-100)
 CONVERGED_FNORM_ABS, CONVERGED_FNORM_RELATIVE, CONVERGED_SNORM_RELATIVE, DIVERGED_MAX_IT = 2, 3, 4, -5
@@ -442,6 +443,14 @@ class MPRGP(Solver):
 
 class ProjectedGaussSeidel(Solver):
     TYPE = SOLVER_PGS
+
+
+class ProjectedConjugateGradient(Solver):
+    TYPE = SOLVER_PCG
+
+
+class ProjectedGradient(Solver):
+    TYPE = SOLVER_PG
 
 
 def srand(seed: int = 1):

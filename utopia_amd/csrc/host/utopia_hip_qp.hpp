@@ -3,6 +3,8 @@
 //   MPRGP                 solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:14-421
 //   ProjectedGaussSeidel  backend/petsc/solvers/nonlinear/constrained/quadratic_programming/
 //                         utopia_petsc_ProjectedGaussSeidel.cpp:24-540 (multicolour sweep order)
+//   ProjectedConjugateGradient  .../quadratic_programming/utopia_ProjectedConjugateGradient.hpp:14-143
+//   ProjectedGradient     .../quadratic_programming/utopia_ProjectedGradient.hpp:18-230
 #pragma once
 
 #include <cfloat>
@@ -247,6 +249,141 @@ private:
     std::vector<int32_t> offsets_;
     HipContextPtr ctx_;
     HipVector dinv_, r_, c_, x_old_;
+};
+
+// ---------------------------------------------------------------------------
+// ProjectedConjugateGradient::apply (utopia_ProjectedConjugateGradient.hpp:33-123),
+// step for step; the second A*pk of the reference equals the first (pk is
+// unchanged in between), so it is reused.
+// ---------------------------------------------------------------------------
+class ProjectedConjugateGradient final : public QPSolverBase {
+public:
+    void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
+        QPSolverBase::update(op);
+        const int64_t n = op->rows();
+        for (HipVector *v : {&x_old_, &x_half_, &uk_, &wk_, &zk_, &pk_, &Apk_, &d_}) *v = HipVector(op->context(), n, 0.0);
+    }
+    bool apply(const HipVector &b, HipVector &x) override {
+        const HipCrsMatrix &A = *this->op_;
+        const int64_t n = A.rows();
+        uhip_ctx c = A.c();
+        double *s = A.context()->scalars();
+        if (x.empty() || x.size() != n) x.zeros_like(b);
+        box_.fill_empty_bounds(A.context(), n);
+        const double *lb = box_.lower->data(), *ub = box_.upper->data();
+        this->status_.clear();
+        x_old_.copy_from(x);
+        A.residual(x, b, uk_);  // uk = b - A x
+        zk_.set(0.0);           // init_memory: zk.zeros
+        pk_.copy_from(uk_);
+        pk_.scale(-1.0);        // pk = -uk
+        bool converged = false;
+        const int64_t check_each = 10;
+        int64_t iteration = 0;
+        while (!converged) {
+            hip_check(uhip_spmv_dot(c, &A.raw(), pk_.data(), Apk_.data(), s + S_PAP), "Apk, pk.Apk");
+            const double *u1[1] = {uk_.data()}, *p1[1] = {pk_.data()};
+            hip_check(uhip_vec_dots_f64(c, n, 1, u1, p1, s + S_UP), "uk.pk");
+            double h[2];
+            A.context()->read_scalars(s + S_PAP, 2, h);  // S_PAP, S_UP adjacent
+            const double alpha = h[1] / h[0];
+            if (alpha == 0. || std::isinf(alpha) || std::isnan(alpha)) break;
+            hip_check(uhip_vec_waxpby_f64(c, n, alpha, pk_.data(), 1.0, x_old_.data(), x_half_.data()), "x_half");
+            hip_check(uhip_box_project_f64(c, n, x_half_.data(), lb, ub, x.data()), "project");
+            A.residual(x, b, uk_);
+            hip_check(uhip_pcg_wz_f64(c, n, uk_.data(), lb, ub, pk_.data(), zk_.data(), wk_.data()), "w, z");
+            const double *w1[1] = {wk_.data()}, *a1[1] = {Apk_.data()};
+            hip_check(uhip_vec_dots_f64(c, n, 1, w1, a1, s + S_WAP), "wk.Apk");
+            double h3[3];
+            A.context()->read_scalars(s + S_PAP, 3, h3);  // pk.Apk, uk.pk, wk.Apk
+            const double beta = h3[2] / h3[0];
+            hip_check(uhip_vec_waxpby_f64(c, n, 1.0, wk_.data(), beta, zk_.data(), pk_.data()), "pk = wk + beta zk");
+            if (iteration % check_each == 0 || std::isinf(beta) || std::isnan(beta)) {
+                d_.copy_from(x_old_);
+                d_.axpy(-1.0, x);
+                const double diff = d_.norm2();
+                this->print_iter(iteration, diff);
+                converged = this->check_convergence(iteration, 1, 1, diff);
+            }
+            ++iteration;
+            if (converged) break;
+            x_old_.copy_from(x);
+        }
+        return converged;
+    }
+
+private:
+    enum { S_PAP = 20, S_UP = 21, S_WAP = 22 };
+    HipVector x_old_, x_half_, uk_, wk_, zk_, pk_, Apk_, d_;
+};
+
+// ---------------------------------------------------------------------------
+// ProjectedGradient::solve / solve_unconstrained (utopia_ProjectedGradient.hpp:54-213)
+// ---------------------------------------------------------------------------
+class ProjectedGradient final : public QPSolverBase {
+public:
+    void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
+        QPSolverBase::update(op);
+        const int64_t n = op->rows();
+        for (HipVector *v : {&x_old_, &x_half_, &p_, &u_, &Ap_, &d_}) *v = HipVector(op->context(), n, 0.0);
+    }
+    bool apply(const HipVector &b, HipVector &x) override {
+        const HipCrsMatrix &A = *this->op_;
+        const int64_t n = A.rows();
+        uhip_ctx c = A.c();
+        double *s = A.context()->scalars();
+        if (x.empty() || x.size() != n) x.zeros_like(b);
+        this->status_.clear();
+        const bool bounded = box_.has_bound();
+        if (bounded) box_.fill_empty_bounds(A.context(), n);
+        const double *lb = bounded ? box_.lower->data() : nullptr, *ub = bounded ? box_.upper->data() : nullptr;
+        x_old_.copy_from(x);
+        A.residual(x, b, u_);  // u = b - A x
+        p_.copy_from(u_);
+        double alpha = 1.;
+        bool converged = false;
+        const int64_t check_each = 20;
+        int64_t iteration = 0;
+        while (!converged) {
+            if (bounded) {
+                hip_check(uhip_vec_waxpby_f64(c, n, 1.0, x.data(), alpha, p_.data(), x_half_.data()), "x_half");
+                hip_check(uhip_box_project_f64(c, n, x_half_.data(), lb, ub, x.data()), "project");
+                A.residual(x, b, u_);
+                hip_check(uhip_pg_direction_f64(c, n, x.data(), u_.data(), lb, ub, p_.data()), "direction");
+            } else {
+                x.axpy(alpha, p_);
+                A.residual(x, b, p_);  // p = b - A x
+            }
+            if (iteration % check_each == 0) {
+                d_.copy_from(x_old_);
+                d_.axpy(-1.0, x);
+                const double diff = d_.norm2();
+                this->print_iter(iteration, diff);
+                converged = this->check_convergence(iteration, 1, 1, diff);
+            }
+            ++iteration;
+            if (converged) break;
+            x_old_.copy_from(x);
+            hip_check(uhip_spmv_dot(c, &A.raw(), p_.data(), Ap_.data(), s + S_PAP), "Ap, p.Ap");
+            const double *a1[1] = {bounded ? u_.data() : p_.data()}, *p1[1] = {p_.data()};
+            hip_check(uhip_vec_dots_f64(c, n, 1, a1, p1, s + S_NUM), bounded ? "u.p" : "p.p");
+            double h[2];
+            A.context()->read_scalars(s + S_PAP, 2, h);
+            alpha = h[1] / h[0];
+            if (std::isinf(alpha) || alpha == 0. || std::isnan(alpha)) {
+                d_.copy_from(x_old_);
+                d_.axpy(-1.0, x);
+                const double diff = d_.norm2();
+                converged = this->check_convergence(iteration, 1, 1, diff);
+                break;
+            }
+        }
+        return converged;
+    }
+
+private:
+    enum { S_PAP = 24, S_NUM = 25 };
+    HipVector x_old_, x_half_, p_, u_, Ap_, d_;
 };
 
 }  // namespace utopia

@@ -3,12 +3,44 @@
 // the multicolour projected Gauss-Seidel sweep with its graph colouring.
 #include <hipcub/hipcub.hpp>
 
+#include <cfloat>
+
 #include "uhip_common.hpp"
 
 namespace uhip {
 
 // VariableBoundSolverInterface::project (utopia_VariableBoundSolverInterface.hpp:107-155)
 __device__ __forceinline__ double clamp_box(double x, double l, double u) { return (l >= x) ? l : ((u <= x) ? u : x); }
+
+// ProjectedConjugateGradient::apply, element loop (utopia_ProjectedConjugateGradient.hpp:71-97).
+// The active test reads z (the reference's x_view is a view of zk): elem = z_i,
+// val = (|elem - ub_i| <= eps or |elem - lb_i| <= eps) ? max(u_i, 0) : u_i,
+// z_i = (val == 0) ? max(p_i, 0) : p_i, w_i = val.
+__global__ __launch_bounds__(kBlock) void k_pcg_wz(int64_t n, const double *__restrict__ u,
+                                                   const double *__restrict__ lb, const double *__restrict__ ub,
+                                                   const double *__restrict__ p, double *__restrict__ z,
+                                                   double *__restrict__ w) {
+    const double eps = DBL_EPSILON;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double elem = z[i];
+        const double ui = u[i];
+        const double val = (fabs(elem - ub[i]) <= eps || fabs(elem - lb[i]) <= eps) ? fmax(ui, 0.0) : ui;
+        z[i] = (val == 0.0) ? fmax(p[i], 0.0) : p[i];
+        w[i] = val;
+    }
+}
+
+// ProjectedGradient::solve direction (utopia_ProjectedGradient.hpp:157-170):
+// p_i = 0 where x_i is within 10 eps of a bound, else u_i.
+__global__ __launch_bounds__(kBlock) void k_pg_dir(int64_t n, const double *__restrict__ x,
+                                                   const double *__restrict__ u, const double *__restrict__ lb,
+                                                   const double *__restrict__ ub, double *__restrict__ p) {
+    const double tol = 10.0 * DBL_EPSILON;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double xi = x[i];
+        p[i] = (fabs(xi - ub[i]) <= tol || fabs(xi - lb[i]) <= tol) ? 0.0 : u[i];
+    }
+}
 
 // MPRGP::get_fi (utopia_MPRGP.hpp:266-287)
 __device__ __forceinline__ double free_grad(double x, double g, double l, double u) {
@@ -262,6 +294,24 @@ __global__ __launch_bounds__(kBlock) void k_pgs_dinv(int32_t n, const int32_t *_
 using namespace uhip;
 
 extern "C" {
+
+int uhip_pcg_wz_f64(uhip_ctx ctx, int64_t n, const double *u, const double *lb, const double *ub, const double *p,
+                    double *z, double *w) {
+    UHIP_REQUIRE(ctx && u && lb && ub && p && z && w, "uhip_pcg_wz_f64: null argument");
+    if (n <= 0) return UHIP_OK;
+    hipLaunchKernelGGL(k_pcg_wz, dim3(grid_for(n, 4, 1 << 16)), dim3(kBlock), 0, ctx->stream, n, u, lb, ub, p, z, w);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_pg_direction_f64(uhip_ctx ctx, int64_t n, const double *x, const double *u, const double *lb,
+                          const double *ub, double *p) {
+    UHIP_REQUIRE(ctx && x && u && lb && ub && p, "uhip_pg_direction_f64: null argument");
+    if (n <= 0) return UHIP_OK;
+    hipLaunchKernelGGL(k_pg_dir, dim3(grid_for(n, 4, 1 << 16)), dim3(kBlock), 0, ctx->stream, n, x, u, lb, ub, p);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
 
 int uhip_box_project_f64(uhip_ctx ctx, int64_t n, const double *x, const double *lb, const double *ub, double *out) {
     UHIP_REQUIRE(ctx && x && lb && ub && out, "uhip_box_project_f64: null argument");

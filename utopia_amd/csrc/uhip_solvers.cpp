@@ -24,6 +24,7 @@ struct uhip_solver_s {
     std::shared_ptr<MixedPrecisionMultigrid> mixed;
     std::shared_ptr<MPRGP> mprgp;
     std::shared_ptr<ProjectedGaussSeidel> pgs;
+    std::shared_ptr<QPSolverBase> qp;  // the QP solver of the MPRGP / PGS / PCG / PG types
     HipVector lb, ub;
     bool has_lb = false, has_ub = false;
     SolutionStatus status;
@@ -89,6 +90,7 @@ void build(uhip_solver_s *s) {
     s->mg.reset();
     s->mprgp.reset();
     s->pgs.reset();
+    s->qp.reset();
     switch (s->type) {
         case UHIP_SOLVER_CG:
         case UHIP_SOLVER_CG_JACOBI:
@@ -129,6 +131,7 @@ void build(uhip_solver_s *s) {
             q->set_eig_comp_tol(param(s, "eig_comp_tol", 1e-1));
             q->power_method_max_it((int64_t)param(s, "power_method_max_it", 10));
             s->mprgp = q;
+            s->qp = q;
             s->solver = q;
             break;
         }
@@ -138,6 +141,21 @@ void build(uhip_solver_s *s) {
             q->use_symmetric_sweep(param(s, "use_symmetric_sweep", 1) != 0);
             q->check_s_norm_each((int64_t)param(s, "check_s_norm_each", 10));
             s->pgs = q;
+            s->qp = q;
+            s->solver = q;
+            break;
+        }
+        case UHIP_SOLVER_PCG: {
+            auto q = std::make_shared<ProjectedConjugateGradient>();
+            apply_iterative_params(s, *q);
+            s->qp = q;
+            s->solver = q;
+            break;
+        }
+        case UHIP_SOLVER_PG: {
+            auto q = std::make_shared<ProjectedGradient>();
+            apply_iterative_params(s, *q);
+            s->qp = q;
             s->solver = q;
             break;
         }
@@ -150,8 +168,7 @@ void set_bounds(uhip_solver_s *s) {
     BoxConstraints box;
     if (s->has_lb) box.lower = std::make_shared<HipVector>(s->lb);
     if (s->has_ub) box.upper = std::make_shared<HipVector>(s->ub);
-    if (s->mprgp) s->mprgp->set_box_constraints(box);
-    if (s->pgs) s->pgs->set_box_constraints(box);
+    if (s->qp) s->qp->set_box_constraints(box);
 }
 
 int do_update(uhip_solver_s *s) {
@@ -170,7 +187,7 @@ extern "C" {
 
 int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out) {
     if (!ctx || !out) return UHIP_ERR_INVALID;
-    if (type < UHIP_SOLVER_CG || type > UHIP_SOLVER_PGS) return UHIP_ERR_INVALID;
+    if (type < UHIP_SOLVER_CG || type > UHIP_SOLVER_PG) return UHIP_ERR_INVALID;
     try {
         auto *s = new uhip_solver_s();
         s->type = type;
@@ -195,8 +212,7 @@ int uhip_solver_set_param(uhip_solver s, const char *key, double value) {
     // tolerances / iteration limits also act on an already built solver
     if (s->cg) apply_iterative_params(s, *s->cg);
     else if (s->mg && s->type == UHIP_SOLVER_MG) apply_iterative_params(s, *s->mg);
-    else if (s->mprgp) apply_iterative_params(s, *s->mprgp);
-    else if (s->pgs) apply_iterative_params(s, *s->pgs);
+    else if (s->qp) apply_iterative_params(s, *s->qp);
     return UHIP_OK;
 }
 
