@@ -322,6 +322,14 @@ int uhip_mprgp_expand_f64(uhip_ctx ctx, int64_t n, double *x, const double *g, c
 /* proportioning step (utopia_MPRGP.hpp:223-235): x -= a beta; g -= a Abeta; p = get_fi(x, g) */
 int uhip_mprgp_proportion_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *beta, const double *Abeta,
                               const double *lb, const double *ub, double *p, double alpha);
+/* ActiveSet::determine (solvers/nonlinear/constrained/utopia_ActiveSet.hpp:41-79): indicator_i = 1 where
+ * x_i <= lb_i + tol or x_i >= ub_i - tol, else 0; changed_d[0] = number of entries that changed */
+int uhip_active_set_f64(uhip_ctx ctx, int64_t n, const double *x, const double *lb, const double *ub, double tol,
+                        double *indicator, double *changed_d);
+/* IPTruncatedTransfer::truncate_interpolation (transfer/utopia_IPTruncatedTransfer.hpp:203-223):
+ * zero the values of the rows with indicator == 1 (FP64, pattern kept). DIA and SELL-64
+ * plans hold their own copy of the values: re-plan after the call. */
+int uhip_csr_zero_rows(uhip_ctx ctx, uhip_csr *A, const double *indicator);
 /* ProjectedConjugateGradient element step (solvers/nonlinear/constrained/quadratic_programming/
  * utopia_ProjectedConjugateGradient.hpp:71-97): with elem = z_i (the reference's view reads zk),
  * w_i = val = (|elem-ub_i| <= eps or |elem-lb_i| <= eps) ? max(u_i,0) : u_i; z_i = val == 0 ? max(p_i,0) : p_i */
@@ -360,6 +368,7 @@ typedef struct uhip_solver_s *uhip_solver;
 #define UHIP_SOLVER_PGS 6         /* ProjectedGaussSeidel (multicolour) */
 #define UHIP_SOLVER_PCG 7         /* ProjectedConjugateGradient */
 #define UHIP_SOLVER_PG 8          /* ProjectedGradient */
+#define UHIP_SOLVER_MMG 9         /* MonotoneMultigrid (PGS smoother, truncated finest transfer) */
 
 /* isolver_lsolve_init */
 int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out);

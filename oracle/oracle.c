@@ -515,6 +515,59 @@ void or_mg_set_colors(or_mg *mg, int l, const int32_t *color, int32_t ncolors) {
     mg->ncolors[l] = ncolors;
 }
 
+/* Jones-Plassmann colouring with the smallest available colour: in each round
+ * every uncoloured row whose (hash, index) exceeds that of all its uncoloured
+ * neighbours takes the smallest colour none of its coloured neighbours has.
+ * hash(i) = splitmix64 finaliser of (i xor seed). Needs a structurally
+ * symmetric pattern. (The colouring that replaces the reference's
+ * lexicographic Gauss-Seidel order; no reference counterpart.) */
+static uint64_t jp_mix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void or_jp_color(const or_csr *A, uint64_t seed, int32_t *color, int32_t *ncolors) {
+    const int32_t n = A->nrows;
+    int32_t *next = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t i = 0; i < n; ++i) color[i] = -1;
+    int32_t left = n, maxc = -1;
+    while (left > 0) {
+        for (int32_t i = 0; i < n; ++i) {
+            next[i] = color[i];
+            if (color[i] >= 0) continue;
+            const uint64_t hi = jp_mix((uint64_t)i ^ seed);
+            int best = 1;
+            for (int32_t e = A->rowptr[i]; e < A->rowptr[i + 1] && best; ++e) {
+                const int32_t j = A->col[e];
+                if (j == i || color[j] >= 0) continue;
+                const uint64_t hj = jp_mix((uint64_t)j ^ seed);
+                if (hj > hi || (hj == hi && j > i)) best = 0;
+            }
+            if (!best) continue;
+            int32_t c = 0;
+            for (;;) {
+                int used = 0;
+                for (int32_t e = A->rowptr[i]; e < A->rowptr[i + 1] && !used; ++e)
+                    used = (A->col[e] != i) && (color[A->col[e]] == c);
+                if (!used) break;
+                ++c;
+            }
+            next[i] = c;
+        }
+        for (int32_t i = 0; i < n; ++i) {
+            if (color[i] < 0 && next[i] >= 0) {
+                --left;
+                if (next[i] > maxc) maxc = next[i];
+            }
+            color[i] = next[i];
+        }
+    }
+    *ncolors = maxc + 1;
+    free(next);
+}
+
 /* GaussSeidel::smooth with local_sweeps (solvers/smoothers/utopia_GaussSeidel.hpp:45-57,133-183):
  * per sweep r = rhs - A x; c = 0; one forward then (symmetric) one backward pass,
  * c_i = d_inv_i (r_i - sum_{j != i} a_ij c_j); x += c. The row order of a pass is
@@ -889,6 +942,164 @@ done:
 }
 
 /* ------------------------------------------------------ ProjectedGaussSeidel */
+
+/* ---------------------------------------------------- MonotoneMultigrid */
+
+/* One projected Gauss-Seidel step in colour order (ProjectedGaussSeidel::step,
+ * backend/petsc/.../utopia_petsc_ProjectedGaussSeidel.cpp:331-412, rows visited
+ * colour by colour as the GPU's multicolour sweep): r = b - A x; c = 0; symmetric
+ * projected sweeps with bounds lb - x, ub - x; x += c. */
+static void pgs_colored_step(const or_csr *A, const double *dinv, const int32_t *color, int32_t ncolors,
+                             const double *b, const double *lb, const double *ub, int constrained, double *x,
+                             double *r, double *c) {
+    const int32_t n = A->nrows;
+    or_spmv(A, x, r);
+    for (int32_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    memset(c, 0, sizeof(double) * (size_t)n);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int32_t t = 0; t < ncolors; ++t) {
+            const int32_t k = pass == 0 ? t : ncolors - 1 - t;
+            for (int32_t i = 0; i < n; ++i) {
+                if (color[i] != k) continue;
+                const double lbl = lb[i] - x[i], ubl = ub[i] - x[i];
+                double val = r[i];
+                if (constrained && dinv[i] <= 0.0) {
+                    c[i] = (val == 0.0) ? 0.0 : ((val > 0.0) ? lbl : ubl);
+                } else {
+                    for (int32_t j = A->rowptr[i]; j < A->rowptr[i + 1]; ++j)
+                        if (A->col[j] != i && A->val[j] != 0.0) val -= A->val[j] * c[A->col[j]];
+                    /* unconstrained_step (:414-455): no projection */
+                    c[i] = constrained ? fmax(lbl, fmin(dinv[i] * val, ubl)) : dinv[i] * val;
+                }
+            }
+        }
+    for (int32_t i = 0; i < n; ++i) x[i] = x[i] + c[i];
+}
+
+typedef struct {
+    int nlevels;
+    const or_csr *const *interp;
+    int pre, post;
+    or_csr It, Rt, AH;
+    or_mg *mg;   /* coarse-space Multigrid (nlevels > 2) */
+    double *chol; /* dense Cholesky of AH (nlevels == 2) */
+} mmg_state;
+
+/* galerkin_assembly (utopia_MonotoneMultigrid.hpp:217-231) */
+static void mmg_galerkin(mmg_state *m, const or_csr *A) {
+    or_csr_free(&m->Rt);
+    or_csr_free(&m->AH);
+    or_transpose(&m->It, &m->Rt);
+    or_ptap(A, &m->It, &m->AH);
+    fix_semidefinite(&m->AH);
+    if (m->nlevels > 2) {
+        if (m->mg) or_mg_destroy(m->mg);
+        /* Multigrid(GaussSeidel, coarse solver): masks off, semidefinite fix on (:59-62) */
+        m->mg = or_mg_create(m->nlevels - 1, m->interp, 2, m->pre, m->post, 1, 0, 1);
+        or_mg_update(m->mg, &m->AH);
+        for (int l = 1; l < m->nlevels - 1; ++l) {
+            const or_csr *G = or_mg_level(m->mg, l);
+            int32_t *col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(G->nrows > 0 ? G->nrows : 1));
+            int32_t nc = 0;
+            or_jp_color(G, 0x5eedull, col, &nc);
+            or_mg_set_colors(m->mg, l, col, nc);
+            free(col);
+        }
+    } else {
+        const int32_t nh = m->AH.nrows;
+        free(m->chol);
+        m->chol = (double *)calloc((size_t)nh * nh + 1, sizeof(double));
+        for (int32_t i = 0; i < nh; ++i)
+            for (int32_t j = m->AH.rowptr[i]; j < m->AH.rowptr[i + 1]; ++j)
+                m->chol[(int64_t)i * nh + m->AH.col[j]] += m->AH.val[j];
+        or_cholesky(nh, m->chol);
+    }
+}
+
+/* MonotoneMultigrid::apply / step (solvers/multilevel/utopia_MonotoneMultigrid.hpp:158-318)
+ * with ProjectedGaussSeidel as the QP smoother, ActiveSet tol 0, the finest
+ * interpolation truncated (IPTruncatedTransfer), coarse_space_it Multigrid cycles
+ * (GaussSeidel smoothing, dense Cholesky coarse solve, IterativeSolver tolerances
+ * 1e-9) or, with two levels, the coarse solve. interp: nlevels-1 interpolations,
+ * coarse to fine. */
+int or_mmg_solve(const or_csr *A, const double *b, const double *lb_in, const double *ub_in, double *x,
+                 int nlevels, const or_csr *const *interp, int pre, int post, int64_t coarse_space_it,
+                 double active_tol, double atol, double rtol, double stol, int64_t max_it, or_status *st) {
+    const int32_t n = A->nrows;
+    const int has_bound = lb_in || ub_in;
+    double *lb = vnew(n), *ub = vnew(n), *dinv = vnew(n), *r = vnew(n), *c = vnew(n), *ind = vnew(n),
+           *x_old = vnew(n), *help = vnew(n), *grad = vnew(n);
+    int32_t *color = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t ncolors = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        lb[i] = lb_in ? lb_in[i] : -DBL_MAX;
+        ub[i] = ub_in ? ub_in[i] : DBL_MAX;
+    }
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t j = A->rowptr[i]; j < A->rowptr[i + 1]; ++j)
+            if (A->col[j] == i) dinv[i] = (fabs(A->val[j]) > 0.0) ? (1 / A->val[j]) : 0.0;
+    or_jp_color(A, 0x5eedull, color, &ncolors);
+    mmg_state m;
+    memset(&m, 0, sizeof(m));
+    m.nlevels = nlevels;
+    m.interp = interp;
+    m.pre = pre;
+    m.post = post;
+    csr_copy(interp[nlevels - 2], &m.It);
+    mmg_galerkin(&m, A);
+    memset(st, 0, sizeof(*st));
+    vcopy(n, x, x_old);
+    int converged = 0;
+    for (int64_t it = 1; it < max_it && !converged; ++it) {
+        for (int s = 0; s < (has_bound ? pre : pre + 1); ++s)
+            pgs_colored_step(A, dinv, color, ncolors, b, lb, ub, has_bound, x, r, c);
+        if (has_bound) { /* ActiveSet::determine (utopia_ActiveSet.hpp:41-79) */
+            int changed = 0;
+            for (int32_t i = 0; i < n; ++i) {
+                const double v = (x[i] <= lb[i] + active_tol || x[i] >= ub[i] - active_tol) ? 1.0 : 0.0;
+                changed += v != ind[i];
+                ind[i] = v;
+            }
+            if (changed) {
+                or_csr_free(&m.It);
+                csr_copy(interp[nlevels - 2], &m.It);
+                for (int32_t i = 0; i < m.It.nrows; ++i)
+                    if (ind[i] == 1.0)
+                        for (int32_t j = m.It.rowptr[i]; j < m.It.rowptr[i + 1]; ++j) m.It.val[j] = 0.0;
+                mmg_galerkin(&m, A);
+            }
+        }
+        or_spmv(A, x, r);
+        for (int32_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+        const int32_t nh = m.Rt.nrows;
+        double *rc = vnew(nh), *cc = vnew(nh), *cf = vnew(n);
+        or_spmv(&m.Rt, r, rc);
+        if (nlevels > 2) or_mg_apply(m.mg, rc, cc, coarse_space_it, 1e-9, 1e-9);
+        else or_cholesky_solve(nh, m.chol, rc, cc);
+        or_spmv(&m.It, cc, cf);
+        for (int32_t i = 0; i < n; ++i) x[i] = x[i] + cf[i];
+        free(rc), free(cc), free(cf);
+        for (int s = 0; s < (has_bound ? post : post + 1); ++s)
+            pgs_colored_step(A, dinv, color, ncolors, b, lb, ub, has_bound, x, r, c);
+        for (int32_t i = 0; i < n; ++i) help[i] = x[i] - x_old[i];
+        const double x_diff = or_norm2(n, help);
+        or_spmv(A, x, grad);
+        for (int32_t i = 0; i < n; ++i) grad[i] = grad[i] - b[i];
+        for (int32_t i = 0; i < n; ++i) {
+            const double v = x[i] - grad[i];
+            help[i] = ((lb[i] >= v) ? lb[i] : ((ub[i] <= v) ? ub[i] : v)) - x[i];
+        }
+        const double g_norm = or_norm2(n, help);
+        converged = check_convergence(it, g_norm, 1, x_diff, atol, rtol, stol, max_it, st);
+        if (!converged) vcopy(n, x, x_old);
+    }
+    st->converged = converged;
+    if (m.mg) or_mg_destroy(m.mg);
+    or_csr_free(&m.It), or_csr_free(&m.Rt), or_csr_free(&m.AH);
+    free(m.chol);
+    free(lb), free(ub), free(dinv), free(r), free(c), free(ind), free(x_old), free(help), free(grad), free(color);
+    return converged;
+}
 
 /* ProjectedConjugateGradient::apply
  * (solvers/nonlinear/constrained/quadratic_programming/utopia_ProjectedConjugateGradient.hpp:33-123).

@@ -69,6 +69,8 @@ const double *or_mg_mask(const or_mg *mg, int l);
 void or_mg_eigs(const or_mg *mg, int l, double *eig_max, double *eig_min);
 /* Override the Chebyshev bounds of level l (tests comparing cycles bit-for-bit). */
 void or_mg_set_eigs(or_mg *mg, int l, double eig_max, double eig_min);
+/* Jones-Plassmann colouring (smallest available colour, splitmix64 hash of i ^ seed) */
+void or_jp_color(const or_csr *A, uint64_t seed, int32_t *color, int32_t *ncolors);
 /* smoother 2 (multicolour Gauss-Seidel): the colouring of level l, after or_mg_update */
 void or_mg_set_colors(or_mg *mg, int l, const int32_t *color, int32_t ncolors);
 void or_gs_smooth(const or_csr *A, const double *dinv, const int32_t *color, int32_t ncolors, const double *rhs,
@@ -83,6 +85,11 @@ void or_chebyshev_smooth(const or_csr *A, const double *b, double *x, int nu, do
 /* MPRGP (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:57-387) */
 int or_mprgp_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double atol,
                    double rtol, int64_t max_it, double eps_eig_est, int64_t power_method_max_it, or_status *st);
+/* MonotoneMultigrid with multicolour projected Gauss-Seidel (JP colouring, seed 0x5eed),
+ * GaussSeidel-smoothed coarse Multigrid; interp: nlevels-1 interpolations, coarse to fine */
+int or_mmg_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, int nlevels,
+                 const or_csr *const *interp, int pre, int post, int64_t coarse_space_it, double active_tol,
+                 double atol, double rtol, double stol, int64_t max_it, or_status *st);
 /* ProjectedConjugateGradient / ProjectedGradient (quadratic_programming/); NULL bound = none */
 int or_pcg_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double atol,
                  double rtol, double stol, int64_t max_it, or_status *st);

@@ -62,12 +62,14 @@ def lib():
             "or_mg_set_eigs": (None, [P, I, D, D]),
             "or_mg_set_colors": (None, [P, I, P, ctypes.c_int32]),
             "or_gs_smooth": (None, [P, P, P, ctypes.c_int32, P, P, I, I]),
+            "or_jp_color": (None, [P, ctypes.c_uint64, P, P]),
             "or_mg_destroy": (None, [P]),
             "or_power_method": (D, [P, D, I64]),
             "or_chebyshev_smooth": (None, [P, P, P, I, D, D]),
             "or_mprgp_solve": (I, [P, P, P, P, P, D, D, I64, D, I64, P]),
             "or_pgs_solve": (I, [P, P, P, P, P, D, I64, I, I64, P]),
             "or_pcg_solve": (I, [P, P, P, P, P, D, D, D, I64, P]),
+            "or_mmg_solve": (I, [P, P, P, P, P, I, P, I, I, I64, D, D, D, D, I64, P]),
             "or_pg_solve": (I, [P, P, P, P, P, D, D, D, I64, P]),
             "or_set_threads": (None, [I]),
             "or_get_threads": (I, []),
@@ -265,6 +267,15 @@ class MG:
         lib().or_mg_set_colors(self.h, l, _p(color), int(ncolors))
 
 
+def jp_color(A, seed=0x5eed):
+    """Jones-Plassmann colouring: (colour per row, number of colours)."""
+    M = Mat.of(A)
+    color = np.zeros(max(A.shape[0], 1), np.int32)
+    nc = ctypes.c_int32()
+    lib().or_jp_color(M.ref, ctypes.c_uint64(seed), _p(color), ctypes.byref(nc))
+    return color[:A.shape[0]], nc.value
+
+
 def gs_smooth(A, dinv, color, ncolors, b, x0=None, sweeps=1, symmetric=True):
     """Multicolour symmetric Gauss-Seidel sweeps (GaussSeidel::smooth)."""
     M = Mat.of(A)
@@ -297,6 +308,22 @@ def _qp_call(fn, A, b, lb, ub, x0, atol, rtol, stol, max_it):
     ub = None if ub is None else np.ascontiguousarray(ub, np.float64)
     st = or_status()
     fn(M.ref, _p(b), _p(lb), _p(ub), _p(x), atol, rtol, stol, max_it, ctypes.byref(st))
+    return x, status_dict(st)
+
+
+def mmg_solve(A, b, interps, lb=None, ub=None, x0=None, pre=3, post=3, coarse_space_it=1, active_set_tol=0.0,
+              atol=1e-9, rtol=1e-9, stol=1e-11, max_it=1000):
+    """MonotoneMultigrid (interps coarse to fine)."""
+    M = Mat.of(A)
+    Is = [Mat.of(I) for I in interps]
+    arr = (ctypes.c_void_p * len(Is))(*[ctypes.addressof(m.s) for m in Is])
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+    lb = None if lb is None else np.ascontiguousarray(lb, np.float64)
+    ub = None if ub is None else np.ascontiguousarray(ub, np.float64)
+    st = or_status()
+    lib().or_mmg_solve(M.ref, _p(b), _p(lb), _p(ub), _p(x), len(Is) + 1, arr, pre, post, coarse_space_it,
+                       active_set_tol, atol, rtol, stol, max_it, ctypes.byref(st))
     return x, status_dict(st)
 
 

@@ -399,6 +399,16 @@ def test_dense_cholesky_reports_indefinite(ctx):
     assert info.value == 3
 
 
+def test_graph_coloring_equals_oracle(ctx):
+    for A in (ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 12, 11, 9, seed=2), ua.gen_stencil(ua.STENCIL_2D9, 40, 31)):
+        D = ua.DeviceCsr(ctx, A)
+        col = ua.DeviceArray(ctx, A.shape[0], np.int32)
+        nc = ctypes.c_int32()
+        ua.check(L().uhip_graph_color(ctx.h, D.ref, ctypes.c_uint64(0x5eed), P(col), ctypes.byref(nc)))
+        co, nco = orc.jp_color(A, 0x5eed)
+        assert nc.value == nco and np.array_equal(col.numpy(), co)
+
+
 def test_graph_coloring_is_proper(ctx):
     A = ua.gen_stencil(ua.STENCIL_2D9, 60, 50, shift=1e-3)
     D = ua.DeviceCsr(ctx, A)

@@ -371,3 +371,53 @@ def test_error_paths_report_status(ctx):
     cg = ua.ConjugateGradient(ctx, "jacobi", atol=1e-10, max_it=50)
     cg.solve(A, b)
     assert cg.status["iterates"] == 0
+
+
+def _monotone_mg_problem(n_levels=6, n_coarse=50):
+    from test_oracle_pins import _monotone_mg_problem as mk
+    return mk(n_levels, n_coarse)
+
+
+@pytest.mark.parametrize("levels", [6, 2])
+def test_monotone_multigrid_matches_oracle_on_reference_problem(ctx, levels):
+    """QPSolverTest::monotone_mg_test: 1601 dofs, bounds [-0.8, 200], 5/5 projected
+    Gauss-Seidel steps, active-set tol 1e-15, max_it 40 (GaussSeidel coarse smoothing,
+    dense coarse solve)."""
+    A, b, Is, lb, ub = _monotone_mg_problem(6, 50)
+    Is = Is[6 - levels:]
+    q = ua.MonotoneMultigrid(ctx, pre_smoothing_steps=5, post_smoothing_steps=5, active_set_tol=1e-15, max_it=40,
+                             atol=1e-9, rtol=1e-9, stol=1e-11)
+    q.set_transfer_operators(Is)
+    q.update(A)
+    q.set_box_constraints(lb, ub)
+    x = q.apply(b)
+    xo, so = orc.mmg_solve(A, b, Is, lb=lb, ub=ub, pre=5, post=5, active_set_tol=1e-15, max_it=40)
+    st = q.status
+    print(f"mmg {levels} levels: gpu it={st['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(st["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-9
+    act, act_o = np.abs(x - lb) < 1e-12, np.abs(xo - lb) < 1e-12
+    assert act.sum() > 0 and np.array_equal(act, act_o)
+    assert np.all(x >= lb)
+    assert q.info("active_set_changes") >= 1
+
+
+def test_monotone_multigrid_2d_obstacle_matches_oracle(ctx):
+    """2D nine-point obstacle problem on a 3-level hierarchy."""
+    n = 64
+    A = ua.gen_stencil(ua.STENCIL_2D9, n, n, shift=1e-3)
+    b = ua.gen_uniform(21, n * n)
+    lb = -0.05 - 0.15 * ua.gen_uniform(22, n * n, 0, 1)
+    Is = ua.mg_hierarchy(n, n, 1, 3)
+    q = ua.MonotoneMultigrid(ctx, pre_smoothing_steps=3, post_smoothing_steps=3, coarse_space_it=2, max_it=200,
+                             atol=1e-10, rtol=0, stol=0)
+    q.set_transfer_operators(Is)
+    q.update(A)
+    q.set_box_constraints(lb, None)
+    x = q.apply(b)
+    xo, so = orc.mmg_solve(A, b, Is, lb=lb, pre=3, post=3, coarse_space_it=2, atol=1e-10, rtol=0, stol=0, max_it=200)
+    st = q.status
+    print(f"mmg 2d 64^2: gpu it={st['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert st["reason"] > 0 and abs(st["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-8
+    _kkt(A, b, x, lb, 1e-8)

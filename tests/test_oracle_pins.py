@@ -429,3 +429,158 @@ def test_projected_cg_is_rounding_sensitive():
     x0, _ = orc.pcg_solve(A, b, ub=ub, max_it=10, stol=0)
     x1, _ = orc.pcg_solve(A, bb, ub=ub, max_it=10, stol=0)
     assert np.linalg.norm(x1 - x0) < 1e-12 * np.linalg.norm(x0)
+
+
+def test_jones_plassmann_colouring_is_proper_and_small():
+    for A in (ua.gen_stencil(ua.STENCIL_2D5, 17, 11), ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 9, 8, 7, seed=1),
+              ua.gen_stencil(ua.STENCIL_2D9, 13, 12)):
+        color, nc = orc.jp_color(A)
+        G = A.to_scipy().tocoo()
+        off = G.row != G.col
+        assert not np.any(color[G.row[off]] == color[G.col[off]])
+        deg = np.diff(A.rowptr).max() - 1
+        assert color.min() == 0 and color.max() == nc - 1 and nc <= deg + 1
+
+
+def _monotone_mg_problem(n_levels=6, n_coarse=50):
+    """QPSolverTest::monotone_mg_test (tests/utopia_QPSolverTest.cpp:718-777) on
+    MultiLevelTestProblem1D<Poisson1D> (tests/test_problems/large_scale_benchmark/
+    utopia_MultilevelTestProblem1D.hpp:56-193, utopia_Poisson1D.hpp:101-290, problem
+    type 2): dofs (n_coarse + 1) 2^l + ..., H = tridiag(-1, 2, -1) / h with identity
+    boundary rows (explicit zeros kept), x0 = 0, rhs = g = H x0 - f = -10 h inside,
+    interpolations with rows 2j -> (j, 1) and 2j+1 -> (j, .5), (j+1, .5), the finest with
+    its two boundary entries zeroed (remove_bc), bounds [-0.8, 200].
+    Returns A, b, interps (coarse to fine), lb, ub."""
+    dofs = [n_coarse + 1]
+    for _ in range(1, n_levels):
+        dofs.append((dofs[-1] - 1) * 2 + 1)
+    interps = []
+    for l in range(n_levels - 1):
+        nc, nf = dofs[l], dofs[l + 1]
+        rp, col, val = [0], [], []
+        for k in range(nf):
+            j = k // 2
+            if k % 2 == 0:
+                col.append(j)
+                val.append(1.0)
+            elif j + 1 < nc:
+                col += [j, j + 1]
+                val += [0.5, 0.5]
+            rp.append(len(col))
+        val = np.array(val)
+        if l == n_levels - 2:
+            val[rp[0]] = 0.0            # I(0, 0)
+            val[rp[nf - 1]] = 0.0       # I(last_h, last_H)
+        interps.append(ua.Csr(np.array(rp, np.int32), np.array(col, np.int32), val, (nf, nc)))
+    n = dofs[-1]
+    h = 1.0 / (n - 1)
+    rp, col, val = [0], [], []
+    for i in range(n):
+        if i == 0:
+            col += [0, 1]
+            val += [1.0, 0.0]
+        elif i == n - 1:
+            col += [n - 2, n - 1]
+            val += [0.0, 1.0]
+        else:
+            col += [i - 1, i, i + 1]
+            val += [-1.0 / h, 2.0 / h, -1.0 / h]
+        rp.append(len(col))
+    A = ua.Csr(np.array(rp, np.int32), np.array(col, np.int32), np.array(val), (n, n))
+    b = np.full(n, -10.0 * h)
+    b[0] = b[-1] = 0.0
+    return A, b, interps, np.full(n, -0.8), np.full(n, 200.0)
+
+
+def _obstacle_1d_bruteforce(Ad, b, lb):
+    """The unique minimiser of 1/2 x'Ax - b'x over x >= lb for a tridiagonal M-matrix:
+    its contact set is an interval (discrete maximum principle), so every interval
+    [a, e) is tried and the one meeting the KKT conditions is returned."""
+    n = b.size
+    for a in range(n + 1):
+        for e in range(a, n + 1):
+            act = np.zeros(n, bool)
+            act[a:e] = True
+            x = lb.copy()
+            free = ~act
+            x[free] = np.linalg.solve(Ad[np.ix_(free, free)], b[free] - Ad[np.ix_(free, act)] @ lb[act])
+            g = Ad @ x - b
+            if np.all(x >= lb - 1e-12) and np.all(g[act] >= -1e-10) and np.all(np.abs(g[free]) < 1e-10):
+                return x, act
+    return None, None
+
+
+def test_monotone_mg_solves_reference_obstacle_problem():
+    A, b, Is, lb, ub = _monotone_mg_problem(3, 10)
+    Ad = A.to_scipy().toarray()
+    x_ref, act = _obstacle_1d_bruteforce(Ad, b, lb)
+    assert x_ref is not None and 0 < act.sum() < b.size - 2
+    for levels in (2, 3):
+        x, st = orc.mmg_solve(A, b, Is[3 - levels:], lb=lb, ub=ub, pre=5, post=5, active_set_tol=1e-15, atol=1e-12,
+                              rtol=0, stol=0, max_it=40)
+        assert st["converged"] and st["iterates"] < 40
+        assert np.abs(x - x_ref).max() < 1e-10
+
+
+def _colour_sgs(A, x, b, steps):
+    """Symmetric Gauss-Seidel in the Jones-Plassmann colour order as two sparse
+    triangular solves of the colour-permuted system: forward (D + L) c1 = r,
+    backward (D + U) c = r - L c1; x += c."""
+    color, _ = orc.jp_color(A)
+    perm = np.argsort(color, kind="stable")
+    M = A.to_scipy()[perm][:, perm].tocsr()
+    D = sp.diags(M.diagonal())
+    L, U = sp.tril(M, -1), sp.triu(M, 1)
+    for _ in range(steps):
+        r = (b - A.to_scipy() @ x)[perm]
+        c1 = spla.spsolve_triangular((D + L).tocsr(), r, lower=True)
+        c = spla.spsolve_triangular((D + U).tocsr(), r - L @ c1, lower=False)
+        x = x.copy()
+        x[perm] += c
+    return x
+
+
+def test_monotone_mg_unconstrained_step_is_two_grid_cycle():
+    """Without bounds one MonotoneMultigrid step is pre+1 SGS steps, the exact
+    Galerkin coarse correction I (I'AI)^-1 I'r, and post+1 SGS steps."""
+    A, b, Is, _, _ = _monotone_mg_problem(3, 10)
+    x0 = np.sin(np.arange(b.size))
+    x, _ = orc.mmg_solve(A, b, Is[-1:], x0=x0, pre=2, post=1, atol=0, rtol=0, stol=0, max_it=2)
+    S = A.to_scipy()
+    I = Is[-1].to_scipy().toarray()
+    y = _colour_sgs(A, x0, b, 3)
+    AH = I.T @ S.toarray() @ I
+    y = y + I @ np.linalg.solve(AH, I.T @ (b - S @ y))
+    y = _colour_sgs(A, y, b, 2)
+    assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max()
+
+
+def test_monotone_mg_coarse_correction_is_truncated_on_the_active_set():
+    """pre = post = 0: one step is the coarse correction through the finest
+    interpolation with the rows of active nodes zeroed; active nodes do not move."""
+    A, b, Is, lb, ub = _monotone_mg_problem(3, 10)
+    n = b.size
+    x0 = 0.1 * np.cos(np.arange(n))
+    act = (np.arange(n) % 3 == 1) | ((np.arange(n) > 15) & (np.arange(n) < 24))
+    x0[act] = lb[act]
+    x, _ = orc.mmg_solve(A, b, Is[-1:], lb=lb, ub=ub, x0=x0, pre=0, post=0, atol=0, rtol=0, stol=0, max_it=2)
+    S = A.to_scipy().toarray()
+    It = Is[-1].to_scipy().toarray()
+    It[act] = 0.0
+    AH = It.T @ S @ It
+    dead = np.abs(np.diag(AH)) < 1e-12
+    assert dead.any()  # coarse nodes whose children are all active
+    AH[dead, dead] += 1.0
+    y = x0 + It @ np.linalg.solve(AH, It.T @ (b - S @ x0))
+    assert np.array_equal(x[act], x0[act])
+    assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max()
+
+
+def test_monotone_mg_coarse_space_multigrid_tends_to_exact_coarse_solve():
+    A, b, Is, lb, ub = _monotone_mg_problem(5, 6)
+    kw = dict(lb=lb, ub=ub, pre=1, post=1, atol=0, rtol=0, stol=0, max_it=2)
+    x2, _ = orc.mmg_solve(A, b, Is[-1:], **kw)
+    x5, _ = orc.mmg_solve(A, b, Is, coarse_space_it=80, **kw)
+    x5_1, _ = orc.mmg_solve(A, b, Is, coarse_space_it=1, **kw)
+    assert np.abs(x5 - x2).max() <= 1e-8 * np.abs(x2).max()
+    assert np.abs(x5_1 - x2).max() > 1e-6 * np.abs(x2).max()

@@ -37,7 +37,7 @@ _NP = {F64: np.float64, F32: np.float32}
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
 (SOLVER_CG, SOLVER_CG_JACOBI, SOLVER_CG_MG, SOLVER_CG_MG_MIXED, SOLVER_MG, SOLVER_MPRGP, SOLVER_PGS,
- SOLVER_PCG, SOLVER_PG) = range(9)
+ SOLVER_PCG, SOLVER_PG, SOLVER_MMG) = range(10)
 
 # ConvergenceReason (solvers/utopia_ConvergenceReason.hpp:84-100)
 CONVERGED_FNORM_ABS, CONVERGED_FNORM_RELATIVE, CONVERGED_SNORM_RELATIVE, DIVERGED_MAX_IT = 2, 3, 4, -5
@@ -451,6 +451,12 @@ class ProjectedConjugateGradient(Solver):
 
 class ProjectedGradient(Solver):
     TYPE = SOLVER_PG
+
+
+class MonotoneMultigrid(Solver):
+    """MonotoneMultigrid: set_transfer_operators (coarse to fine; the finest is
+    truncated on the active set) before update."""
+    TYPE = SOLVER_MMG
 
 
 def srand(seed: int = 1):

@@ -430,6 +430,27 @@ public:
     // FP32 copy of the values, same pattern.
     HipCrsMatrixT<float> cast_f32() const;
 
+    // Deep copy (same dtype and pattern), automatic plan.
+    HipCrsMatrixT copy() const {
+        uhip_csr m = m_;
+        void *p = nullptr;
+        const size_t vb = sizeof(T);
+        hip_check(uhip_malloc(c(), &p, sizeof(int32_t) * ((size_t)m_.nrows + 1)), "alloc rowptr");
+        m.rowptr = (int32_t *)p;
+        hip_check(uhip_malloc(c(), &p, sizeof(int32_t) * (size_t)(m_.nnz > 0 ? m_.nnz : 1)), "alloc col");
+        m.col = (int32_t *)p;
+        hip_check(uhip_malloc(c(), &p, vb * (size_t)(m_.nnz > 0 ? m_.nnz : 1)), "alloc val");
+        m.val = p;
+        hip_check(uhip_memcpy_dtod(c(), m.rowptr, m_.rowptr, sizeof(int32_t) * ((size_t)m_.nrows + 1)), "copy rowptr");
+        hip_check(uhip_memcpy_dtod(c(), m.col, m_.col, sizeof(int32_t) * (size_t)m_.nnz), "copy col");
+        hip_check(uhip_memcpy_dtod(c(), m.val, m_.val, vb * (size_t)m_.nnz), "copy val");
+        HipCrsMatrixT o;
+        o.adopt(ctx_, m, 0);
+        return o;
+    }
+    // Re-plan after the values changed in place (DIA / SELL-64 plans copy them).
+    void replan() { plan(m_.format == UHIP_FMT_ROWBLOCK ? (m_.block_capk == 5 ? 1024 : (m_.block_capk == 20 ? 4096 : 2048)) : 0); }
+
 private:
     void release() {
         if (!ctx_) return;

@@ -188,6 +188,19 @@ public:
     }
     ~ProjectedGaussSeidel() override { free_groups(); }
 
+    // ProjectedGaussSeidel::smooth (utopia_petsc_ProjectedGaussSeidel.cpp:111-132): with bounds,
+    // `sweeps` projected steps; without, the reference loop
+    // `while (unconstrained_step(...) && it++ < sweeps)` runs sweeps + 1 steps.
+    bool smooth(const HipVector &b, HipVector &x, int64_t sweeps) {
+        const HipCrsMatrix &A = *this->op_;
+        if (x.empty() || x.size() != A.rows()) x.zeros_like(b);
+        const bool constrained = box_.has_bound();
+        if (constrained) box_.fill_empty_bounds(A.context(), A.rows());
+        const int64_t steps = constrained ? sweeps : sweeps + 1;
+        for (int64_t s = 0; s < steps; ++s) step(A, b, x, constrained);
+        return true;
+    }
+
     // apply (utopia_petsc_ProjectedGaussSeidel.cpp:134-178)
     bool apply(const HipVector &b, HipVector &x) override {
         const HipCrsMatrix &A = *this->op_;

@@ -189,6 +189,32 @@ __global__ __launch_bounds__(kBlock) void k_mprgp_prop(int64_t n, double *__rest
     }
 }
 
+// ActiveSet::determine (solvers/nonlinear/constrained/utopia_ActiveSet.hpp:41-79):
+// ind_i = (x_i <= l_i + tol or x_i >= u_i - tol); partial count of changed entries.
+__global__ __launch_bounds__(kBlock) void k_active_set(int64_t n, const double *__restrict__ x,
+                                                       const double *__restrict__ lb, const double *__restrict__ ub,
+                                                       double tol, double *__restrict__ ind,
+                                                       double *__restrict__ partials) {
+    double changed = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double xi = x[i];
+        const double v = (xi <= lb[i] + tol || xi >= ub[i] - tol) ? 1.0 : 0.0;
+        changed += (v != ind[i]) ? 1.0 : 0.0;
+        ind[i] = v;
+    }
+    changed = block_sum<kBlock>(changed);
+    if (threadIdx.x == 0) partials[blockIdx.x] = changed;
+}
+
+// IPTruncatedTransfer::truncate_interpolation (transfer/utopia_IPTruncatedTransfer.hpp:203-223):
+// the values of rows i with ind_i == 1 become zero (pattern kept).
+__global__ __launch_bounds__(kBlock) void k_zero_rows(int32_t n, const int32_t *__restrict__ rowptr,
+                                                      const double *__restrict__ ind, double *__restrict__ val) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n || ind[r] != 1.0) return;
+    for (int32_t e = rowptr[r]; e < rowptr[r + 1]; ++e) val[e] = 0.0;
+}
+
 // ---- multicolour projected Gauss-Seidel ----
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -294,6 +320,26 @@ __global__ __launch_bounds__(kBlock) void k_pgs_dinv(int32_t n, const int32_t *_
 using namespace uhip;
 
 extern "C" {
+
+int uhip_active_set_f64(uhip_ctx ctx, int64_t n, const double *x, const double *lb, const double *ub, double tol,
+                        double *indicator, double *changed_d) {
+    UHIP_REQUIRE(ctx && x && lb && ub && indicator && changed_d, "uhip_active_set_f64: null argument");
+    const int g = grid_for(n, 4, kMaxRedBlocks);
+    UHIP_TRY(ensure_partials(ctx, g));
+    hipLaunchKernelGGL(k_active_set, dim3(g), dim3(kBlock), 0, ctx->stream, n, x, lb, ub, tol, indicator,
+                       ctx->partials);
+    UHIP_CHECK_LAUNCH();
+    return finish_sums(ctx, 1, g, ctx->max_partials, changed_d, 0u);
+}
+
+int uhip_csr_zero_rows(uhip_ctx ctx, uhip_csr *A, const double *indicator) {
+    UHIP_REQUIRE(ctx && A && indicator && A->dtype == UHIP_F64, "uhip_csr_zero_rows: bad argument");
+    if (A->nrows == 0) return UHIP_OK;
+    hipLaunchKernelGGL(k_zero_rows, dim3((A->nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, A->nrows,
+                       A->rowptr, indicator, (double *)A->val);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
 
 int uhip_pcg_wz_f64(uhip_ctx ctx, int64_t n, const double *u, const double *lb, const double *ub, const double *p,
                     double *z, double *w) {

@@ -7,6 +7,7 @@
 #include <map>
 #include <string>
 
+#include "host/utopia_hip_mmg.hpp"
 #include "host/utopia_hip_multigrid.hpp"
 #include "host/utopia_hip_qp.hpp"
 
@@ -24,7 +25,8 @@ struct uhip_solver_s {
     std::shared_ptr<MixedPrecisionMultigrid> mixed;
     std::shared_ptr<MPRGP> mprgp;
     std::shared_ptr<ProjectedGaussSeidel> pgs;
-    std::shared_ptr<QPSolverBase> qp;  // the QP solver of the MPRGP / PGS / PCG / PG types
+    std::shared_ptr<QPSolverBase> qp;  // the QP solver of the MPRGP / PGS / PCG / PG / MMG types
+    std::shared_ptr<MonotoneMultigrid> mmg;
     HipVector lb, ub;
     bool has_lb = false, has_ub = false;
     SolutionStatus status;
@@ -91,6 +93,7 @@ void build(uhip_solver_s *s) {
     s->mprgp.reset();
     s->pgs.reset();
     s->qp.reset();
+    s->mmg.reset();
     switch (s->type) {
         case UHIP_SOLVER_CG:
         case UHIP_SOLVER_CG_JACOBI:
@@ -145,6 +148,20 @@ void build(uhip_solver_s *s) {
             s->solver = q;
             break;
         }
+        case UHIP_SOLVER_MMG: {
+            auto q = std::make_shared<MonotoneMultigrid>();
+            apply_iterative_params(s, *q);
+            q->pre_smoothing_steps((int)param(s, "pre_smoothing_steps", 3));
+            q->post_smoothing_steps((int)param(s, "post_smoothing_steps", 3));
+            q->coarse_space_it((int64_t)param(s, "coarse_space_it", 1));
+            q->active_set_tol(param(s, "active_set_tol", 0.0));
+            if (s->transfers.empty()) throw HipError(UHIP_ERR_INVALID, "MonotoneMultigrid: set_transfer_operators first");
+            q->set_transfer_operators(s->transfers);
+            s->mmg = q;
+            s->qp = q;
+            s->solver = q;
+            break;
+        }
         case UHIP_SOLVER_PCG: {
             auto q = std::make_shared<ProjectedConjugateGradient>();
             apply_iterative_params(s, *q);
@@ -187,7 +204,7 @@ extern "C" {
 
 int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out) {
     if (!ctx || !out) return UHIP_ERR_INVALID;
-    if (type < UHIP_SOLVER_CG || type > UHIP_SOLVER_PG) return UHIP_ERR_INVALID;
+    if (type < UHIP_SOLVER_CG || type > UHIP_SOLVER_MMG) return UHIP_ERR_INVALID;
     try {
         auto *s = new uhip_solver_s();
         s->type = type;
@@ -352,6 +369,7 @@ int uhip_solver_get_info(uhip_solver s, const char *key, double *value) {
     else if (k == "coarse_n") *value = s->mg ? s->mg->level_operator(0).rows() : 0;
     else if (k == "ncolors" && s->pgs) *value = s->pgs->ncolors();
     else if (k == "alpha_bar" && s->mprgp) *value = s->mprgp->alpha_bar();
+    else if (k == "active_set_changes" && s->mmg) *value = (double)s->mmg->active_set_changes();
     else if (k.rfind("eig_max_", 0) == 0 && s->mg) *value = s->mg->level_eig_max(std::stoi(k.substr(8)));
     else if (k.rfind("eig_min_", 0) == 0 && s->mg) *value = s->mg->level_eig_min(std::stoi(k.substr(8)));
     else if (k.rfind("level_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(10))).nnz();
