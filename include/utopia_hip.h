@@ -369,15 +369,27 @@ typedef struct uhip_solver_s *uhip_solver;
 #define UHIP_SOLVER_PCG 7         /* ProjectedConjugateGradient */
 #define UHIP_SOLVER_PG 8          /* ProjectedGradient */
 #define UHIP_SOLVER_MMG 9         /* MonotoneMultigrid (PGS smoother, truncated finest transfer) */
+#define UHIP_SOLVER_AMG 10        /* AlgebraicMultigrid (Agglomerate transfers) as a solver */
+#define UHIP_SOLVER_CG_AMG 11     /* ConjugateGradient + AlgebraicMultigrid preconditioner, max_it 1 */
 
 /* isolver_lsolve_init */
 int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out);
+/* QPSolverRegistry::find(backend, type) (backend/polymorphic/utopia_polymorphic_QPSolver_impl.hpp:81-97,
+ * names registered at :119-125) and OmniLinearSolver's "type" (utopia_polymorphic_LinearSolver_impl.hpp:20-23,
+ * SolverType names solvers/utopia_SolverType.hpp:33-34). backend: "any" or "hip"; type: "cg" / "utopia_cg"
+ * (CG), "cg_jacobi", "cg_mg", "cg_mg_mixed", "mg", "mprgp", "pgs", "pcg", "pg", "mmg", "amg" (the
+ * PETSc LinearSolverFactory's name, backend/petsc/solvers/linear/utopia_petsc_LinearSolverFactory.cpp:73),
+ * "cg_amg". An unknown backend
+ * or type is UHIP_ERR_INVALID with the reference's message ("solver X not found") in uhip_last_error();
+ * the reference falls back to its default solver instead, which this library does not carry. */
+int uhip_solver_create_by_name(uhip_ctx ctx, const char *backend, const char *type, uhip_solver *out);
 /* isolver_lsolve_destroy */
 int uhip_solver_destroy(uhip_solver s);
 /* read(Input&)-style parameters: atol rtol stol max_it verbose reset_initial_guess refine
  * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
  * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour) scale_max_eig scale_min_eig power_method_tol
- * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each. Read at the next update.
+ * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each coarse_space_it
+ * active_set_tol; AlgebraicMultigrid: n_levels bmax weight. Read at the next update.
  * autotune (reps; 0 = off) applies to the SpMV plans of matrices built after it is set. */
 int uhip_solver_set_param(uhip_solver s, const char *key, double value);
 /* Multigrid::set_transfer_operators (utopia_LinearMultiLevel.hpp:65-79), host CSR, coarse to fine */
@@ -403,6 +415,10 @@ int uhip_solver_get_status(uhip_solver s, int64_t *iterates, double *gradient_no
 int uhip_solver_get_info(uhip_solver s, const char *key, double *value);
 int uhip_solver_get_level(uhip_solver s, int l, int32_t *rowptr, int32_t *col, double *val);
 int uhip_solver_get_mask(uhip_solver s, int l, double *mask);
+/* interpolation l (coarse to fine; sizes from get_info transfer_rows_<l> transfer_cols_<l>
+ * transfer_nnz_<l>): the given ones, or those AlgebraicMultigrid built (Agglomerate::create_transfer,
+ * solvers/multilevel/agglomerate/utopia_Agglomerate.hpp:37-39) */
+int uhip_solver_get_transfer(uhip_solver s, int l, int32_t *rowptr, int32_t *col, double *val);
 int uhip_solver_set_level_eigs(uhip_solver s, int l, double eig_max, double eig_min);
 /* colour of every row of level l (GaussSeidel smoother) and the number of colours */
 int uhip_solver_get_colors(uhip_solver s, int l, int32_t *color, int32_t *ncolors);

@@ -943,6 +943,66 @@ done:
 
 /* ------------------------------------------------------ ProjectedGaussSeidel */
 
+/* ------------------------------------------------------- Agglomerate (AMG) */
+
+/* Agglomerate::create_transfer_matrix (solvers/multilevel/agglomerate/utopia_Agglomerate.hpp:50-178),
+ * use_additive_correction off: rows with one stored value get no aggregate (-2); a row not yet
+ * aggregated opens aggregate k and claims up to bmax rows j with weight*max_{j!=i}|a_ij| < |a_ij|
+ * that are not yet aggregated, in storage order; claiming none, it takes the aggregate of the
+ * largest |a_ij| of its row (diagonal included) instead; rows still unassigned become singletons.
+ * P(i, parent_i) = 1 where max_{j!=i}|a_ij| != 0, else 0. Returns the number of aggregates. */
+int32_t or_agglomerate(const or_csr *A, int64_t bmax, double weight, or_csr *P) {
+    const int32_t n = A->nrows;
+    int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    double *amax = vnew(n);
+    for (int32_t i = 0; i < n; ++i) parent[i] = (A->rowptr[i + 1] - A->rowptr[i] == 1) ? -2 : -1;
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e)
+            if (A->col[e] != i && fabs(A->val[e]) > amax[i]) amax[i] = fabs(A->val[e]);
+    int32_t k = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (parent[i] != -1) continue;
+        parent[i] = k;
+        int64_t claimed = 0;
+        for (int32_t e = A->rowptr[i]; e < A->rowptr[i + 1] && claimed < bmax; ++e) {
+            const int32_t j = A->col[e];
+            if (parent[j] == -1 && fabs(A->val[e]) > weight * amax[i]) {
+                parent[j] = k;
+                ++claimed;
+            }
+        }
+        if (claimed) {
+            ++k;
+        } else {
+            int32_t jmax = -1;
+            double vmax = 0;
+            for (int32_t e = A->rowptr[i]; e < A->rowptr[i + 1]; ++e)
+                if (fabs(A->val[e]) > vmax) vmax = fabs(A->val[e]), jmax = A->col[e];
+            parent[i] = (jmax >= 0 && jmax != i) ? parent[jmax] : -1;
+        }
+    }
+    for (int32_t i = 0; i < n; ++i)
+        if (parent[i] == -1) parent[i] = k++;
+    P->nrows = n;
+    P->ncols = k;
+    P->rowptr = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    P->col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    P->val = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    int64_t z = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (parent[i] >= 0) {
+            P->col[z] = parent[i];
+            P->val[z] = amax[i] != 0 ? 1.0 : 0.0;
+            ++z;
+        }
+        P->rowptr[i + 1] = (int32_t)z;
+    }
+    P->nnz = z;
+    free(parent);
+    free(amax);
+    return k;
+}
+
 /* ---------------------------------------------------- MonotoneMultigrid */
 
 /* One projected Gauss-Seidel step in colour order (ProjectedGaussSeidel::step,

@@ -85,6 +85,8 @@ void or_chebyshev_smooth(const or_csr *A, const double *b, double *x, int nu, do
 /* MPRGP (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:57-387) */
 int or_mprgp_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, double atol,
                    double rtol, int64_t max_it, double eps_eig_est, int64_t power_method_max_it, or_status *st);
+/* Agglomerate::create_transfer_matrix: piecewise-constant aggregation P (n x returned count) */
+int32_t or_agglomerate(const or_csr *A, int64_t bmax, double weight, or_csr *P);
 /* MonotoneMultigrid with multicolour projected Gauss-Seidel (JP colouring, seed 0x5eed),
  * GaussSeidel-smoothed coarse Multigrid; interp: nlevels-1 interpolations, coarse to fine */
 int or_mmg_solve(const or_csr *A, const double *b, const double *lb, const double *ub, double *x, int nlevels,

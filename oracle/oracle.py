@@ -69,6 +69,7 @@ def lib():
             "or_mprgp_solve": (I, [P, P, P, P, P, D, D, I64, D, I64, P]),
             "or_pgs_solve": (I, [P, P, P, P, P, D, I64, I, I64, P]),
             "or_pcg_solve": (I, [P, P, P, P, P, D, D, D, I64, P]),
+            "or_agglomerate": (ctypes.c_int32, [P, I64, D, P]),
             "or_mmg_solve": (I, [P, P, P, P, P, I, P, I, I, I64, D, D, D, D, I64, P]),
             "or_pg_solve": (I, [P, P, P, P, P, D, D, D, I64, P]),
             "or_set_threads": (None, [I]),
@@ -104,6 +105,10 @@ class Mat:
     @property
     def ref(self):
         return ctypes.byref(self.s)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=self.shape)
 
 
 def _from_owned(s: or_csr):
@@ -160,6 +165,25 @@ def ptap(A, P):
     MA, MP = Mat.of(A), Mat.of(P)
     lib().or_ptap(MA.ref, MP.ref, ctypes.byref(out))
     return _from_owned(out)
+
+
+def agglomerate(A, bmax=3, weight=1.0 / 3):
+    """Agglomerate transfer (interpolation) of A as (rowptr, col, val, shape)."""
+    out = or_csr()
+    lib().or_agglomerate(Mat.of(A).ref, bmax, weight, ctypes.byref(out))
+    return _from_owned(out)
+
+
+def amg_transfers(A, n_levels, bmax=3, weight=1.0 / 3):
+    """AlgebraicMultigridBuilder::build (solvers/multilevel/utopia_AlgebraicMultigrid.hpp:23-52):
+    P_l = agglomerate(A_{l+1}), A_l = P_l^T A_{l+1} P_l from the finest level down.
+    Returns the interpolations and the operators, both coarse to fine, as Mat."""
+    Ps, As = [], [Mat.of(A)]
+    for _ in range(n_levels - 1):
+        P = Mat(*agglomerate(As[0], bmax, weight))
+        As.insert(0, Mat(*ptap(As[0], P)))
+        Ps.insert(0, P)
+    return Ps, As
 
 
 def cholesky(Adense):

@@ -373,6 +373,28 @@ def test_error_paths_report_status(ctx):
     assert cg.status["iterates"] == 0
 
 
+def test_solver_by_registry_name(ctx):
+    """QPSolverRegistry / OmniLinearSolver names select the same solvers as the type codes."""
+    A, b, lb = _qp(32, 5)
+    x1 = ua.MPRGP(ctx, atol=1e-9, rtol=0, max_it=10000)
+    x1.update(A)
+    x1.set_box_constraints(lb, None)
+    ua.srand(1)
+    y1 = x1.apply(b)
+    q = ua.Solver.by_name(ctx, "mprgp", atol=1e-9, rtol=0, max_it=10000)
+    q.update(A)
+    q.set_box_constraints(lb, None)
+    ua.srand(1)
+    y2 = q.apply(b)
+    assert np.array_equal(y1, y2)
+    cg = ua.Solver.by_name(ctx, "cg_jacobi", backend="hip", atol=1e-10, rtol=0, max_it=500)
+    xc = cg.solve(A, b)
+    assert np.linalg.norm(b - A.to_scipy() @ xc) <= 1e-10 * 1.01
+    for backend, name in (("any", "ssnewton_x"), ("petsc", "cg")):
+        with pytest.raises(ua.HipError, match="not found"):
+            ua.Solver.by_name(ctx, name, backend=backend)
+
+
 def _monotone_mg_problem(n_levels=6, n_coarse=50):
     from test_oracle_pins import _monotone_mg_problem as mk
     return mk(n_levels, n_coarse)
@@ -421,3 +443,84 @@ def test_monotone_multigrid_2d_obstacle_matches_oracle(ctx):
     assert st["reason"] > 0 and abs(st["iterates"] - so["iterates"]) <= 1
     assert rel(x, xo) <= 1e-8
     _kkt(A, b, x, lb, 1e-8)
+
+
+def _amg_reference_problem(n=100):
+    from test_oracle_pins import _amg_reference_problem as mk
+    return mk(n)
+
+
+def _same_csr(a, rp, col, val):
+    return np.array_equal(a.rowptr, rp) and np.array_equal(a.col, col) and np.array_equal(a.val, val)
+
+
+@pytest.mark.parametrize("case", ["ref1d", "varcoef32"])
+def test_amg_hierarchy_matches_oracle_exactly(ctx, case):
+    """Agglomerate transfers and the P^T A P operators of every level, bit for bit."""
+    if case == "ref1d":
+        A, _ = _amg_reference_problem(100)
+        L = 6
+    else:
+        A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 32, 32, 32, seed=3)
+        L = 5
+    s = ua.AlgebraicMultigrid(ctx, n_levels=L, smoother=2)
+    s.update(A)
+    Ps, As = orc.amg_transfers(A, L)
+    assert int(s.info("n_levels")) == L
+    for l in range(L - 1):
+        P = s.transfer(l)
+        assert P.shape == Ps[l].shape and _same_csr(P, Ps[l].rowptr, Ps[l].col, Ps[l].val)
+    for l in range(L):
+        G = s.level(l)
+        assert _same_csr(G, As[l].rowptr, As[l].col, As[l].val)
+
+
+def _oracle_amg(A, L, pre, post, colors):
+    Ps, As = orc.amg_transfers(A, L)
+    mg = orc.MG(Ps, smoother=2, pre=pre, post=post, masks=False)
+    mg.update(A)
+    for l in range(1, L):
+        mg.set_colors(l, *colors(l))
+    return mg
+
+
+def test_cg_amg_reference_problem_matches_oracle(ctx):
+    """AMGTest::test_1D's problem (|b - A x|_1 < 1e-5) with CG over a 6-level
+    AlgebraicMultigrid (GaussSeidel 3/3)."""
+    A, b = _amg_reference_problem(100)
+    cg = ua.ConjugateGradient(ctx, "amg", n_levels=6, smoother=2, pre_smoothing_steps=3, post_smoothing_steps=3,
+                              atol=1e-9, rtol=0, stol=0, max_it=200)
+    x = cg.solve(A, b)
+    mg = _oracle_amg(A, 6, 3, 3, cg.colors)
+    xo, so = mg.cg_solve(b, atol=1e-9, max_it=200)
+    print(f"cg-amg 1d: gpu it={cg.status['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-10
+    assert np.abs(b - A.to_scipy() @ x).sum() < 1e-5
+
+
+def test_amg_solver_cycles_match_oracle(ctx):
+    A, b = _amg_reference_problem(100)
+    s = ua.AlgebraicMultigrid(ctx, n_levels=6, smoother=2, pre_smoothing_steps=3, post_smoothing_steps=3, max_it=20,
+                              atol=0, rtol=0, stol=0)
+    x = s.solve(A, b)
+    mg = _oracle_amg(A, 6, 3, 3, s.colors)
+    xo = mg.apply(b, max_it=20, atol=0, rtol=0)
+    assert s.status["iterates"] == 20
+    assert rel(x, xo) <= 1e-12
+
+
+def test_cg_amg_varcoef_matches_oracle(ctx):
+    n = 32
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=3)
+    b = ua.gen_uniform(5, n ** 3)
+    atol = 1e-10 * np.linalg.norm(b)
+    cg = ua.ConjugateGradient(ctx, "amg", n_levels=5, smoother=2, pre_smoothing_steps=2, post_smoothing_steps=2,
+                              atol=atol, rtol=0, stol=0, max_it=500)
+    x = cg.solve(A, b)
+    mg = _oracle_amg(A, 5, 2, 2, cg.colors)
+    xo, so = mg.cg_solve(b, atol=atol, max_it=500)
+    print(f"cg-amg 32^3: gpu it={cg.status['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-10
+    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol

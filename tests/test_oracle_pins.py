@@ -584,3 +584,69 @@ def test_monotone_mg_coarse_space_multigrid_tends_to_exact_coarse_solve():
     x5_1, _ = orc.mmg_solve(A, b, Is, coarse_space_it=1, **kw)
     assert np.abs(x5 - x2).max() <= 1e-8 * np.abs(x2).max()
     assert np.abs(x5_1 - x2).max() > 1e-6 * np.abs(x2).max()
+
+
+def _amg_reference_problem(n=100):
+    """AMGTest::test_1D (tests/utopia_AMGTest.cpp:69-121): assemble_laplacian_1D with
+    bc = true (identity rows 0 and n-1 holding one value,
+    tests/test_problems/utopia_assemble_laplacian_1D.hpp:11-44), b = 2 with zero ends."""
+    rp, col, val = [0], [], []
+    for i in range(n):
+        if i in (0, n - 1):
+            col.append(i)
+            val.append(1.0)
+        else:
+            col += [i - 1, i, i + 1]
+            val += [-1.0, 2.0, -1.0]
+        rp.append(len(col))
+    A = ua.Csr(np.array(rp, np.int32), np.array(col, np.int32), np.array(val), (n, n))
+    b = np.full(n, 2.0)
+    b[0] = b[-1] = 0.0
+    return A, b
+
+
+def test_agglomerate_pairs_the_1d_laplacian_and_galerkin_is_the_laplacian_again():
+    """On the 1D Laplacian each open row claims its right neighbour (its left one is taken),
+    the single-value boundary rows join no aggregate, and P^T A P of the pairs is again
+    tridiag(-1, 2, -1)."""
+    A, _ = _amg_reference_problem(100)
+    rp, col, val, shape = orc.agglomerate(A)
+    assert shape == (100, 49)
+    assert rp[1] == 0 and rp[100] - rp[99] == 0  # rows 0 and 99: no entry
+    assert np.array_equal(col, (np.arange(1, 99) - 1) // 2) and np.all(val == 1.0)
+    Ps, As = orc.amg_transfers(A, 2)
+    Ac = As[0].to_scipy().toarray()
+    m = 49
+    assert np.array_equal(Ac, 2 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1))
+
+
+def test_agglomerate_hand_computed_five_point_3x3():
+    """5-point Laplacian on a 3x3 grid (all couplings strong): row 0 claims 1 and 3,
+    row 2 claims 5, row 4 claims 7; rows 6 and 8 find every neighbour taken, their largest
+    entry is their diagonal, so they stay open and end as singletons."""
+    A = ua.gen_stencil(ua.STENCIL_2D5, 3, 3)
+    assert np.array_equal(A.to_scipy().diagonal(), np.full(9, 4.0))
+    rp, col, val, shape = orc.agglomerate(A)
+    assert shape == (9, 5)
+    assert np.array_equal(col, [0, 0, 1, 0, 2, 1, 3, 2, 4]) and np.array_equal(np.diff(rp), np.ones(9))
+    # bmax caps the claims: with bmax = 1 row 0 takes only 1, row 2 takes 5, row 3 takes 4,
+    # row 6 takes 7 and row 8 is left over
+    rp1, col1, _, shape1 = orc.agglomerate(A, bmax=1)
+    assert np.array_equal(col1, [0, 0, 1, 2, 2, 1, 3, 3, 4]) and shape1 == (9, 5)
+    # weight: with weight 1 nothing is strong (|a_ij| = max|a_ij|), every row a singleton
+    rpw, colw, _, shapew = orc.agglomerate(A, weight=1.0)
+    assert shapew == (9, 9) and np.array_equal(colw, np.arange(9))
+
+
+def test_cg_amg_solves_reference_amg_problem():
+    """AMGTest asserts |b - A x|_1 < 1e-5 after AlgebraicMultigrid (6 levels) on this
+    problem; here with GaussSeidel smoothing inside CG."""
+    A, b = _amg_reference_problem(100)
+    Ps, As = orc.amg_transfers(A, 6)
+    assert [P.shape for P in Ps] == [(7, 4), (13, 7), (25, 13), (49, 25), (100, 49)]
+    mg = orc.MG(Ps, smoother=2, pre=3, post=3, masks=False)
+    mg.update(A)
+    for l in range(1, 6):
+        mg.set_colors(l, *orc.jp_color(As[l]))
+    x, st = mg.cg_solve(b, atol=1e-9, max_it=200)
+    assert st["converged"] and np.abs(b - A.to_scipy() @ x).sum() < 1e-5

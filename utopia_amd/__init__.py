@@ -37,7 +37,7 @@ _NP = {F64: np.float64, F32: np.float32}
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
 (SOLVER_CG, SOLVER_CG_JACOBI, SOLVER_CG_MG, SOLVER_CG_MG_MIXED, SOLVER_MG, SOLVER_MPRGP, SOLVER_PGS,
- SOLVER_PCG, SOLVER_PG, SOLVER_MMG) = range(10)
+ SOLVER_PCG, SOLVER_PG, SOLVER_MMG, SOLVER_AMG, SOLVER_CG_AMG) = range(12)
 
 # ConvergenceReason (solvers/utopia_ConvergenceReason.hpp:84-100)
 CONVERGED_FNORM_ABS, CONVERGED_FNORM_RELATIVE, CONVERGED_SNORM_RELATIVE, DIVERGED_MAX_IT = 2, 3, 4, -5
@@ -307,9 +307,24 @@ class Solver:
         for k, v in params.items():
             self.set_param(k, v)
 
+    @classmethod
+    def by_name(cls, ctx: Context, type_name: str, backend: str = "any", **params) -> "Solver":
+        """A solver by its registry name, as QPSolverRegistry::find(backend, type) and
+        OmniLinearSolver's "type" select one ("cg", "mprgp", "pgs", "pcg", "pg", "mmg", ...)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        check(lib().uhip_solver_create_by_name(ctx.h, backend.encode(), type_name.encode(), ctypes.byref(h)),
+              f"uhip_solver_create_by_name {backend}/{type_name}")
+        self.h = h
+        self.n = None
+        for k, v in params.items():
+            self.set_param(k, v)
+        return self
+
     def __del__(self):
         try:
-            if _alive() and self.h:
+            if _alive() and getattr(self, "h", None):
                 lib().uhip_solver_destroy(self.h)
                 self.h = None
         except Exception:
@@ -401,6 +416,17 @@ class Solver:
         check(lib().uhip_solver_get_level(self.h, l, _ptr(rp), _ptr(col), _ptr(val)), "get_level")
         return Csr(rp, col[:nnz], val[:nnz], (n, n))
 
+    def transfer(self, l: int) -> Csr:
+        """Interpolation l (coarse to fine) of a multigrid solver."""
+        n = int(self.info(f"transfer_rows_{l}"))
+        m = int(self.info(f"transfer_cols_{l}"))
+        nnz = int(self.info(f"transfer_nnz_{l}"))
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1), np.float64)
+        check(lib().uhip_solver_get_transfer(self.h, l, _ptr(rp), _ptr(col), _ptr(val)), "get_transfer")
+        return Csr(rp, col[:nnz], val[:nnz], (n, m))
+
     def mask(self, l: int) -> np.ndarray:
         n = int(self.info(f"level_rows_{l}"))
         out = np.zeros(n, np.float64)
@@ -423,11 +449,12 @@ class ConjugateGradient(Solver):
     """ConjugateGradient<HipCrsMatrix, HipVector, HOMEMADE>.
 
     preconditioner: None, "jacobi" (InvDiagPreconditioner), "multigrid" (FP64
-    V-cycle) or "multigrid_mixed" (FP32 V-cycle inside the FP64 CG).
+    V-cycle), "multigrid_mixed" (FP32 V-cycle inside the FP64 CG) or "amg"
+    (AlgebraicMultigrid, one cycle).
     """
 
     _TYPES = {None: SOLVER_CG, "jacobi": SOLVER_CG_JACOBI, "multigrid": SOLVER_CG_MG,
-              "multigrid_mixed": SOLVER_CG_MG_MIXED}
+              "multigrid_mixed": SOLVER_CG_MG_MIXED, "amg": SOLVER_CG_AMG}
 
     def __init__(self, ctx: Context, preconditioner: Optional[str] = None, **params):
         super().__init__(ctx, self._TYPES[preconditioner], **params)
@@ -451,6 +478,12 @@ class ProjectedConjugateGradient(Solver):
 
 class ProjectedGradient(Solver):
     TYPE = SOLVER_PG
+
+
+class AlgebraicMultigrid(Solver):
+    """AlgebraicMultigrid: Agglomerate transfers built from the operator at update
+    (params n_levels, bmax, weight, and the Multigrid smoother/cycle parameters)."""
+    TYPE = SOLVER_AMG
 
 
 class MonotoneMultigrid(Solver):

@@ -402,6 +402,11 @@ public:
     void must_generate_masks(bool v) { masks_ = v; }
     bool must_generate_masks() const { return masks_; }
     void fix_semidefinite_operators(bool v) { fix_semi_ = v; }
+    // LinearMultiLevel::set_linear_operators (utopia_LinearMultiLevel.hpp:138-150), used with
+    // set_perform_galerkin_assembly(false) (utopia_Multigrid.hpp:129-153): the level operators
+    // are taken as given (coarse to fine); no masks are generated and no semidefinite fix applied.
+    void set_linear_operators(const std::vector<std::shared_ptr<const HipCrsMatrix>> &A) { given_ = A; }
+    void set_perform_galerkin_assembly(bool v) { galerkin_ = v; }
     int n_levels() const { return n_levels_; }
     const HipCrsMatrix &level_operator(int l) const { return *A_[(size_t)l]; }
     const HipVector &mask(int l) const { return mask_[(size_t)l]; }
@@ -426,9 +431,10 @@ public:
         auto ctx = op->context();
         A_.assign((size_t)L, nullptr);
         A_[(size_t)L - 1] = op;
+        const bool masks = masks_ && galerkin_;
         // masks (MultiLevelMask::generate_masks, before the Galerkin products)
         mask_.assign((size_t)L, HipVector());
-        if (masks_) {
+        if (masks) {
             HipVector &mf = mask_[(size_t)L - 1];
             mf = HipVector(ctx, op->rows());
             hip_check(uhip_csr_offdiag_indicator(op->c(), &op->raw(), std::numeric_limits<double>::epsilon() * 1e6,
@@ -450,10 +456,15 @@ public:
             }
         }
         // Galerkin assembly (LinearMultiLevel::galerkin_assembly)
-        for (int l = L - 1; l > 0; --l) {
-            auto Ac = std::make_shared<HipCrsMatrix>(ptap(*A_[(size_t)l], *I_[(size_t)l - 1]));
-            if (fix_semi_) hip_check(uhip_csr_fix_semidefinite(Ac->c(), &Ac->raw()), "fix_semidefinite");
-            A_[(size_t)l - 1] = Ac;
+        if (galerkin_) {
+            for (int l = L - 1; l > 0; --l) {
+                auto Ac = std::make_shared<HipCrsMatrix>(ptap(*A_[(size_t)l], *I_[(size_t)l - 1]));
+                if (fix_semi_) hip_check(uhip_csr_fix_semidefinite(Ac->c(), &Ac->raw()), "fix_semidefinite");
+                A_[(size_t)l - 1] = Ac;
+            }
+        } else {
+            if ((int)given_.size() != L) throw HipError(UHIP_ERR_INVALID, "Multigrid: set_linear_operators first");
+            for (int l = 0; l < L - 1; ++l) A_[(size_t)l] = given_[(size_t)l];
         }
         // smoothers, levels 1..L-1 in this order (rand() draws of the power method)
         auto cheb = std::dynamic_pointer_cast<Chebyshev3level<HipCrsMatrix, HipVector>>(smoother_);
@@ -466,7 +477,7 @@ public:
         eng_.pre = pre_;
         eng_.post = post_;
         eng_.mg_type = mg_type_;
-        eng_.masks = masks_;
+        eng_.masks = masks;
         eng_.smoother = cheb ? MG_CHEBYSHEV : (gs ? MG_GAUSS_SEIDEL : MG_JACOBI);
         eng_.gs_symmetric = gs ? gs->symmetric() : true;
         for (int l = 0; l < L; ++l) {
@@ -476,7 +487,7 @@ public:
                 lv.R = R_[(size_t)l - 1];
                 lv.P = I_[(size_t)l - 1];
             }
-            if (masks_) lv.mask = mask_[(size_t)l];
+            if (masks) lv.mask = mask_[(size_t)l];
         }
         for (int l = 1; l < L; ++l) {
             auto &lv = eng_.levels[(size_t)l];
@@ -553,10 +564,10 @@ private:
     std::shared_ptr<Smoother> smoother_;
     std::shared_ptr<Factorization> coarse_;
     std::vector<std::shared_ptr<HipCrsMatrix>> I_, R_;
-    std::vector<std::shared_ptr<const HipCrsMatrix>> A_;
+    std::vector<std::shared_ptr<const HipCrsMatrix>> A_, given_;
     std::vector<HipVector> mask_;
     int n_levels_ = 0, pre_ = 3, post_ = 3, mg_type_ = 1;
-    bool masks_ = true, fix_semi_ = false;
+    bool masks_ = true, fix_semi_ = false, galerkin_ = true;
     VCycle<double> eng_;
     HipVector r_top_, c_top_;
     double setup_s_ = 0;

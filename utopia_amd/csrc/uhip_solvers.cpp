@@ -7,6 +7,7 @@
 #include <map>
 #include <string>
 
+#include "host/utopia_hip_amg.hpp"
 #include "host/utopia_hip_mmg.hpp"
 #include "host/utopia_hip_multigrid.hpp"
 #include "host/utopia_hip_qp.hpp"
@@ -27,12 +28,17 @@ struct uhip_solver_s {
     std::shared_ptr<ProjectedGaussSeidel> pgs;
     std::shared_ptr<QPSolverBase> qp;  // the QP solver of the MPRGP / PGS / PCG / PG / MMG types
     std::shared_ptr<MonotoneMultigrid> mmg;
+    std::shared_ptr<AlgebraicMultigrid> amg;
     HipVector lb, ub;
     bool has_lb = false, has_ub = false;
     SolutionStatus status;
     double seconds = 0, setup_seconds = 0;
     int64_t total_iterations = 0, refinements = 0;
 };
+
+namespace uhip {
+int set_error(int code, const char *fmt, ...);  // uhip_runtime.hip
+}
 
 namespace {
 
@@ -45,14 +51,16 @@ double param(const uhip_solver_s *s, const char *k, double def) {
     return it == s->params.end() ? def : it->second;
 }
 
+// uhip_last_error() reports the exception's message (for kernel-layer failures it
+// already embeds that layer's message)
 int fail(const std::exception &e) {
     const auto *he = dynamic_cast<const HipError *>(&e);
-    // uhip_last_error() already holds the message for C-ABI failures
-    return he ? he->code() : UHIP_ERR_INVALID;
+    return uhip::set_error(he ? he->code() : UHIP_ERR_INVALID, "%s", e.what());
 }
 
 template <class S>
-void apply_iterative_params(uhip_solver_s *s, S &solver) {
+void apply_iterative_params(uhip_solver_s *s, S &solver_) {
+    IterativeSolver<HipCrsMatrix, HipVector> &solver = solver_;
     solver.atol(param(s, "atol", solver.atol()));
     solver.rtol(param(s, "rtol", solver.rtol()));
     solver.stol(param(s, "stol", solver.stol()));
@@ -60,7 +68,7 @@ void apply_iterative_params(uhip_solver_s *s, S &solver) {
     solver.verbose(param(s, "verbose", 0) != 0);
 }
 
-std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s) {
+std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s, bool with_transfers = true) {
     std::shared_ptr<IterativeSolver<HipCrsMatrix, HipVector>> sm;
     if ((int)param(s, "smoother", 0) == 1) {
         sm = std::make_shared<Jacobi<HipCrsMatrix, HipVector>>();
@@ -82,8 +90,19 @@ std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s) {
     mg->mg_type((int)param(s, "mg_type", 1));
     mg->must_generate_masks(param(s, "must_generate_masks", 1) != 0);
     mg->fix_semidefinite_operators(param(s, "fix_semidefinite_operators", 0) != 0);
-    mg->set_transfer_operators(s->transfers);
+    if (with_transfers) mg->set_transfer_operators(s->transfers);
     return mg;
+}
+
+// AlgebraicMultigrid (n_levels, Agglomerate bmax / weight) over a Multigrid with the
+// smoother and cycle parameters of the MG types
+std::shared_ptr<AlgebraicMultigrid> make_amg(uhip_solver_s *s) {
+    auto agg = std::make_shared<Agglomerate>();
+    agg->bmax((int64_t)param(s, "bmax", 3));
+    agg->weight(param(s, "weight", 1.0 / 3));
+    auto amg = std::make_shared<AlgebraicMultigrid>(make_multigrid(s, false), agg);
+    amg->n_levels((int)param(s, "n_levels", 2));
+    return amg;
 }
 
 void build(uhip_solver_s *s) {
@@ -94,6 +113,7 @@ void build(uhip_solver_s *s) {
     s->pgs.reset();
     s->qp.reset();
     s->mmg.reset();
+    s->amg.reset();
     switch (s->type) {
         case UHIP_SOLVER_CG:
         case UHIP_SOLVER_CG_JACOBI:
@@ -126,6 +146,29 @@ void build(uhip_solver_s *s) {
             apply_iterative_params(s, *mg);
             s->mg = mg;
             s->solver = mg;
+            break;
+        }
+        case UHIP_SOLVER_AMG: {
+            auto amg = make_amg(s);
+            apply_iterative_params(s, *amg);
+            apply_iterative_params(s, *amg->algorithm());  // read(in) reaches the Multigrid too
+            s->amg = amg;
+            s->mg = amg->algorithm();
+            s->solver = amg;
+            break;
+        }
+        case UHIP_SOLVER_CG_AMG: {
+            auto cg = std::make_shared<ConjugateGradient<HipCrsMatrix, HipVector>>();
+            apply_iterative_params(s, *cg);
+            cg->reset_initial_guess(param(s, "reset_initial_guess", 0) != 0);
+            auto amg = make_amg(s);
+            amg->max_it(1);
+            cg->set_preconditioner(amg);
+            cg->refine(param(s, "refine", 0) != 0);
+            s->amg = amg;
+            s->mg = amg->algorithm();
+            s->cg = cg;
+            s->solver = cg;
             break;
         }
         case UHIP_SOLVER_MPRGP: {
@@ -204,7 +247,7 @@ extern "C" {
 
 int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out) {
     if (!ctx || !out) return UHIP_ERR_INVALID;
-    if (type < UHIP_SOLVER_CG || type > UHIP_SOLVER_MMG) return UHIP_ERR_INVALID;
+    if (type < UHIP_SOLVER_CG || type > UHIP_SOLVER_CG_AMG) return UHIP_ERR_INVALID;
     try {
         auto *s = new uhip_solver_s();
         s->type = type;
@@ -214,6 +257,21 @@ int uhip_solver_create(uhip_ctx ctx, int type, uhip_solver *out) {
     } catch (const std::exception &e) {
         return fail(e);
     }
+}
+
+int uhip_solver_create_by_name(uhip_ctx ctx, const char *backend, const char *type, uhip_solver *out) {
+    if (!ctx || !out || !backend || !type) return UHIP_ERR_INVALID;
+    static const std::map<std::string, int> registry = {
+        {"cg", UHIP_SOLVER_CG},          {"utopia_cg", UHIP_SOLVER_CG}, {"cg_jacobi", UHIP_SOLVER_CG_JACOBI},
+        {"cg_mg", UHIP_SOLVER_CG_MG},    {"cg_mg_mixed", UHIP_SOLVER_CG_MG_MIXED},
+        {"mg", UHIP_SOLVER_MG},          {"mprgp", UHIP_SOLVER_MPRGP},  {"pgs", UHIP_SOLVER_PGS},
+        {"pcg", UHIP_SOLVER_PCG},        {"pg", UHIP_SOLVER_PG},        {"mmg", UHIP_SOLVER_MMG},
+        {"amg", UHIP_SOLVER_AMG},        {"cg_amg", UHIP_SOLVER_CG_AMG}};
+    const std::string b(backend), t(type);
+    if (b != "any" && b != "hip") return uhip::set_error(UHIP_ERR_INVALID, "[Error] backend %s not found", backend);
+    auto it = registry.find(t);
+    if (it == registry.end()) return uhip::set_error(UHIP_ERR_INVALID, "[Error] solver %s not found", type);
+    return uhip_solver_create(ctx, it->second, out);
 }
 
 int uhip_solver_destroy(uhip_solver s) {
@@ -229,6 +287,10 @@ int uhip_solver_set_param(uhip_solver s, const char *key, double value) {
     // tolerances / iteration limits also act on an already built solver
     if (s->cg) apply_iterative_params(s, *s->cg);
     else if (s->mg && s->type == UHIP_SOLVER_MG) apply_iterative_params(s, *s->mg);
+    else if (s->amg) {
+        apply_iterative_params(s, *s->amg);
+        apply_iterative_params(s, *s->amg->algorithm());
+    }
     else if (s->qp) apply_iterative_params(s, *s->qp);
     return UHIP_OK;
 }
@@ -370,6 +432,9 @@ int uhip_solver_get_info(uhip_solver s, const char *key, double *value) {
     else if (k == "ncolors" && s->pgs) *value = s->pgs->ncolors();
     else if (k == "alpha_bar" && s->mprgp) *value = s->mprgp->alpha_bar();
     else if (k == "active_set_changes" && s->mmg) *value = (double)s->mmg->active_set_changes();
+    else if (k.rfind("transfer_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->interpolations().at((size_t)std::stoi(k.substr(13)))->nnz();
+    else if (k.rfind("transfer_rows_", 0) == 0 && s->mg) *value = (double)s->mg->interpolations().at((size_t)std::stoi(k.substr(14)))->rows();
+    else if (k.rfind("transfer_cols_", 0) == 0 && s->mg) *value = (double)s->mg->interpolations().at((size_t)std::stoi(k.substr(14)))->cols();
     else if (k.rfind("eig_max_", 0) == 0 && s->mg) *value = s->mg->level_eig_max(std::stoi(k.substr(8)));
     else if (k.rfind("eig_min_", 0) == 0 && s->mg) *value = s->mg->level_eig_min(std::stoi(k.substr(8)));
     else if (k.rfind("level_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(10))).nnz();
@@ -384,6 +449,20 @@ int uhip_solver_get_level(uhip_solver s, int l, int32_t *rowptr, int32_t *col, d
     if (!s || !s->mg || l < 0 || l >= s->mg->n_levels()) return UHIP_ERR_INVALID;
     try {
         const auto &m = s->mg->level_operator(l).raw();
+        uhip_ctx c = s->ctx->get();
+        hip_check(uhip_memcpy_dtoh(c, rowptr, m.rowptr, sizeof(int32_t) * ((size_t)m.nrows + 1)), "rowptr");
+        hip_check(uhip_memcpy_dtoh(c, col, m.col, sizeof(int32_t) * (size_t)m.nnz), "col");
+        hip_check(uhip_memcpy_dtoh(c, val, m.val, sizeof(double) * (size_t)m.nnz), "val");
+        return UHIP_OK;
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int uhip_solver_get_transfer(uhip_solver s, int l, int32_t *rowptr, int32_t *col, double *val) {
+    if (!s || !s->mg || l < 0 || l >= (int)s->mg->interpolations().size()) return UHIP_ERR_INVALID;
+    try {
+        const auto &m = s->mg->interpolations()[(size_t)l]->raw();
         uhip_ctx c = s->ctx->get();
         hip_check(uhip_memcpy_dtoh(c, rowptr, m.rowptr, sizeof(int32_t) * ((size_t)m.nrows + 1)), "rowptr");
         hip_check(uhip_memcpy_dtoh(c, col, m.col, sizeof(int32_t) * (size_t)m.nnz), "col");
