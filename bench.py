@@ -38,11 +38,13 @@ def spmv_bytes(n, nnz, val_bytes):
     return (val_bytes + 4) * nnz + (4 + 2 * val_bytes) * n + 4
 
 
-def dia_bytes(n, ndiag, val_bytes):
-    # DIA: ndiag value slots per row (absent entries stored as zeros), the per-row
-    # mask (1/2/4 B for <= 8/16/32 offsets), x read once, y written; offsets table
+def dia_bytes(n, ndiag, val_bytes, sym=0):
+    # DIA: ndiag value slots per row (absent entries stored as zeros; symmetric
+    # storage keeps the ndiag // 2 + 1 offsets >= 0), the per-row mask (1/2/4 B for
+    # <= 8/16/32 offsets), x read once, y written; offsets table
     mb = 1 if ndiag <= 8 else (2 if ndiag <= 16 else 4)
-    return val_bytes * ndiag * n + mb * n + 2 * val_bytes * n + 4 * ndiag
+    stored = ndiag // 2 + 1 if sym else ndiag
+    return val_bytes * stored * n + mb * n + 2 * val_bytes * n + 4 * ndiag
 
 
 def time_kernel(ua, ctx, launch, reps):
@@ -76,8 +78,10 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
         csr_b = spmv_bytes(n_rows, A.nnz, vb)
         ty = "double" if dt is np.float64 else "float"
         if D.format == ua.FMT_DIA:
-            nbytes = dia_bytes(n_rows, D.m.ndiag, vb)
-            label = f"k_spmv_dia<{ty}, EpiStore<{ty}>> (DIA, {D.m.ndiag} offsets, per-row mask)"
+            nbytes = dia_bytes(n_rows, D.m.ndiag, vb, D.m.dia_sym)
+            stored = f"{D.m.ndiag // 2 + 1} of {D.m.ndiag} offsets stored (symmetric)" if D.m.dia_sym else \
+                f"{D.m.ndiag} offsets"
+            label = f"k_spmv_dia<{ty}, EpiStore<{ty}>> (DIA, {stored}, per-row mask)"
         elif D.format == ua.FMT_SELL64:
             nbytes, label = csr_b, f"k_spmv_sell<{ty}, EpiStore<{ty}>> (SELL-64)"
         else:
@@ -292,6 +296,8 @@ def main(argv=None):
     ap.add_argument("--levels", type=int, default=5)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune", type=int, default=5,
+                    help="reps > 0: every SpMV plan of the hierarchy is chosen by timing (uhip_csr_autotune)")
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV / streaming kernel timings (for counter collection runs)")
@@ -332,7 +338,7 @@ def main(argv=None):
     for form, kind in (("fp64", "multigrid"), ("mixed", "multigrid_mixed")):
         ua.srand(1)
         cg = ua.ConjugateGradient(ctx, kind, atol=atol, rtol=0, stol=0, max_it=1000, pre_smoothing_steps=2,
-                                  post_smoothing_steps=2)
+                                  post_smoothing_steps=2, autotune=a.autotune)
         cg.set_transfer_operators(interps)
         t0 = time.perf_counter()
         cg.update(A)

@@ -175,6 +175,8 @@ typedef struct uhip_csr {
     void *dia_val;      /* DIA: ndiag x ld values (ld = nrows rounded up to 64), diagonal-major, zero where absent */
     void *dia_mask;     /* DIA: per-row bit d set iff the row stores offset d (uint8 / uint16 / uint32
                          * for ndiag <= 8 / 16 / 32) */
+    int32_t dia_sym;    /* DIA of a bitwise-symmetric matrix: dia_val holds only the offsets >= 0
+                         * (ndiag/2 + 1 arrays); offset -o of row r is read as offset +o of row r - o */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)
@@ -190,7 +192,9 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
  * of 64 rows stored column-major (one wavefront per slice, every load
  * coalesced). UHIP_FMT_DIA stores a square matrix with sorted rows and at most
  * 32 distinct offsets col - row as one value array per offset plus a per-row
- * bitmask (no column indices; fails with UHIP_ERR_UNSUPPORTED otherwise).
+ * bitmask (no column indices; fails with UHIP_ERR_UNSUPPORTED otherwise); when
+ * the pattern and the values are symmetric bit for bit, only the offsets >= 0 are
+ * stored (dia_sym = 1).
  * UHIP_FMT_AUTO takes DIA when it applies with at most 25% padding, else
  * SELL-64 when its padding adds at most 25% to nnz, else the row-block CSR
  * kernel. Each y_i is the sum of exactly the row's CSR entries in column

@@ -118,7 +118,34 @@ DIA_MATRICES = {
     "2d9_shift": lambda: ua.gen_stencil(ua.STENCIL_2D9, 33, 31, shift=1e-3),
     "galerkin27": lambda: _galerkin(18),
     "edgecoef_odd": lambda: ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 45, 13, seed=3),
+    "asym_values": lambda: _perturbed(ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 9, 8, 7, seed=4), 100, 1),
+    "asym_pattern": lambda: _dropped(ua.gen_stencil(ua.STENCIL_2D5, 11, 9), 40),
 }
+
+# symmetric DIA storage (offsets >= 0 only) is taken exactly when the values are
+# symmetric bit for bit
+DIA_SYMMETRIC = {"2d5": 1, "3d7_varcoef": 1, "2d9_shift": 1, "edgecoef_odd": 1, "asym_values": 0,
+                 "asym_pattern": 0}
+
+
+def _perturbed(A, row, k):
+    """A with entry k of row `row` (an off-diagonal) scaled by 1 + 2^-20 (kept in FP32 too)."""
+    val = A.val.copy()
+    e = A.rowptr[row] + k
+    assert A.col[e] != row
+    val[e] *= 1 + 2.0 ** -20
+    return ua.Csr(A.rowptr, A.col, val, A.shape)
+
+
+def _dropped(A, row):
+    """A without the first entry of row `row` (an off-diagonal): pattern no longer symmetric."""
+    e = A.rowptr[row]
+    assert A.col[e] != row
+    keep = np.ones(A.nnz, bool)
+    keep[e] = False
+    rp = A.rowptr.copy()
+    rp[row + 1:] -= 1
+    return ua.Csr(rp, A.col[keep], A.val[keep], A.shape)
 
 
 @pytest.mark.parametrize("name", sorted(DIA_MATRICES))
@@ -128,6 +155,8 @@ def test_spmv_dia_bit_exact(ctx, name, dt):
     n = A.shape[0]
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.format == ua.FMT_DIA
+    if name in DIA_SYMMETRIC:
+        assert D.m.dia_sym == DIA_SYMMETRIC[name]
     x = ua.gen_uniform(17, n)
     if dt is np.float32:
         x = x.astype(np.float32)
@@ -153,6 +182,16 @@ def test_dia_skips_absent_entries_exactly(ctx):
     y = yd.numpy()
     ref = orc.spmv(A, x)
     assert np.isfinite(y[7]) and np.array_equal(np.isfinite(y), np.isfinite(ref))
+    assert np.array_equal(y[np.isfinite(ref)], ref[np.isfinite(ref)])
+    # the mirrored lower diagonal: row 8 (start of grid line 1) has no entry at column 7
+    assert D.m.dia_sym == 1
+    x = ua.gen_uniform(4, 64)
+    x[7] = np.inf
+    xd = ua.DeviceArray.from_numpy(ctx, x)
+    D.spmv(xd, yd)
+    y = yd.numpy()
+    ref = orc.spmv(A, x)
+    assert np.isfinite(y[8]) and np.array_equal(np.isfinite(y), np.isfinite(ref))
     assert np.array_equal(y[np.isfinite(ref)], ref[np.isfinite(ref)])
 
 

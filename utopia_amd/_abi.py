@@ -41,6 +41,7 @@ class uhip_csr(ctypes.Structure):
         ("dia_off", ctypes.c_void_p),
         ("dia_val", ctypes.c_void_p),
         ("dia_mask", ctypes.c_void_p),
+        ("dia_sym", ctypes.c_int32),
     ]
 
 

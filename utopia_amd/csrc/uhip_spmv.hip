@@ -200,6 +200,34 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int32_t n, int32_t ndiag, c
     mask[r] = (TM)m;
 }
 
+__device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_uint(a) == __float_as_uint(b); }
+
+// flag[0] = 1 unless every stored entry (r, r + o), o > 0, has a stored mirror
+// (r + o, r) with the same bits, and the pattern is symmetric.
+template <typename T, typename TM>
+__global__ __launch_bounds__(kBlock) void k_dia_symcheck(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
+                                                         const T *__restrict__ dval, const TM *__restrict__ mask,
+                                                         int32_t *__restrict__ flag) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t ld = dia_ld(n);
+    const int32_t k = ndiag >> 1;
+    const uint32_t m = (uint32_t)mask[r];
+    bool ok = true;
+    for (int d = k + 1; d < ndiag; ++d) {
+        const int32_t o = off[d], md = ndiag - 1 - d;
+        const bool up = (m >> d) & 1u;
+        const bool lo = (r + o < n) && (((uint32_t)mask[r + o] >> md) & 1u);
+        if (up != lo) ok = false;
+        if (up && lo) {
+            const T a = dval[(int64_t)d * ld + r], b = dval[(int64_t)md * ld + r + o];
+            if (!same_bits(a, b)) ok = false;
+        }
+    }
+    if (!ok) flag[0] = 1;
+}
+
 // Plan DIA arrays; UHIP_ERR_UNSUPPORTED (nothing allocated) when the matrix is
 // not square, has unsorted or duplicate columns in a row, more than 32 distinct
 // offsets, or ndiag * n > max_ratio * nnz.
@@ -247,9 +275,45 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     }
 #undef UHIP_DIA_FILL
     UHIP_CHECK_LAUNCH();
+    // symmetric offsets -o..0..+o: keep only 0..+o when the values are symmetric bit for bit
+    bool sym_offs = (nd & 1) && offs[(size_t)nd / 2] == 0;
+    for (int32_t d = 0; d < nd && sym_offs; ++d) sym_offs = offs[(size_t)d] == -offs[(size_t)(nd - 1 - d)];
+    int32_t sym = 0;
+    if (sym_offs && nd > 1) {
+        UHIP_CHECK_HIP(hipMemsetAsync(flags, 0, sizeof(int32_t), s));
+#define UHIP_DIA_SYM(T, TM)                                                                                      \
+    hipLaunchKernelGGL((k_dia_symcheck<T, TM>), g, b, 0, s, n, nd, table, (const T *)A->dia_val,              \
+                       (const TM *)A->dia_mask, flags)
+        if (A->dtype == UHIP_F64) {
+            if (mb == 1) UHIP_DIA_SYM(double, uint8_t);
+            else if (mb == 2) UHIP_DIA_SYM(double, uint16_t);
+            else UHIP_DIA_SYM(double, uint32_t);
+        } else {
+            if (mb == 1) UHIP_DIA_SYM(float, uint8_t);
+            else if (mb == 2) UHIP_DIA_SYM(float, uint16_t);
+            else UHIP_DIA_SYM(float, uint32_t);
+        }
+#undef UHIP_DIA_SYM
+        UHIP_CHECK_LAUNCH();
+        int32_t asym = 1;
+        UHIP_CHECK_HIP(hipMemcpyAsync(&asym, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        UHIP_CHECK_HIP(hipStreamSynchronize(s));
+        if (asym == 0) {
+            const size_t ld = (size_t)dia_ld(n), keep = (size_t)(nd / 2 + 1);
+            void *half = nullptr;
+            UHIP_CHECK_HIP(hipMalloc(&half, vb * keep * ld));
+            UHIP_CHECK_HIP(hipMemcpyAsync(half, (const char *)A->dia_val + vb * (size_t)(nd / 2) * ld, vb * keep * ld,
+                                          hipMemcpyDeviceToDevice, s));
+            UHIP_CHECK_HIP(hipStreamSynchronize(s));
+            UHIP_CHECK_HIP(hipFree(A->dia_val));
+            A->dia_val = half;
+            sym = 1;
+        }
+    }
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
+    A->dia_sym = sym;
     A->format = UHIP_FMT_DIA;
     A->nparts = (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
     return UHIP_OK;
@@ -481,6 +545,7 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     A->dia_val = nullptr;
     A->dia_mask = nullptr;
     A->ndiag = 0;
+    A->dia_sym = 0;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));

@@ -170,7 +170,11 @@ inline int32_t dia_rows_per_wg(int dtype) { return kDiaWG * (dtype == UHIP_F64 ?
 
 __host__ __device__ constexpr int32_t dia_ld(int32_t n) { return (n + 63) & ~63; }
 
-template <typename TAcc, class Epi, typename TA, typename TX, typename TM>
+// SYM: val holds the offsets >= 0 only (diagonals ndiag/2 .. ndiag-1); diagonal d < ndiag/2
+// (offset -o) is read from the stored +o at row r - o. Those values are read twice (the
+// mirror read o rows later hits in L2), so SYM loads them through the cache; the full
+// layout reads every value once and streams it.
+template <typename TAcc, class Epi, typename TA, typename TX, typename TM, bool SYM>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
                                                      const TA *__restrict__ val, const TM *__restrict__ mask,
                                                      const TX *__restrict__ x, Epi epi, double *__restrict__ partials,
@@ -195,18 +199,29 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
             acc[i] = TAcc(0);
         }
         constexpr int G = DiaRows<TA>::G;
+        const int32_t k = ndiag >> 1;
         for (int d = 0; d < ndiag; d += G) {
             TA vv[G][R];
             TX xv[G][R];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 if (d + u < ndiag) {
-                    const TA *vd = val + (int64_t)(d + u) * ld;
-                    const int32_t o = off[d + u];
+                    const int32_t dd = d + u, o = off[dd];
+                    if constexpr (SYM) {
+                        const TA *vd = val + (int64_t)(dd < k ? k - dd : dd - k) * ld;
+                        const int32_t sh = dd < k ? o : 0;
 #pragma unroll
-                    for (int i = 0; i < R; ++i) {
-                        vv[u][i] = ld_stream(vd + min(row[i], (int32_t)ld - 1));
-                        xv[u][i] = x[((m[i] >> (d + u)) & 1u) ? row[i] + o : 0];
+                        for (int i = 0; i < R; ++i) {
+                            vv[u][i] = vd[min(max(row[i] + sh, 0), (int32_t)ld - 1)];
+                            xv[u][i] = x[((m[i] >> dd) & 1u) ? row[i] + o : 0];
+                        }
+                    } else {
+                        const TA *vd = val + (int64_t)dd * ld;
+#pragma unroll
+                        for (int i = 0; i < R; ++i) {
+                            vv[u][i] = ld_stream(vd + min(row[i], (int32_t)ld - 1));
+                            xv[u][i] = x[((m[i] >> dd) & 1u) ? row[i] + o : 0];
+                        }
                     }
                 }
             }
@@ -246,18 +261,20 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
         const dim3 g(A->nparts), blk(kDiaWG);
         const TA *v = (const TA *)A->dia_val;
-        if (A->ndiag <= 8)
-            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint8_t>), g, blk, 0, stream_of(ctx), A->nrows,
-                               A->ndiag, A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, ctx->partials,
-                               ctx->max_partials);
-        else if (A->ndiag <= 16)
-            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint16_t>), g, blk, 0, stream_of(ctx), A->nrows,
-                               A->ndiag, A->dia_off, v, (const uint16_t *)A->dia_mask, x, epi, ctx->partials,
-                               ctx->max_partials);
-        else
-            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint32_t>), g, blk, 0, stream_of(ctx), A->nrows,
-                               A->ndiag, A->dia_off, v, (const uint32_t *)A->dia_mask, x, epi, ctx->partials,
-                               ctx->max_partials);
+#define UHIP_DIA_LAUNCH(TM, SYM)                                                                                  \
+    hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, TM, SYM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, \
+                       A->dia_off, v, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
+        if (A->ndiag <= 8) {
+            if (A->dia_sym) UHIP_DIA_LAUNCH(uint8_t, true);
+            else UHIP_DIA_LAUNCH(uint8_t, false);
+        } else if (A->ndiag <= 16) {
+            if (A->dia_sym) UHIP_DIA_LAUNCH(uint16_t, true);
+            else UHIP_DIA_LAUNCH(uint16_t, false);
+        } else {
+            if (A->dia_sym) UHIP_DIA_LAUNCH(uint32_t, true);
+            else UHIP_DIA_LAUNCH(uint32_t, false);
+        }
+#undef UHIP_DIA_LAUNCH
         UHIP_CHECK_LAUNCH();
         return UHIP_OK;
     }
