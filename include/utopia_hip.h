@@ -176,7 +176,9 @@ typedef struct uhip_csr {
     void *dia_mask;     /* DIA: per-row bit d set iff the row stores offset d (uint8 / uint16 / uint32
                          * for ndiag <= 8 / 16 / 32) */
     int32_t dia_sym;    /* DIA of a bitwise-symmetric matrix: dia_val holds only the offsets >= 0
-                         * (ndiag/2 + 1 arrays); offset -o of row r is read as offset +o of row r - o */
+                         * (ndiag/2 + 1 arrays); offset -o of row r is read as offset +o of row r - o.
+                         * 2: the same storage of a 7-point operator whose largest offset is a multiple
+                         * of 64, run by the plane-blocked kernel */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)

@@ -118,6 +118,8 @@ DIA_MATRICES = {
     "2d9_shift": lambda: ua.gen_stencil(ua.STENCIL_2D9, 33, 31, shift=1e-3),
     "galerkin27": lambda: _galerkin(18),
     "edgecoef_odd": lambda: ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 45, 13, seed=3),
+    "3d7_planes": lambda: ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 8, 11, seed=6),
+    "3d7_planes_one_slab": lambda: ua.gen_stencil(ua.STENCIL_3D7, 8, 8, 3),
     "asym_values": lambda: _perturbed(ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 9, 8, 7, seed=4), 100, 1),
     "asym_pattern": lambda: _dropped(ua.gen_stencil(ua.STENCIL_2D5, 11, 9), 40),
 }
@@ -125,7 +127,7 @@ DIA_MATRICES = {
 # symmetric DIA storage (offsets >= 0 only) is taken exactly when the values are
 # symmetric bit for bit
 DIA_SYMMETRIC = {"2d5": 1, "3d7_varcoef": 1, "2d9_shift": 1, "edgecoef_odd": 1, "asym_values": 0,
-                 "asym_pattern": 0}
+                 "asym_pattern": 0, "3d7_planes": 2, "3d7_planes_one_slab": 2}
 
 
 def _perturbed(A, row, k):
@@ -156,7 +158,8 @@ def test_spmv_dia_bit_exact(ctx, name, dt):
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.format == ua.FMT_DIA
     if name in DIA_SYMMETRIC:
-        assert D.m.dia_sym == DIA_SYMMETRIC[name]
+        # the plane-blocked 7-point kernel (2) serves FP32 operators; FP64 ones keep 1
+        assert D.m.dia_sym == (min(DIA_SYMMETRIC[name], 1) if dt is np.float64 else DIA_SYMMETRIC[name])
     x = ua.gen_uniform(17, n)
     if dt is np.float32:
         x = x.astype(np.float32)
@@ -209,13 +212,22 @@ def test_dia_rejects_unsorted_and_rectangular(ctx):
         ua.DeviceCsr(ctx, ua.gen_prolongation(8, 8), fmt=ua.FMT_DIA)
 
 
-@pytest.mark.parametrize("fmt", [ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA])
-def test_fused_epilogues_both_formats_with_dot(ctx, fmt):
-    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 20, 18, 13, seed=9)
+@pytest.mark.parametrize("fmt,dims", [(ua.FMT_ROWBLOCK, (20, 18, 13)), (ua.FMT_SELL64, (20, 18, 13)),
+                                      (ua.FMT_DIA, (20, 18, 13)), (ua.FMT_DIA, (16, 16, 13))])
+def test_fused_epilogues_both_formats_with_dot(ctx, fmt, dims):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, *dims, seed=9)
     n = A.shape[0]
     x, b = ua.gen_uniform(1, n), ua.gen_uniform(2, n)
     Ax = orc.spmv(A, x)
     D = ua.DeviceCsr(ctx, A, fmt=fmt)
+    D32 = ua.DeviceCsr(ctx, A, dtype=np.float32, fmt=fmt)
+    if dims == (16, 16, 13):
+        assert D32.m.dia_sym == 2  # plane-blocked 7-point kernel, with a partial last slab
+    x32, b32 = x.astype(np.float32), b.astype(np.float32)
+    x32d, b32d = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray.from_numpy(ctx, b32)
+    out32 = ua.DeviceArray(ctx, n, np.float32)
+    D32.residual(x32d, b32d, out32)
+    assert np.array_equal(out32.numpy(), b32 - orc.spmv_f32(A, x32))
     xd, bd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b)
     out = ua.DeviceArray(ctx, n)
     D.residual(xd, bd, out)
@@ -251,8 +263,9 @@ def test_spmv_empty_matrix_rows(ctx):
     assert np.array_equal(yd.numpy(), np.zeros(5))
 
 
-def test_fused_spmv_epilogues(ctx):
-    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 12, 10, 8, seed=9)
+@pytest.mark.parametrize("dims", [(12, 10, 8), (16, 8, 10)])
+def test_fused_spmv_epilogues(ctx, dims):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, *dims, seed=9)
     n = A.shape[0]
     x, b, z, m = (ua.gen_uniform(s, n) for s in (1, 2, 3, 4))
     m = (m > 0).astype(np.float64)

@@ -311,11 +311,18 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
         }
     }
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    // FP32 symmetric 7-point operators with the largest offset a multiple of 64 take the
+    // plane-blocked kernel (its FP64 instances hold 110+ VGPRs, occupancy 4, and lose
+    // more on the epilogue loads than they gain on the matrix)
+    if (sym && A->dtype == UHIP_F32 && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] &&
+        offs[6] % 64 == 0 && (int64_t)n + (int64_t)kDia7Planes * offs[6] < INT32_MAX)
+        sym = 2;
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->dia_sym = sym;
     A->format = UHIP_FMT_DIA;
-    A->nparts = (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
+    A->nparts = sym == 2 ? dia7z_workgroups(n, offs[6])
+                         : (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
     return UHIP_OK;
 }
 

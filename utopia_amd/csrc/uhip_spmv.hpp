@@ -249,6 +249,91 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
     }
 }
 
+// Symmetric 7-point DIA (offsets -S2, -S1, -1, 0, +1, +S1, +S2, S2 a multiple of 64;
+// stored: main, +1, +S1, +S2), blocked over the largest offset: a wave owns 64
+// consecutive rows of kDia7Planes consecutive planes (plane stride S2), lane l
+// row slab * P * S2 + z * S2 + seg * 64 + l. The -S2 mirror and x at +-S2 come from
+// the neighbouring plane in registers, the -1 mirror from a wave shuffle of the +1
+// values, so every stored value is read once and streams; the row sums keep the
+// column order (bit-identical to the CSR sum).
+constexpr int kDia7Planes = 4;
+
+template <typename TAcc, class Epi, typename TA, typename TX>
+__global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int32_t *__restrict__ off,
+                                                       const TA *__restrict__ val, const uint8_t *__restrict__ mask,
+                                                       const TX *__restrict__ x, Epi epi,
+                                                       double *__restrict__ partials, int64_t stride) {
+    constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
+    constexpr int P = kDia7Planes;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int64_t ld = dia_ld(nrows);
+    const int32_t S1 = off[5], S2 = off[6];
+    const int32_t segs = S2 >> 6;
+    const int32_t wv = b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6);
+    const int32_t slab = wv / segs, seg = wv - slab * segs;
+    const int64_t slab0 = (int64_t)slab * P * S2;
+    double part[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) part[d] = 0.0;
+    if (slab0 < nrows) {
+        const TA *v0 = val, *v1 = val + ld, *vS1 = val + 2 * ld, *vS2 = val + 3 * ld;
+        const int32_t last = (int32_t)ld - 1;
+        int32_t r[P];
+        uint32_t m[P];
+        TA a0[P], a1[P], aS1[P], aS2[P], mS1[P];
+        TX xc[P], xm1[P], xp1[P], xmS1[P], xpS1[P];
+#pragma unroll
+        for (int z = 0; z < P; ++z) {
+            r[z] = (int32_t)(slab0 + (int64_t)z * S2) + seg * 64 + lane;
+            const int32_t rc = min(r[z], last);
+            m[z] = r[z] < nrows ? (uint32_t)mask[r[z]] : 0u;
+            a0[z] = ld_stream(v0 + rc);
+            a1[z] = ld_stream(v1 + rc);
+            aS1[z] = ld_stream(vS1 + rc);
+            aS2[z] = ld_stream(vS2 + rc);
+            mS1[z] = ld_stream(vS1 + max(rc - S1, 0));
+            xc[z] = x[min(r[z], nrows - 1)];
+            xm1[z] = x[((m[z] >> 2) & 1u) ? r[z] - 1 : 0];
+            xp1[z] = x[((m[z] >> 4) & 1u) ? r[z] + 1 : 0];
+            xmS1[z] = x[((m[z] >> 1) & 1u) ? r[z] - S1 : 0];
+            xpS1[z] = x[((m[z] >> 5) & 1u) ? r[z] + S1 : 0];
+        }
+        const TA mS2_0 = ld_stream(vS2 + max(min(r[0], last) - S2, 0));
+        const TX xmS2_0 = x[(m[0] & 1u) ? r[0] - S2 : 0];
+        const TX xpS2_P = x[((m[P - 1] >> 6) & 1u) ? r[P - 1] + S2 : 0];
+#pragma unroll
+        for (int z = 0; z < P; ++z) {
+            const TA up = __shfl_up(a1[z], 1, 64);
+            const TA edge = v1[max(min(r[z], last) - 1, 0)];
+            const TA m1 = lane == 0 ? edge : up;
+            const TA vals[7] = {z == 0 ? mS2_0 : aS2[z > 0 ? z - 1 : 0], mS1[z], m1, a0[z], a1[z], aS1[z], aS2[z]};
+            const TX xs[7] = {z == 0 ? xmS2_0 : xc[z > 0 ? z - 1 : 0], xmS1[z], xm1[z], xc[z], xp1[z], xpS1[z],
+                              z == P - 1 ? xpS2_P : xc[z < P - 1 ? z + 1 : 0]};
+            TAcc acc = TAcc(0);
+#pragma unroll
+            for (int d = 0; d < 7; ++d) {
+                const TAcc p = (TAcc)vals[d] * (TAcc)xs[d];
+                acc = ((m[z] >> d) & 1u) ? acc + p : acc;
+            }
+            if (r[z] < nrows) epi(r[z], acc, part);
+        }
+    }
+    if constexpr (Epi::kDots > 0) {
+#pragma unroll
+        for (int d = 0; d < Epi::kDots; ++d) {
+            const double t = block_sum<kDiaWG>(part[d]);
+            if (threadIdx.x == 0) partials[d * stride + b] = t;
+        }
+    }
+}
+
+inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2) {
+    const int64_t slabs = ((int64_t)nrows + (int64_t)kDia7Planes * S2 - 1) / ((int64_t)kDia7Planes * S2);
+    const int64_t waves = slabs * (S2 / 64);
+    return (int32_t)((waves + kDiaWG / 64 - 1) / (kDiaWG / 64));
+}
+
 // Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
 // Matrix values have the accumulation type TAcc.
 template <typename TAcc, typename TX, class Epi>
@@ -264,7 +349,10 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
 #define UHIP_DIA_LAUNCH(TM, SYM)                                                                                  \
     hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, TM, SYM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, \
                        A->dia_off, v, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
-        if (A->ndiag <= 8) {
+        if (A->dia_sym == 2) {
+            hipLaunchKernelGGL((k_spmv_dia7z<TAcc, Epi, TA, TX>), g, blk, 0, stream_of(ctx), A->nrows, A->dia_off, v,
+                               (const uint8_t *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials);
+        } else if (A->ndiag <= 8) {
             if (A->dia_sym) UHIP_DIA_LAUNCH(uint8_t, true);
             else UHIP_DIA_LAUNCH(uint8_t, false);
         } else if (A->ndiag <= 16) {
