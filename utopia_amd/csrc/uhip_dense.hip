@@ -160,7 +160,9 @@ __global__ __launch_bounds__(256) void k_identity(int n, double *__restrict__ B)
         B[e] = (e / n == e % n) ? 1.0 : 0.0;
 }
 
-// y = M x, one wavefront per row, fixed lane order.
+// y = M x, one wavefront per row. Lane l keeps eight chains, chain u summing columns
+// l + 64 u + 512 j in order (eight loads in flight per lane); the chains are combined
+// in a fixed tree and the lanes by the butterfly, so the result is the same on every run.
 template <typename TX, typename TY>
 __global__ __launch_bounds__(256) void k_gemv(int n, const double *__restrict__ M, const TX *__restrict__ x,
                                               TY *__restrict__ y) {
@@ -168,8 +170,20 @@ __global__ __launch_bounds__(256) void k_gemv(int n, const double *__restrict__ 
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     const double *m = M + (int64_t)row * n;
-    double s = 0.0;
-    for (int c = lane; c < n; c += 64) s += m[c] * (double)x[c];
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int c = lane;
+    for (; c + 7 * 64 < n; c += 8 * 64) {
+        double mv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            mv[u] = m[c + u * 64];
+            xv[u] = (double)x[c + u * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += mv[u] * xv[u];
+    }
+    for (int u = 0; c < n; c += 64, ++u) a[u] += m[c] * (double)x[c];
+    double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     s = wave_sum(s);
     if (lane == 0) y[row] = (TY)s;
 }
