@@ -524,3 +524,25 @@ def test_cg_amg_varcoef_matches_oracle(ctx):
     assert abs(cg.status["iterates"] - so["iterates"]) <= 1
     assert rel(x, xo) <= 1e-10
     assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol
+
+
+@pytest.mark.parametrize("pc", [None, "jacobi"])
+def test_cg_reference_hexa_fem_system(ctx, golden, pc):
+    """The reference's K x = M rhs fixture (petsc_matlab_connection): SpMV bit-exact,
+    CG iterations and solution equal to the oracle's, residual below the reference's 1e-6."""
+    d = golden / "utopia_data"
+    K = petsc_io.read_mat(d / "K_hexa_10x10x10_3D")
+    M = petsc_io.read_mat(d / "M_hexa_10x10x10_3D")
+    rhs = petsc_io.read_vec(d / "RHS_10x10x10_hexa_3D")
+    Md = ua.DeviceCsr(ctx, M)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, rhs), ua.DeviceArray(ctx, M.shape[0])
+    Md.spmv(xd, yd)
+    b = yd.numpy()
+    assert np.array_equal(b, orc.spmv(M, rhs))
+    cg = ua.ConjugateGradient(ctx, pc, atol=1e-10, rtol=0, stol=0, max_it=2000)
+    x = cg.solve(K, b)
+    xo, so = orc.cg_solve(K, b, precond=0 if pc is None else 1, atol=1e-10, rtol=0, stol=0, max_it=2000)
+    print(f"hexa fem cg pc={pc}: gpu it={cg.status['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-10
+    assert np.linalg.norm(b - K.to_scipy() @ x) < 1e-6

@@ -650,3 +650,19 @@ def test_cg_amg_solves_reference_amg_problem():
         mg.set_colors(l, *orc.jp_color(As[l]))
     x, st = mg.cg_solve(b, atol=1e-9, max_it=200)
     assert st["converged"] and np.abs(b - A.to_scipy() @ x).sum() < 1e-5
+
+
+def test_cg_solves_reference_hexa_fem_system(golden):
+    """PetscTest::petsc_matlab_connection (tests/petsc/utopia_PetscTest.cpp:396-421):
+    K x = M rhs on the 10x10x10 hexahedral stiffness / mass matrices, |M rhs - K x| < 1e-6
+    (the reference solves it with BiCGStab; CG converges on it too)."""
+    d = golden / "utopia_data"
+    K = petsc_io.read_mat(d / "K_hexa_10x10x10_3D")
+    M = petsc_io.read_mat(d / "M_hexa_10x10x10_3D")
+    rhs = petsc_io.read_vec(d / "RHS_10x10x10_hexa_3D")
+    assert K.shape == M.shape == (1331, 1331)
+    b = orc.spmv(M, rhs)
+    assert np.abs(b - M.to_scipy() @ rhs).max() <= 1e-14 * (abs(M.to_scipy()) @ np.abs(rhs)).max()
+    for pc in (0, 1):
+        x, st = orc.cg_solve(K, b, precond=pc, atol=1e-10, rtol=0, stol=0, max_it=2000)
+        assert st["converged"] and np.linalg.norm(b - K.to_scipy() @ x) < 1e-6
