@@ -134,3 +134,26 @@ def test_petsc_binary_roundtrip_and_reference_fixture(tmp_path, golden):
     assert np.array_equal(A2.rowptr, A.rowptr) and np.array_equal(A2.col, A.col) and np.array_equal(A2.val, A.val)
     assert np.array_equal(petsc_io.read_vec(tmp_path / "b"), b)
     assert (tmp_path / "A").read_bytes() == (golden / "utopia_data" / "f_A").read_bytes()
+
+
+def test_matrix_market_reference_fixture_against_scipy(tmp_path, golden):
+    """gre_343_343_crg.mm (the reference's MatrixMarket fixture): the reader agrees with
+    scipy.io.mmread entry for entry; write -> read round trips exactly; symmetric and
+    pattern files mirror / fill as the format says."""
+    import scipy.io
+    path = golden / "utopia_data" / "gre_343_343_crg.mm"
+    A = petsc_io.read_matrix_market(path)
+    S = scipy.io.mmread(str(path)).tocsr()
+    S.sum_duplicates()
+    S.sort_indices()
+    assert A.shape == S.shape == (343, 343)
+    assert np.array_equal(A.rowptr, S.indptr) and np.array_equal(A.col, S.indices) and np.array_equal(A.val, S.data)
+    petsc_io.write_matrix_market(tmp_path / "A.mm", A)
+    B = petsc_io.read_matrix_market(tmp_path / "A.mm")
+    assert np.array_equal(B.rowptr, A.rowptr) and np.array_equal(B.col, A.col) and np.array_equal(B.val, A.val)
+    (tmp_path / "s.mm").write_text("%%MatrixMarket matrix coordinate real symmetric\n% c\n3 3 4\n1 1 2\n2 1 -1\n"
+                                   "3 2 -1.5\n2 1 0.25\n")
+    C = petsc_io.read_matrix_market(tmp_path / "s.mm").to_scipy().toarray()
+    assert np.array_equal(C, np.array([[2, -0.75, 0], [-0.75, 0, -1.5], [0, -1.5, 0]]))
+    (tmp_path / "p.mm").write_text("%%MatrixMarket matrix coordinate pattern skew-symmetric\n2 2 1\n2 1\n")
+    assert np.array_equal(petsc_io.read_matrix_market(tmp_path / "p.mm").to_scipy().toarray(), [[0, -1], [1, 0]])

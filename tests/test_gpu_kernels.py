@@ -508,3 +508,21 @@ def test_context_autotune_applies_to_automatic_plans(ctx):
     assert np.array_equal(yd.numpy(), orc.spmv(A, x))
     del D, xd, yd
     c2.close()
+
+
+def test_spmv_matrix_market_reference_fixture(ctx, golden):
+    """gre_343_343_crg.mm (general, non-symmetric): every plan and both precisions give
+    the oracle's bits."""
+    from utopia_amd import petsc_io
+    A = petsc_io.read_matrix_market(golden / "utopia_data" / "gre_343_343_crg.mm")
+    x = ua.gen_uniform(31, A.shape[1])
+    for fmt in (ua.FMT_AUTO, ua.FMT_ROWBLOCK, ua.FMT_SELL64):
+        D = ua.DeviceCsr(ctx, A, fmt=fmt)
+        xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, A.shape[0])
+        D.spmv(xd, yd)
+        assert np.array_equal(yd.numpy(), orc.spmv(A, x)), fmt
+    D = ua.DeviceCsr(ctx, A, dtype=np.float32)
+    x32 = x.astype(np.float32)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray(ctx, A.shape[0], np.float32)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv_f32(A, x32))

@@ -68,3 +68,63 @@ def write_vec(path, x: np.ndarray):
     with open(path, "wb") as f:
         f.write(np.array([VEC_FILE_CLASSID, x.size], ">i4").tobytes())
         f.write(np.asarray(x, ">f8").tobytes())
+
+
+# ---------------------------------------------------------------- MatrixMarket
+# Coordinate MatrixMarket files (the text format utopia reads through the Trilinos
+# backend, tests/trilinos/utopia_TrilinosTest.cpp:983-988, and that
+# data/matrixmarket/gre_343_343_crg.mm holds). Header
+# "%%MatrixMarket matrix coordinate <real|integer|pattern> <general|symmetric|skew-symmetric>",
+# '%' comment lines, a "rows cols entries" line, then 1-based "i j [value]" lines.
+# symmetric / skew-symmetric files store one triangle; the other is mirrored
+# (negated for skew). Entries repeated at one position are summed; rows come out
+# with sorted columns.
+
+def read_matrix_market(path) -> Csr:
+    with open(path, "r") as f:
+        header = f.readline().split()
+        if len(header) < 5 or header[0].lower() != "%%matrixmarket" or header[1].lower() != "matrix":
+            raise ValueError(f"{path}: not a MatrixMarket matrix file")
+        fmt, field, sym = (h.lower() for h in header[2:5])
+        if fmt != "coordinate":
+            raise ValueError(f"{path}: only coordinate MatrixMarket files are supported (got {fmt})")
+        if field not in ("real", "integer", "pattern") or sym not in ("general", "symmetric", "skew-symmetric"):
+            raise ValueError(f"{path}: unsupported field/symmetry {field}/{sym}")
+        line = f.readline()
+        while line.startswith("%") or not line.strip():
+            line = f.readline()
+        m, n, nz = (int(t) for t in line.split()[:3])
+        data = np.loadtxt(f, ndmin=2) if nz > 0 else np.zeros((0, 3))
+    if data.shape[0] != nz:
+        raise ValueError(f"{path}: header says {nz} entries, found {data.shape[0]}")
+    i = data[:, 0].astype(np.int64) - 1
+    j = data[:, 1].astype(np.int64) - 1
+    v = np.ones(nz) if field == "pattern" else data[:, 2].astype(np.float64)
+    if sym != "general":
+        off = i != j
+        i, j, v = (np.concatenate([i, j[off]]), np.concatenate([j, i[off]]),
+                   np.concatenate([v, (-1.0 if sym == "skew-symmetric" else 1.0) * v[off]]))
+    if nz and (i.min() < 0 or j.min() < 0 or i.max() >= m or j.max() >= n):
+        raise ValueError(f"{path}: index out of range")
+    # sum repeated positions in file order, then sort each row by column
+    key = i * n + j
+    order = np.argsort(key, kind="stable")
+    key, v = key[order], v[order]
+    uniq, start = np.unique(key, return_index=True)
+    vals = np.add.reduceat(v, start) if v.size else v
+    rows = (uniq // n).astype(np.int64)
+    cols = (uniq % n).astype(np.int32)
+    rowptr = np.zeros(m + 1, np.int64)
+    np.add.at(rowptr, rows + 1, 1)
+    np.cumsum(rowptr, out=rowptr)
+    return Csr(rowptr.astype(np.int32), cols, np.asarray(vals, np.float64), (m, n))
+
+
+def write_matrix_market(path, A: Csr):
+    m, n = A.shape
+    rows = np.repeat(np.arange(m), np.diff(A.rowptr.astype(np.int64)))
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{m} {n} {A.nnz}\n")
+        for r, c, v in zip(rows, A.col, A.val):
+            f.write(f"{r + 1} {c + 1} {float(v)!r}\n")
