@@ -160,7 +160,8 @@ def bench_batched(a):
     Xs = ua.gen_normal(a.seed + 8, p.n * k)
     log(f"batched: 64 systems, sides {min(sides)}..{max(sides)}, rows {p.n}, nnz {p.A.nnz}, levels {p.levels}, "
         f"generated in {time.perf_counter() - t0:.1f} s")
-    bs = bt.BatchedSolver(ctx, k, mixed=True, rtol=1e-8, max_it=500, pre_smoothing_steps=2, post_smoothing_steps=2)
+    bs = bt.BatchedSolver(ctx, k, mixed=True, rtol=1e-8, max_it=500, pre_smoothing_steps=2, post_smoothing_steps=2,
+                          autotune=a.autotune)
     t0 = time.perf_counter()
     bs.setup(p)
     setup = time.perf_counter() - t0

@@ -179,6 +179,8 @@ typedef struct uhip_csr {
                          * (ndiag/2 + 1 arrays); offset -o of row r is read as offset +o of row r - o.
                          * 2: the same storage of a 7-point operator whose largest offset is a multiple
                          * of 64, run by the plane-blocked kernel */
+    int32_t spmm_kernel; /* multi-RHS products (uhip_spmm*): 0 by row length (wavefront per row when
+                          * nnz >= 32 rows), 1 one thread per row, 2 one wavefront per row */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)
@@ -445,6 +447,12 @@ int uhip_lsolve_set_verbosity(uhip_solver s, int level);
  * (backend/petsc/utopia_petsc_Matrix.cpp:1137-1152) with per-row sums in entry order,
  * except for matrices averaging >= 32 entries per row (dense blocks), whose rows
  * are summed by one wavefront each in a fixed tree order. */
+/* Autotuner of the multi-RHS product: times Y = A X (width k in {1,2,4,8,16}, A's
+ * precision) with one thread per row and with one wavefront per row, reps launches
+ * each, and keeps the faster in A->spmm_kernel (choice and its average ms returned).
+ * The thread-per-row kernel sums each row in column order, the wavefront kernel in a
+ * fixed tree. No reference analogue (utopia vectors are single-column). */
+int uhip_csr_autotune_spmm(uhip_ctx ctx, uhip_csr *A, int k, int reps, int32_t *choice, float *ms);
 int uhip_spmm(uhip_ctx ctx, const uhip_csr *A, int k, const void *X, int x_dtype, void *Y, int y_dtype, int op,
               const void *B, int b_dtype);
 /* One Chebyshev3level step (solvers/smoothers/utopia_Chebyshev3level.hpp:150-175)
@@ -487,7 +495,9 @@ typedef struct uhip_bsolver_s *uhip_bsolver;
 int uhip_bsolver_create(uhip_ctx ctx, int k, int mixed, uhip_bsolver *out);
 int uhip_bsolver_destroy(uhip_bsolver s);
 /* rtol max_it pre_smoothing_steps post_smoothing_steps scale_max_eig scale_min_eig
- * power_method_tol power_method_max_it use_graph */
+ * power_method_tol power_method_max_it use_graph graph_iterations; autotune (reps > 0: at setup
+ * the multi-RHS kernel of every level operator, transfer and coarse block inverse is chosen by
+ * uhip_csr_autotune_spmm) */
 int uhip_bsolver_set_param(uhip_bsolver s, const char *key, double value);
 /* Block-diagonal operator (device CSR, FP64) with seg[0..nseg] row offsets;
  * nlev-1 interpolations coarse to fine (device CSR, finest last), coarse_seg

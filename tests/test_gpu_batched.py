@@ -229,3 +229,38 @@ def test_batched_rejects_bad_setup(ctx):
         bs.setup(bad)
     with pytest.raises(ua.HipError):
         bt.BatchedSolver(ctx, 3)
+
+
+def test_spmm_autotuner_and_autotuned_batched_solve(ctx):
+    """uhip_csr_autotune_spmm keeps one of the two kernels and the product stays exact
+    (thread-per-row: column order; the tree-ordered wavefront kernel within rounding);
+    a batched solve with autotuned kernels converges like the heuristic one."""
+    A = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 60, 50, seed=2)
+    k = 8
+    X = ua.gen_uniform(3, A.shape[1] * k).reshape(-1, k)
+    D = ua.DeviceCsr(ctx, A)
+    choice, ms = ctypes.c_int32(), ctypes.c_float()
+    ua.check(ua.lib().uhip_csr_autotune_spmm(ctx.h, D.ref, k, 5, ctypes.byref(choice), ctypes.byref(ms)))
+    assert choice.value in (1, 2) and D.m.spmm_kernel == choice.value and ms.value > 0
+    Xd, Yd = ua.DeviceArray.from_numpy(ctx, X.ravel()), ua.DeviceArray(ctx, A.shape[0] * k)
+    ua.check(ua.lib().uhip_spmm(ctx.h, D.ref, k, ctypes.c_void_p(Xd.ptr), ua.F64, ctypes.c_void_p(Yd.ptr), ua.F64, 0,
+                                None, 0))
+    Y = Yd.numpy().reshape(-1, k)
+    ref = np.stack([orc.spmv(A, np.ascontiguousarray(X[:, j])) for j in range(k)], 1)
+    if choice.value == 1:
+        assert np.array_equal(Y, ref)
+    else:
+        assert np.abs(Y - ref).max() <= 1e-14 * np.abs(ref).max()
+    p = bt.make_batch([20, 33, 17, 40, 24], seed=3, levels=3)
+    B = ua.gen_uniform(77, p.n * 4).reshape(p.n, 4)
+    out = []
+    for at in (0, 3):
+        ua.srand(1)
+        bs = bt.BatchedSolver(ctx, 4, mixed=True, rtol=1e-10, max_it=500, pre_smoothing_steps=2,
+                              post_smoothing_steps=2, autotune=at)
+        bs.setup(p)
+        out.append((bs.solve(B), bs.convergence()))
+    (X0, (i0, f0)), (X1, (i1, f1)) = out
+    assert (f0 == 1).all() and (f1 == 1).all()
+    assert np.abs(i0.astype(int) - i1.astype(int)).max() <= 1
+    assert rel(X1, X0) <= 1e-8

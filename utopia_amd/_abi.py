@@ -42,6 +42,7 @@ class uhip_csr(ctypes.Structure):
         ("dia_val", ctypes.c_void_p),
         ("dia_mask", ctypes.c_void_p),
         ("dia_sym", ctypes.c_int32),
+        ("spmm_kernel", ctypes.c_int32),
     ]
 
 

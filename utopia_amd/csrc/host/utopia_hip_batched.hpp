@@ -339,12 +339,26 @@ private:
         const int L = (int)A64_.size();
         lv.clear();
         lv.resize((size_t)L);
+        // reps > 0: the multi-RHS kernel of every level operator and transfer is chosen by timing
+        const int reps = (int)param("autotune", 0);
+        auto tune = [&](const uhip_csr &m) {
+            if (reps > 0)
+                hip_check(uhip_csr_autotune_spmm(ctx_->get(), const_cast<uhip_csr *>(&m), K_, reps, nullptr, nullptr),
+                          "autotune spmm");
+        };
+        if (reps > 0) {
+            tune(A64_.back()->raw());  // the FP64 operator of the outer CG
+            if (coarse_inv_) tune(coarse_inv_->raw());
+        }
         for (int l = 0; l < L; ++l) {
             auto &d = lv[(size_t)l];
             d.A = cast<T>(A64_[(size_t)l]);
+            tune(d.A->raw());
             if (l > 0) {
                 d.R = cast<T>(R64_[(size_t)l]);
                 d.P = cast<T>(P64_[(size_t)l]);
+                tune(d.R->raw());
+                tune(d.P->raw());
             }
             const int64_t nk = (int64_t)layout_[(size_t)l].rows() * K_;
             for (HipVectorT<T> *v : {&d.c, &d.r, &d.rR, &d.rk, &d.pa, &d.pb}) *v = HipVectorT<T>(ctx_, nk, T(0));

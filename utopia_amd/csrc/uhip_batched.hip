@@ -7,6 +7,8 @@
 //             column index are K contiguous values.
 //   segdot    per (segment, column) dot products, two fixed-order stages.
 //   batched CG / Chebyshev steps with per-(segment, column) step lengths.
+#include <algorithm>
+
 #include "uhip_common.hpp"
 
 namespace uhip {
@@ -195,12 +197,14 @@ __global__ __launch_bounds__(256) void k_spmm_wave(int32_t n, const int32_t *__r
 }
 
 // Rows averaging >= 32 entries take the wavefront-per-row kernel (its sums are
-// tree-ordered); all others the row-per-thread kernel (sequential CSR order).
+// tree-ordered); all others the row-per-thread kernel (sequential CSR order),
+// unless the matrix carries an autotuned choice (spmm_kernel 1 or 2).
 template <int K, typename TAcc, class Epi, typename TA, typename TX>
 int launch_spmm(uhip_ctx ctx, const uhip_csr *A, const TX *X, const Epi &e) {
     const int32_t n = A->nrows;
     if (n == 0) return UHIP_OK;
-    if (A->nnz >= 32LL * n)
+    const bool wave = A->spmm_kernel == 2 || (A->spmm_kernel != 1 && A->nnz >= 32LL * n);
+    if (wave)
         hipLaunchKernelGGL((k_spmm_wave<K, TAcc, Epi, TA, TX>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n,
                            A->rowptr, A->col, (const TA *)A->val, X, e);
     else
@@ -434,6 +438,49 @@ int uhip_spmm(uhip_ctx ctx, const uhip_csr *A, int k, const void *X, int x_dtype
     }
 #undef UHIP_SPMM_T
     return set_error(UHIP_ERR_INVALID, "uhip_spmm: dtype combination");
+}
+
+int uhip_csr_autotune_spmm(uhip_ctx ctx, uhip_csr *A, int k, int reps, int32_t *choice, float *ms_out) {
+    UHIP_REQUIRE(ctx && A && reps > 0, "uhip_csr_autotune_spmm: bad argument");
+    UHIP_REQUIRE(k == 1 || k == 2 || k == 4 || k == 8 || k == 16, "uhip_csr_autotune_spmm: K=%d", k);
+    const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
+    void *x = nullptr, *y = nullptr;
+    UHIP_CHECK_HIP(hipMalloc(&x, vb * (size_t)k * (size_t)std::max(A->ncols, 1)));
+    UHIP_CHECK_HIP(hipMalloc(&y, vb * (size_t)k * (size_t)std::max(A->nrows, 1)));
+    UHIP_CHECK_HIP(hipMemsetAsync(x, 0, vb * (size_t)k * (size_t)std::max(A->ncols, 1), ctx->stream));
+    hipEvent_t e0, e1;
+    UHIP_CHECK_HIP(hipEventCreate(&e0));
+    UHIP_CHECK_HIP(hipEventCreate(&e1));
+    const int32_t saved = A->spmm_kernel;
+    float best = 1e30f;
+    int32_t win = 1;
+    int st = UHIP_OK;
+    const int dt = A->dtype;
+    for (int32_t cand = 1; cand <= 2 && st == UHIP_OK; ++cand) {
+        A->spmm_kernel = cand;
+        if ((st = uhip_spmm(ctx, A, k, x, dt, y, dt, 0, nullptr, 0)) != UHIP_OK) break;  // warm-up
+        hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && st == UHIP_OK; ++r) st = uhip_spmm(ctx, A, k, x, dt, y, dt, 0, nullptr, 0);
+        hipEventRecord(e1, ctx->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)reps;
+        if (ms < best) best = ms, win = cand;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(x);
+    hipFree(y);
+    if (st != UHIP_OK) {
+        A->spmm_kernel = saved;
+        return st;
+    }
+    A->spmm_kernel = win;
+    if (choice) *choice = win;
+    if (ms_out) *ms_out = best;
+    return UHIP_OK;
 }
 
 int uhip_spmm_cheb_step(uhip_ctx ctx, const uhip_csr *A, int k, const void *pprev, int p_dtype, const void *rprev,
