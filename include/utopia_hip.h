@@ -229,6 +229,11 @@ int uhip_spmv_residual(uhip_ctx ctx, const uhip_csr *A, const void *x, const voi
 /* y = A x and dot_d[0] = x.y      A.apply(p,Ap) + dot(p,Ap) of ConjugateGradient
  * (solvers/smoothers/utopia_ConjugateGradient_impl.hpp:237-238) */
 int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, double *dot_d);
+/* y = A x, dots_d[0] = x.y, dots_d[1] = z.x: MPRGP's A p, p.Ap and g.p (or A beta, beta.A beta,
+ * g.beta) in one pass (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:126-128,
+ * 211-213) */
+int uhip_spmv_dot2_f64(uhip_ctx ctx, const uhip_csr *A, const double *x, double *y, const double *z,
+                       double *dots_d);
 /* y = mask .* (A x), mask may be NULL   IPRTransfer::restrict + MultiLevelMask::apply
  * (solvers/multilevel/transfer/utopia_IPRTransfer.hpp:130-134, utopia_MultiLevelMask.hpp:34-37) */
 int uhip_spmv_masked(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *mask, void *y);

@@ -526,3 +526,18 @@ def test_spmv_matrix_market_reference_fixture(ctx, golden):
     xd, yd = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray(ctx, A.shape[0], np.float32)
     D.spmv(xd, yd)
     assert np.array_equal(yd.numpy(), orc.spmv_f32(A, x32))
+
+
+def test_spmv_dot2(ctx):
+    A = ua.gen_stencil(ua.STENCIL_2D9, 40, 37, shift=1e-3)
+    n = A.shape[0]
+    x, z = ua.gen_uniform(1, n), ua.gen_uniform(2, n)
+    D = ua.DeviceCsr(ctx, A)
+    xd, zd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, z), ua.DeviceArray(ctx, n)
+    s = ctx.scalars
+    ua.check(L().uhip_spmv_dot2_f64(ctx.h, D.ref, P(xd), P(yd), P(zd), ctypes.c_void_p(s)))
+    Ax = orc.spmv(A, x)
+    assert np.array_equal(yd.numpy(), Ax)
+    d = ctx.read_scalars(s, 2)
+    assert abs(d[0] - x @ Ax) <= 1e-13 * (np.abs(x) @ np.abs(Ax))
+    assert abs(d[1] - z @ x) <= 1e-13 * (np.abs(z) @ np.abs(x))

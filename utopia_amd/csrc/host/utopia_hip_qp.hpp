@@ -77,9 +77,8 @@ public:
         double beta_beta = st[0], fi_fi = st[1];
         while (!converged) {
             if (beta_beta <= (gamma * gamma * fi_fi)) {
-                hip_check(uhip_spmv_dot(c, &A.raw(), p_.data(), Ap_.data(), s + S_PAP), "Ap");
-                const double *gx[1] = {g_.data()}, *px[1] = {p_.data()};
-                hip_check(uhip_vec_dots_f64(c, n, 1, gx, px, s + S_GP), "g.p");
+                static_assert(S_GP == S_PAP + 1, "p.Ap and g.p in consecutive slots");
+                hip_check(uhip_spmv_dot2_f64(c, &A.raw(), p_.data(), Ap_.data(), g_.data(), s + S_PAP), "Ap, g.p");
                 hip_check(uhip_mprgp_alpha_f_f64(c, n, x.data(), p_.data(), lb, ub, s + S_AF), "alpha_f");
                 double h[3];
                 A.context()->read_scalars(s + S_PAP, 3, h);
@@ -99,9 +98,9 @@ public:
                     hip_check(uhip_mprgp_free_grad_f64(c, n, x.data(), g_.data(), lb, ub, p_.data()), "p = fi");
                 }
             } else {
-                hip_check(uhip_spmv_dot(c, &A.raw(), beta_.data(), Abeta_.data(), s + S_BAB), "Abeta");
-                const double *gx[1] = {g_.data()}, *bx[1] = {beta_.data()};
-                hip_check(uhip_vec_dots_f64(c, n, 1, gx, bx, s + S_GB), "g.beta");
+                static_assert(S_GB == S_BAB + 1, "beta.Abeta and g.beta in consecutive slots");
+                hip_check(uhip_spmv_dot2_f64(c, &A.raw(), beta_.data(), Abeta_.data(), g_.data(), s + S_BAB),
+                          "Abeta, g.beta");
                 double h[2];
                 A.context()->read_scalars(s + S_BAB, 2, h);
                 const double beta_Abeta = h[0], g_betta = h[1];

@@ -624,6 +624,15 @@ int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, doubl
     return finish_sums(ctx, 1, A->nparts, ctx->max_partials, dot_d);
 }
 
+int uhip_spmv_dot2_f64(uhip_ctx ctx, const uhip_csr *A, const double *x, double *y, const double *z,
+                       double *dots_d) {
+    UHIP_REQUIRE(ctx && A && x && y && z && dots_d, "uhip_spmv_dot2_f64: null argument");
+    UHIP_REQUIRE(A->nrows == A->ncols && A->dtype == UHIP_F64, "uhip_spmv_dot2_f64: square FP64 matrix");
+    EpiStoreDot2<double, double, double> e{x, z, y};
+    UHIP_TRY(launch_rowblock<double>(ctx, A, x, e));
+    return finish_sums(ctx, 2, A->nparts, ctx->max_partials, dots_d);
+}
+
 int uhip_spmv_masked(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *mask, void *y) {
     UHIP_REQUIRE(ctx && A && x && y, "uhip_spmv_masked: null argument");
     if (A->dtype == UHIP_F64) {

@@ -457,6 +457,21 @@ struct EpiStoreDot {
     }
 };
 
+// y = s, part[0] += x_r s, part[1] += z_r x_r   (A x with x.Ax and z.x in one pass)
+template <typename TAcc, typename TY, typename TX>
+struct EpiStoreDot2 {
+    static constexpr int kDots = 2;
+    const TX *x;
+    const double *z;
+    TY *y;
+    __device__ void operator()(int r, TAcc s, double *part) const {
+        st_stream(y + r, (TY)s);
+        const double xr = (double)x[r];
+        part[0] += xr * (double)s;
+        part[1] += z[r] * xr;
+    }
+};
+
 // y = mask .* s  (restriction followed by MultiLevelMask::apply); mask may be null
 template <typename TAcc, typename TY>
 struct EpiMaskStore {
