@@ -143,6 +143,11 @@ int uhip_cg_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *z, dou
  * cg_xpupdate:  x += s[i_alpha] p; if z: p = z + (s[i_num]/s[i_den]) p, then s[i_den] = s[i_num] */
 int uhip_cg_residual_f64(uhip_ctx ctx, int64_t n, const double *r, const double *Ap, double *r_new, float *r_new32,
                          double *scalars, int i_num, int i_den, int i_alpha, int i_rnorm);
+/* x += s[i_alpha] p; p = z + (s[i_num]/s[i_den]) p with z FP64 or FP32 (an FP32 preconditioner
+ * output converts exactly); the scalar slots are left as they are (a caller alternating the
+ * two r.z slots needs no copy) */
+int uhip_cg_xpupdate2(uhip_ctx ctx, int64_t n, double *x, double *p, const void *z, int z_dtype, double *scalars,
+                      int i_alpha, int i_num, int i_den);
 int uhip_cg_xpupdate_f64(uhip_ctx ctx, int64_t n, double *x, double *p, const double *z, double *scalars, int i_alpha,
                          int i_num, int i_den);
 

@@ -641,6 +641,21 @@ public:
         vcycle_apply_dot(eng_, rhs, z, dot_d, rhs32);
         return true;
     }
+    // The FP32 cycle's result is an FP32 vector: kept in FP32 storage it converts
+    // exactly wherever the FP64 CG reads it.
+    const float *apply_zero_initial_guess_dot_z32(const HipVector &rhs, const float *rhs32, double *dot_d) override {
+        const int64_t n = rhs.size();
+        if (z32_.size() != n || z32_.empty()) z32_ = HipVector32(this->op_->context(), n);
+        const bool fused = eng_.cycle((int)eng_.levels.size() - 1, rhs32, UHIP_F32, z32_.data(), UHIP_F32, rhs.data(),
+                                      dot_d);
+        if (!fused) {
+            if (z64_.size() != n || z64_.empty()) z64_ = HipVector(this->op_->context(), n);
+            hip_check(uhip_vec_cast_f32_f64(rhs.ctx(), n, z32_.data(), z64_.data()), "z cast");
+            const double *a[1] = {z64_.data()}, *b[1] = {rhs.data()};
+            hip_check(uhip_vec_dots_f64(rhs.ctx(), n, 1, a, b, dot_d), "dot(z, r)");
+        }
+        return z32_.data();
+    }
     const Multigrid &fp64() const { return *mg_; }
     // Chebyshev bounds of level l on both the FP64 hierarchy and its FP32 copy
     void set_level_eigs(int l, double emax, double emin) {
@@ -653,7 +668,8 @@ public:
 private:
     std::shared_ptr<Multigrid> mg_;
     VCycle<float> eng_;
-    HipVector32 r32_;
+    HipVector32 r32_, z32_;
+    HipVector z64_;
 };
 
 }  // namespace utopia

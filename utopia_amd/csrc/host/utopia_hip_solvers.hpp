@@ -69,6 +69,9 @@ public:
     // instead of rhs (nullptr: not wanted). A caller that fills it with
     // (float)rhs passes it to apply_zero_initial_guess_dot_f32.
     virtual float *rhs_f32_buffer(int64_t) { return nullptr; }
+    // z = M rhs (zero guess) left in FP32 storage owned by the preconditioner, with
+    // dot_d[0] = z.rhs; nullptr when the preconditioner has no FP32 output
+    virtual const float *apply_zero_initial_guess_dot_z32(const Vector &, const float *, double *) { return nullptr; }
     virtual bool apply_zero_initial_guess_dot_f32(const Vector &rhs, const float *, Vector &sol, double *dot_d) {
         return apply_zero_initial_guess_dot(rhs, sol, dot_d);
     }
@@ -228,10 +231,11 @@ private:
         hip_check(uhip_vec_dots_f64(c, b.size(), 1, rz, zz, s + S_RZ), "dot(r,z)");
         bool stop = false;
         float *r32 = precond_->rhs_f32_buffer(b.size());
+        int i_rz = S_RZ, i_rznew = S_RZNEW;  // the two r.z slots swap roles every iteration
         while (!stop) {
             hip_check(uhip_spmv_dot(c, &A.raw(), p_.data(), Ap_.data(), s + S_PAP), "Ap, dot(p,Ap)");
             // alpha, r_new = r - alpha Ap, |r_new|; x += alpha p is applied with the p update
-            hip_check(uhip_cg_residual_f64(c, b.size(), r_.data(), Ap_.data(), r_new_.data(), r32, s, S_RZ, S_PAP,
+            hip_check(uhip_cg_residual_f64(c, b.size(), r_.data(), Ap_.data(), r_new_.data(), r32, s, i_rz, S_PAP,
                                            S_ALPHA, S_RNORM),
                       "cg residual");
             double h[2];
@@ -250,13 +254,18 @@ private:
                 break;
             }
             // z_new.set(0); precond->apply(r_new, z_new); dot(z_new, r_new)
-            if (r32) precond_->apply_zero_initial_guess_dot_f32(r_new_, r32, z_new_, s + S_RZNEW);
-            else precond_->apply_zero_initial_guess_dot(r_new_, z_new_, s + S_RZNEW);
+            const float *z32 = r32 ? precond_->apply_zero_initial_guess_dot_z32(r_new_, r32, s + i_rznew) : nullptr;
+            if (!z32) {
+                if (r32) precond_->apply_zero_initial_guess_dot_f32(r_new_, r32, z_new_, s + i_rznew);
+                else precond_->apply_zero_initial_guess_dot(r_new_, z_new_, s + i_rznew);
+            }
             // x += alpha p; p = z_new + beta p
-            hip_check(uhip_cg_xpupdate_f64(c, b.size(), x.data(), p_.data(), z_new_.data(), s, S_ALPHA, S_RZNEW, S_RZ),
+            hip_check(uhip_cg_xpupdate2(c, b.size(), x.data(), p_.data(), z32 ? (const void *)z32 : z_new_.data(),
+                                        z32 ? UHIP_F32 : UHIP_F64, s, S_ALPHA, i_rznew, i_rz),
                       "x, p update");
             std::swap(r_, r_new_);
             std::swap(z_, z_new_);
+            std::swap(i_rz, i_rznew);
             this->print_iter(it, r_norm);
             stop = this->check_convergence(it, r_norm, 1, 1);
             it++;
@@ -425,7 +434,7 @@ bool chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, 
         st.beta = beta[(size_t)k];
         st.a_cur = alpha[(size_t)k];
         st.last = last ? 1 : 0;
-        if (last && dot_with && out_dt == UHIP_F64) {
+        if (last && dot_with) {  // z.r in the same pass (z of either precision, r FP64)
             st.dot_with = dot_with;
             st.dot_d = dot_d;
         }
@@ -433,7 +442,7 @@ bool chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, 
         pprev = pnext;
         p_dt = dt;
     }
-    return dot_with != nullptr && out_dt == UHIP_F64;
+    return dot_with != nullptr;
 }
 
 }  // namespace utopia

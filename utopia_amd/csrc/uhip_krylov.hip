@@ -78,15 +78,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double 
 
 // x += s[i_alpha]*p, then (z given) p = z + beta*p with beta = s[i_num]/s[i_den]:
 // the x update of iteration k uses p_k before it is replaced by p_{k+1}.
+template <typename TZ>
 __global__ __launch_bounds__(kBlock) void k_cg_xpupdate(int64_t n, double *__restrict__ x, double *__restrict__ p,
-                                                        const double *__restrict__ z, const double *__restrict__ s,
+                                                        const TZ *__restrict__ z, const double *__restrict__ s,
                                                         int i_alpha, int i_num, int i_den) {
     const double alpha = s[i_alpha];
     const double beta = z ? s[i_num] / s[i_den] : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double pi = p[i];
         __builtin_nontemporal_store(x[i] + alpha * pi, x + i);
-        if (z) __builtin_nontemporal_store(z[i] + beta * pi, p + i);
+        if (z) __builtin_nontemporal_store((double)z[i] + beta * pi, p + i);
     }
 }
 
@@ -165,14 +166,30 @@ int uhip_cg_xpupdate_f64(uhip_ctx ctx, int64_t n, double *x, double *p, const do
                          int i_num, int i_den) {
     UHIP_REQUIRE(ctx && x && p && scalars, "uhip_cg_xpupdate_f64: null argument");
     if (n > 0) {
-        hipLaunchKernelGGL(k_cg_xpupdate, dim3(grid_for(n, 4, 1 << 16)), dim3(kBlock), 0, ctx->stream, n, x, p, z,
-                           scalars, i_alpha, i_num, i_den);
+        hipLaunchKernelGGL(k_cg_xpupdate<double>, dim3(grid_for(n, 4, 1 << 16)), dim3(kBlock), 0, ctx->stream, n, x,
+                           p, z, scalars, i_alpha, i_num, i_den);
         UHIP_CHECK_LAUNCH();
     }
     if (z) {
         hipLaunchKernelGGL(k_scalar_copy, dim3(1), dim3(1), 0, ctx->stream, scalars, i_den, i_num);
         UHIP_CHECK_LAUNCH();
     }
+    return UHIP_OK;
+}
+
+int uhip_cg_xpupdate2(uhip_ctx ctx, int64_t n, double *x, double *p, const void *z, int z_dtype, double *scalars,
+                      int i_alpha, int i_num, int i_den) {
+    UHIP_REQUIRE(ctx && x && p && z && scalars, "uhip_cg_xpupdate2: null argument");
+    UHIP_REQUIRE(z_dtype == UHIP_F64 || z_dtype == UHIP_F32, "uhip_cg_xpupdate2: z dtype");
+    if (n <= 0) return UHIP_OK;
+    const dim3 g(grid_for(n, 4, 1 << 16)), b(kBlock);
+    if (z_dtype == UHIP_F64)
+        hipLaunchKernelGGL(k_cg_xpupdate<double>, g, b, 0, ctx->stream, n, x, p, (const double *)z, scalars, i_alpha,
+                           i_num, i_den);
+    else
+        hipLaunchKernelGGL(k_cg_xpupdate<float>, g, b, 0, ctx->stream, n, x, p, (const float *)z, scalars, i_alpha,
+                           i_num, i_den);
+    UHIP_CHECK_LAUNCH();
     return UHIP_OK;
 }
 
