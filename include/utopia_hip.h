@@ -186,12 +186,16 @@ typedef struct uhip_csr {
                          * of 64, run by the plane-blocked kernel */
     int32_t spmm_kernel; /* multi-RHS products (uhip_spmm*): 0 by row length (wavefront per row when
                           * nnz >= 32 rows), 1 one thread per row, 2 one wavefront per row */
+    int32_t *vi_base;   /* VI: per-row base column (the row's first column; 0 for an empty row) */
+    void *vi_table;     /* VI: the matrix's distinct values (dtype), at most 256 */
+    int32_t vi_nvals;   /* VI: number of distinct values */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)
 #define UHIP_FMT_ROWBLOCK 0
 #define UHIP_FMT_SELL64 1
 #define UHIP_FMT_DIA 2
+#define UHIP_FMT_VI 3 /* value-indexed, row-relative offsets (CSR-VI style, see uhip_csr_plan_format) */
 
 /* Build the row-block plan. block_nnz selects the LDS budget (1024, 2048 or 4096
  * nonzeros per block; 0 = automatic format choice, see uhip_csr_plan_format).
@@ -204,9 +208,14 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
  * bitmask (no column indices; fails with UHIP_ERR_UNSUPPORTED otherwise); when
  * the pattern and the values are symmetric bit for bit, only the offsets >= 0 are
  * stored (dia_sym = 1).
- * UHIP_FMT_AUTO takes DIA when it applies with at most 25% padding, else
- * SELL-64 when its padding adds at most 25% to nnz, else the row-block CSR
- * kernel. Each y_i is the sum of exactly the row's CSR entries in column
+ * UHIP_FMT_VI stores a matrix (rectangular too) whose rows are at most 32 distinct
+ * offsets from their own first column and whose values take at most 256 distinct
+ * bit patterns (geometric and aggregation transfer operators) as a per-row base
+ * column, a per-row slot mask and one 8-bit value code per slot (the value index
+ * of CSR-VI, Kourtis et al. 2008), the distinct values in a table.
+ * UHIP_FMT_AUTO takes DIA when it applies with at most 25% padding, else VI when
+ * it applies with at most 1.5 slots per stored entry, else SELL-64 when its padding
+ * adds at most 25% to nnz, else the row-block CSR kernel. Each y_i is the sum of exactly the row's CSR entries in column
  * order in every format. */
 int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz);
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A);

@@ -96,7 +96,7 @@ def test_spmv_sell64_fp64_bit_exact(ctx, name):
 def test_auto_format_choice(ctx):
     # stencils (few distinct offsets col - row) take DIA; block-diagonal systems of
     # different grid sizes (many offsets) take SELL-64; a matrix with one long row
-    # per slice and rectangular transfers keep row-block CSR
+    # per slice keeps row-block CSR; transfers take VI
     D = ua.DeviceCsr(ctx, ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 16, 16, seed=1))
     assert D.format == ua.FMT_DIA and D.m.ndiag == 7
     assert ua.DeviceCsr(ctx, ua.gen_stencil(ua.STENCIL_2D9, 30, 20, shift=1e-3)).m.ndiag == 9
@@ -104,7 +104,11 @@ def test_auto_format_choice(ctx):
     assert ua.DeviceCsr(ctx, bt.make_batch([9, 13, 17, 21, 25, 29, 33, 37, 41, 45, 49, 53, 57, 61, 65, 69, 73],
                                            levels=1).A).format == ua.FMT_SELL64
     assert ua.DeviceCsr(ctx, MATRICES["rows_up_to_900"]()).format == ua.FMT_ROWBLOCK
-    assert ua.DeviceCsr(ctx, ua.gen_prolongation(16, 16, 16)).format == ua.FMT_ROWBLOCK
+    # trilinear transfers: few distinct values, <= 8 / 27 offsets from each row's first column;
+    # the restriction fills its 27 slots (VI), the interpolation only 3.4 of 8 (row-block CSR)
+    P3 = ua.gen_prolongation(16, 16, 16)
+    assert ua.DeviceCsr(ctx, P3).format == ua.FMT_ROWBLOCK
+    assert ua.DeviceCsr(ctx, ua.Csr.from_scipy(P3.to_scipy().T)).format == ua.FMT_VI
 
 
 def _galerkin(n):
@@ -485,7 +489,7 @@ def test_autotuner_picks_a_plan_and_keeps_results(ctx):
         fmt = ctypes.c_int32()
         ms = ctypes.c_float()
         ua.check(L().uhip_csr_autotune(ctx.h, D.ref, 3, ctypes.byref(fmt), ctypes.byref(ms)))
-        assert fmt.value in (ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA) and fmt.value == D.format
+        assert fmt.value in (ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA, ua.FMT_VI) and fmt.value == D.format
         assert 0 < ms.value < 100
         x = ua.gen_uniform(5, A.shape[1])
         xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, A.shape[0])
@@ -501,7 +505,7 @@ def test_context_autotune_applies_to_automatic_plans(ctx):
     c2.set_autotune(2)
     A = ua.gen_stencil(ua.STENCIL_2D9, 64, 48, shift=1e-3)
     D = ua.DeviceCsr(c2, A)
-    assert D.format in (ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA)
+    assert D.format in (ua.FMT_ROWBLOCK, ua.FMT_SELL64, ua.FMT_DIA, ua.FMT_VI)
     x = ua.gen_uniform(2, A.shape[0])
     xd, yd = ua.DeviceArray.from_numpy(c2, x), ua.DeviceArray(c2, A.shape[0])
     D.spmv(xd, yd)
@@ -541,3 +545,58 @@ def test_spmv_dot2(ctx):
     d = ctx.read_scalars(s, 2)
     assert abs(d[0] - x @ Ax) <= 1e-13 * (np.abs(x) @ np.abs(Ax))
     assert abs(d[1] - z @ x) <= 1e-13 * (np.abs(z) @ np.abs(x))
+
+
+def _agg_interp(n, seed):
+    """A piecewise-constant aggregation transfer (one entry per row, values 1 or 0)."""
+    rng = np.random.default_rng(seed)
+    parent = np.sort(rng.integers(0, n // 3, n)).astype(np.int32)
+    val = np.where(rng.random(n) < 0.05, 0.0, 1.0)
+    return ua.Csr(np.arange(n + 1, dtype=np.int32), parent, val, (n, n // 3))
+
+
+VI_MATRICES = {
+    "P3d": lambda: ua.gen_prolongation(16, 12, 10),
+    "R3d": lambda: ua.Csr.from_scipy(ua.gen_prolongation(16, 12, 10).to_scipy().T),
+    "P2d": lambda: ua.gen_prolongation(34, 20),
+    "agg": lambda: _agg_interp(3001, 4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(VI_MATRICES))
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_spmv_vi_bit_exact(ctx, name, dt):
+    """VI plans (value table + 8-bit codes + row-relative offsets) give the CSR sum's bits
+    for the plain product and the transfer epilogues (y += A x, y = mask .* A x)."""
+    A = VI_MATRICES[name]()
+    m, n = A.shape
+    D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_VI)
+    assert D.format == ua.FMT_VI and 1 <= D.m.vi_nvals <= 256 and 1 <= D.m.ndiag <= 32
+    x = ua.gen_uniform(17, n).astype(dt)
+    ref = orc.spmv_f32(A, x) if dt is np.float32 else orc.spmv(A, x)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, m, dt)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), ref)
+    if dt is np.float64:
+        z = ua.gen_uniform(18, m)
+        yd.upload(z)
+        ua.check(L().uhip_spmv_add(ctx.h, D.ref, P(xd), P(yd)))
+        assert np.array_equal(yd.numpy(), z + ref)
+        mk = (ua.gen_uniform(19, m) > 0).astype(np.float64)
+        mkd = ua.DeviceArray.from_numpy(ctx, mk)
+        ua.check(L().uhip_spmv_masked(ctx.h, D.ref, P(xd), P(mkd), P(yd)))
+        assert np.array_equal(yd.numpy(), mk * ref)
+
+
+def test_vi_rejects_many_values_and_nan(ctx):
+    A = ua.gen_prolongation(40, 30)
+    assert A.nnz > 300
+    val = A.val.copy()
+    val[:300] = np.arange(300) * 0.001 + 2.0  # 300 + distinct values
+    with pytest.raises(ua.HipError):
+        ua.DeviceCsr(ctx, ua.Csr(A.rowptr, A.col, val, A.shape), fmt=ua.FMT_VI)
+    val = A.val.copy()
+    val[5] = np.nan
+    with pytest.raises(ua.HipError):
+        ua.DeviceCsr(ctx, ua.Csr(A.rowptr, A.col, val, A.shape), fmt=ua.FMT_VI)
+    assert ua.DeviceCsr(ctx, ua.Csr(A.rowptr, A.col, val, A.shape)).format != ua.FMT_VI

@@ -32,7 +32,7 @@ def _alive() -> bool:
     return not _EXITING[0]
 
 F64, F32 = 0, 1
-FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64, FMT_DIA = -1, 0, 1, 2
+FMT_AUTO, FMT_ROWBLOCK, FMT_SELL64, FMT_DIA, FMT_VI = -1, 0, 1, 2, 3
 _NP = {F64: np.float64, F32: np.float32}
 _DT = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
@@ -228,7 +228,7 @@ class DeviceCsr:
     """uhip_csr on the device with its row-block plan."""
 
     def __init__(self, ctx: Context, A: Csr, dtype=np.float64, block_nnz: int = 0, fmt: Optional[int] = None):
-        """fmt: None (block_nnz > 0: row-block CSR, 0: automatic), or FMT_AUTO / FMT_ROWBLOCK / FMT_SELL64 / FMT_DIA."""
+        """fmt: None (block_nnz > 0: row-block CSR, 0: automatic), or FMT_AUTO / FMT_ROWBLOCK / FMT_SELL64 / FMT_DIA / FMT_VI."""
         self.ctx = ctx
         self.shape = A.shape
         self._rp = DeviceArray.from_numpy(ctx, A.rowptr.astype(np.int32))

@@ -43,6 +43,9 @@ class uhip_csr(ctypes.Structure):
         ("dia_mask", ctypes.c_void_p),
         ("dia_sym", ctypes.c_int32),
         ("spmm_kernel", ctypes.c_int32),
+        ("vi_base", ctypes.c_void_p),
+        ("vi_table", ctypes.c_void_p),
+        ("vi_nvals", ctypes.c_int32),
     ]
 
 

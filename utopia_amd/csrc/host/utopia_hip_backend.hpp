@@ -377,7 +377,9 @@ public:
         m_.dia_off = nullptr;
         m_.dia_val = nullptr;
         m_.dia_mask = nullptr;
-        m_.nblocks = m_.nslices = m_.nparts = m_.ndiag = m_.dia_sym = m_.spmm_kernel = 0;
+        m_.nblocks = m_.nslices = m_.nparts = m_.ndiag = m_.dia_sym = m_.spmm_kernel = m_.vi_nvals = 0;
+        m_.vi_base = nullptr;
+        m_.vi_table = nullptr;
         m_.format = UHIP_FMT_ROWBLOCK;
         plan(block_nnz);
     }

@@ -121,6 +121,8 @@ __global__ __launch_bounds__(kBlock) void k_dia_init(int32_t *__restrict__ table
     if (threadIdx.x == 0) *flags = 0;
 }
 
+// ROWBASE: offsets are taken from each row's first column instead of its index.
+template <bool ROWBASE>
 __global__ __launch_bounds__(kBlock) void k_dia_scan(int32_t n, const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col, int32_t *__restrict__ gtable,
                                                      int32_t *__restrict__ gflags) {
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_dia_scan(int32_t n, const int32_t *_
             const int32_t c = col[e];
             if (e > a && c <= prev) atomicOr(&bad, 1);
             prev = c;
-            const int32_t key = c - r;
+            const int32_t key = c - (ROWBASE ? col[a] : r);
             const int h = dia_hash(key);
             bool placed = false;
             for (int p = 0; p < kDiaSlots && !placed; ++p) {
@@ -240,8 +242,8 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     int32_t *flags = table + kDiaSlots;
     hipLaunchKernelGGL(k_dia_init, dim3(1), dim3(kBlock), 0, s, table, flags);
     UHIP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_dia_scan, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr, A->col, table,
-                       flags);
+    hipLaunchKernelGGL(k_dia_scan<false>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr, A->col,
+                       table, flags);
     UHIP_CHECK_LAUNCH();
     int32_t h[kDiaSlots + 1];
     UHIP_CHECK_HIP(hipMemcpyAsync(h, table, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -326,6 +328,203 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     return UHIP_OK;
 }
 
+// ---- VI plan (value-indexed, row-relative offsets) ---------------------------
+// Distinct value bit patterns are collected like the offsets: a 512-slot set per
+// workgroup in LDS, merged into one global set. flags bit 2: more than 512 distinct
+// values, bit 3: a NaN value (not indexed).
+constexpr int kViSlots = 512;
+constexpr uint64_t kViEmpty = ~0ull;
+
+__device__ __forceinline__ int vi_hash(uint64_t key) {
+    return (int)(((key ^ (key >> 29)) * 0x9E3779B97F4A7C15ull) >> 55);  // 9 bits
+}
+template <typename T>
+__device__ __forceinline__ uint64_t vi_bits(T v) {
+    if constexpr (sizeof(T) == 8) return (uint64_t)__double_as_longlong((double)v);
+    else return (uint64_t)__float_as_uint((float)v);
+}
+
+__global__ __launch_bounds__(kBlock) void k_vi_init(unsigned long long *__restrict__ table) {
+    for (int i = threadIdx.x; i < kViSlots; i += kBlock) table[i] = kViEmpty;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_vi_scan(int64_t nnz, const T *__restrict__ val,
+                                                    unsigned long long *__restrict__ gtable,
+                                                    int32_t *__restrict__ gflags) {
+    __shared__ unsigned long long t[kViSlots];
+    __shared__ int32_t bad;
+    for (int i = threadIdx.x; i < kViSlots; i += kBlock) t[i] = kViEmpty;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kBlock) {
+        const T v = val[e];
+        if (v != v) {
+            atomicOr(&bad, 8);
+            continue;
+        }
+        const unsigned long long key = vi_bits(v);
+        const int h = vi_hash(key);
+        bool placed = false;
+        for (int p = 0; p < kViSlots && !placed; ++p) {
+            const int slot = (h + p) & (kViSlots - 1);
+            const unsigned long long cur = t[slot];
+            if (cur == key) {
+                placed = true;
+            } else if (cur == kViEmpty) {
+                const unsigned long long old = atomicCAS(&t[slot], kViEmpty, key);
+                placed = (old == kViEmpty || old == key);
+            }
+        }
+        if (!placed) atomicOr(&bad, 4);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kViSlots; i += kBlock) {
+        const unsigned long long key = t[i];
+        if (key == kViEmpty) continue;
+        const int h = vi_hash(key);
+        bool placed = false;
+        for (int p = 0; p < kViSlots && !placed; ++p) {
+            const int slot = (h + p) & (kViSlots - 1);
+            const unsigned long long cur =
+                __hip_atomic_load(&gtable[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key) {
+                placed = true;
+            } else if (cur == kViEmpty) {
+                const unsigned long long old = atomicCAS(&gtable[slot], kViEmpty, key);
+                placed = (old == kViEmpty || old == key);
+            }
+        }
+        if (!placed) atomicOr(gflags, 4);
+    }
+    if (threadIdx.x == 0 && bad) atomicOr(gflags, bad);
+}
+
+// Row r: base = its first column; every slot d gets the code of the entry at column
+// base + off[d] (0 where absent) and bit d of the mask says whether it is present.
+template <typename T, typename TM>
+__global__ __launch_bounds__(kBlock) void k_vi_fill(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
+                                                    int32_t nvals, const unsigned long long *__restrict__ sorted_bits,
+                                                    const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                    const T *__restrict__ val, int32_t *__restrict__ base,
+                                                    uint8_t *__restrict__ codes, TM *__restrict__ mask) {
+    __shared__ int32_t so[32];
+    __shared__ unsigned long long sv[256];
+    if (threadIdx.x < ndiag) so[threadIdx.x] = off[threadIdx.x];
+    for (int i = threadIdx.x; i < nvals; i += kBlock) sv[i] = sorted_bits[i];
+    __syncthreads();
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    int32_t e = rowptr[r];
+    const int32_t e1 = rowptr[r + 1];
+    const int32_t b = e < e1 ? col[e] : 0;
+    uint32_t m = 0;
+    const int64_t ld = dia_ld(n);
+    for (int d = 0; d < ndiag; ++d) {
+        uint8_t code = 0;
+        if (e < e1 && col[e] - b == so[d]) {
+            const unsigned long long key = vi_bits(val[e]);
+            int lo = 0, hi = nvals - 1;  // the key is in the table (collected by k_vi_scan)
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sv[mid] < key) lo = mid + 1;
+                else hi = mid;
+            }
+            code = (uint8_t)lo;
+            m |= 1u << d;
+            ++e;
+        }
+        codes[(int64_t)d * ld + r] = code;
+    }
+    base[r] = b;
+    mask[r] = (TM)m;
+}
+
+template <typename T>
+__global__ void k_vi_table(int32_t nvals, const unsigned long long *__restrict__ sorted_bits, T *__restrict__ table) {
+    const int i = threadIdx.x;
+    if (i >= nvals) return;
+    if constexpr (sizeof(T) == 8) table[i] = (T)__longlong_as_double((long long)sorted_bits[i]);
+    else table[i] = (T)__uint_as_float((unsigned)sorted_bits[i]);
+}
+
+// Plan VI arrays; UHIP_ERR_UNSUPPORTED (nothing kept) when a row is unsorted, the
+// rows reach more than 32 distinct offsets from their first column, the values
+// more than 256 distinct bit patterns (or any NaN), or ndiag * n > max_ratio * nnz.
+static int plan_vi(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
+    hipStream_t s = ctx->stream;
+    const int32_t n = A->nrows;
+    if (n == 0 || A->nnz == 0) return UHIP_ERR_UNSUPPORTED;
+    int32_t *table = nullptr;
+    unsigned long long *vt = nullptr;
+    UHIP_CHECK_HIP(hipMalloc(&table, sizeof(int32_t) * (kDiaSlots + 1)));
+    UHIP_CHECK_HIP(hipMalloc(&vt, sizeof(unsigned long long) * kViSlots));
+    int32_t *flags = table + kDiaSlots;
+    hipLaunchKernelGGL(k_dia_init, dim3(1), dim3(kBlock), 0, s, table, flags);
+    hipLaunchKernelGGL(k_vi_init, dim3(1), dim3(kBlock), 0, s, vt);
+    hipLaunchKernelGGL(k_dia_scan<true>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, A->rowptr, A->col,
+                       table, flags);
+    const dim3 gv(grid_for(A->nnz, 8, 4096));
+    if (A->dtype == UHIP_F64)
+        hipLaunchKernelGGL(k_vi_scan<double>, gv, dim3(kBlock), 0, s, A->nnz, (const double *)A->val, vt, flags);
+    else
+        hipLaunchKernelGGL(k_vi_scan<float>, gv, dim3(kBlock), 0, s, A->nnz, (const float *)A->val, vt, flags);
+    UHIP_CHECK_LAUNCH();
+    int32_t h[kDiaSlots + 1];
+    std::vector<unsigned long long> hv(kViSlots);
+    UHIP_CHECK_HIP(hipMemcpyAsync(h, table, sizeof(h), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipMemcpyAsync(hv.data(), vt, sizeof(unsigned long long) * kViSlots, hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<int32_t> offs;
+    for (int i = 0; i < kDiaSlots; ++i)
+        if (h[i] != kDiaEmpty) offs.push_back(h[i]);
+    std::sort(offs.begin(), offs.end());
+    std::vector<unsigned long long> bits;
+    for (unsigned long long b : hv)
+        if (b != kViEmpty) bits.push_back(b);
+    std::sort(bits.begin(), bits.end());
+    const int32_t nd = (int32_t)offs.size(), nv = (int32_t)bits.size();
+    if (h[kDiaSlots] != 0 || nd < 1 || nd > 32 || nv < 1 || nv > 256 ||
+        (double)nd * (double)n > max_ratio * (double)A->nnz) {
+        hipFree(table);
+        hipFree(vt);
+        return UHIP_ERR_UNSUPPORTED;
+    }
+    const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
+    const size_t mb = nd <= 8 ? 1 : (nd <= 16 ? 2 : 4);
+    UHIP_CHECK_HIP(hipMemcpyAsync(table, offs.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
+    UHIP_CHECK_HIP(hipMemcpyAsync(vt, bits.data(), sizeof(unsigned long long) * (size_t)nv, hipMemcpyHostToDevice, s));
+    UHIP_CHECK_HIP(hipMalloc(&A->dia_val, (size_t)nd * (size_t)dia_ld(n)));
+    UHIP_CHECK_HIP(hipMalloc(&A->dia_mask, mb * (size_t)n));
+    UHIP_CHECK_HIP(hipMalloc(&A->vi_base, sizeof(int32_t) * (size_t)n));
+    UHIP_CHECK_HIP(hipMalloc(&A->vi_table, vb * 256));
+    const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+#define UHIP_VI_FILL(T, TM)                                                                                      \
+    hipLaunchKernelGGL((k_vi_fill<T, TM>), g, b, 0, s, n, nd, table, nv, vt, A->rowptr, A->col, (const T *)A->val, \
+                       A->vi_base, (uint8_t *)A->dia_val, (TM *)A->dia_mask)
+    if (A->dtype == UHIP_F64) {
+        if (mb == 1) UHIP_VI_FILL(double, uint8_t);
+        else if (mb == 2) UHIP_VI_FILL(double, uint16_t);
+        else UHIP_VI_FILL(double, uint32_t);
+        hipLaunchKernelGGL(k_vi_table<double>, dim3(1), dim3(256), 0, s, nv, vt, (double *)A->vi_table);
+    } else {
+        if (mb == 1) UHIP_VI_FILL(float, uint8_t);
+        else if (mb == 2) UHIP_VI_FILL(float, uint16_t);
+        else UHIP_VI_FILL(float, uint32_t);
+        hipLaunchKernelGGL(k_vi_table<float>, dim3(1), dim3(256), 0, s, nv, vt, (float *)A->vi_table);
+    }
+#undef UHIP_VI_FILL
+    UHIP_CHECK_LAUNCH();
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    hipFree(vt);
+    A->dia_off = table;  // first nd slots hold the sorted offsets
+    A->ndiag = nd;
+    A->vi_nvals = nv;
+    A->format = UHIP_FMT_VI;
+    A->nparts = (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
+    return UHIP_OK;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_diag(int32_t n, const int32_t *__restrict__ rowptr,
                                                  const int32_t *__restrict__ col, const T *__restrict__ val,
@@ -399,6 +598,7 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
         int32_t format, block_nnz;
     };
     const Cand cands[] = {{UHIP_FMT_DIA, 2048},
+                          {UHIP_FMT_VI, 2048},
                           {UHIP_FMT_SELL64, 2048},
                           {UHIP_FMT_ROWBLOCK, 1024},
                           {UHIP_FMT_ROWBLOCK, 2048},
@@ -474,7 +674,7 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
     UHIP_REQUIRE(ctx && A && A->rowptr, "uhip_csr_plan: null argument");
     UHIP_REQUIRE(A->nrows >= 0 && A->ncols >= 0, "uhip_csr_plan: bad shape");
     UHIP_REQUIRE(format == UHIP_FMT_AUTO || format == UHIP_FMT_ROWBLOCK || format == UHIP_FMT_SELL64 ||
-                     format == UHIP_FMT_DIA,
+                     format == UHIP_FMT_DIA || format == UHIP_FMT_VI,
                  "uhip_csr_plan: format %d", format);
     if (format == UHIP_FMT_AUTO && ctx->autotune_reps > 0 && A->nrows > 0 && A->nnz > 0)
         return autotune_plan(ctx, A, ctx->autotune_reps, nullptr);
@@ -497,7 +697,16 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
             return set_error(UHIP_ERR_UNSUPPORTED,
                              "uhip_csr_plan: DIA needs a square matrix with sorted rows and <= 32 distinct offsets");
     }
-    if (n > 0 && A->nnz > 0 && format != UHIP_FMT_ROWBLOCK) {
+    if (format == UHIP_FMT_VI || (format == UHIP_FMT_AUTO && n > 0 && A->nnz > 0)) {
+        // the VI kernel gathers x for every slot: automatic only with <= 1.5 slots per entry
+        const int st = plan_vi(ctx, A, format == UHIP_FMT_VI ? 1e30 : 1.5);
+        if (st == UHIP_OK) return UHIP_OK;
+        if (st != UHIP_ERR_UNSUPPORTED) return st;
+        if (format == UHIP_FMT_VI)
+            return set_error(UHIP_ERR_UNSUPPORTED, "uhip_csr_plan: VI needs sorted rows, <= 32 offsets from each "
+                                                   "row's first column and <= 256 distinct values");
+    }
+    if (n > 0 && A->nnz > 0 && format != UHIP_FMT_ROWBLOCK && format != UHIP_FMT_VI) {
         const int st = plan_sell(ctx, A, format == UHIP_FMT_SELL64 ? 1e30 : 1.25);
         if (st == UHIP_OK) return UHIP_OK;
         if (st != UHIP_ERR_UNSUPPORTED) return st;
@@ -542,12 +751,18 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
 
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     UHIP_REQUIRE(A, "uhip_csr_plan_destroy: null matrix");
-    if (A->blocks || A->slice_off || A->sell_col || A->sell_val || A->dia_off || A->dia_val || A->dia_mask) {
+    if (A->blocks || A->slice_off || A->sell_col || A->sell_val || A->dia_off || A->dia_val || A->dia_mask ||
+        A->vi_base || A->vi_table) {
         if (ctx) UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (A->dia_off) UHIP_CHECK_HIP(hipFree(A->dia_off));
     if (A->dia_val) UHIP_CHECK_HIP(hipFree(A->dia_val));
     if (A->dia_mask) UHIP_CHECK_HIP(hipFree(A->dia_mask));
+    if (A->vi_base) UHIP_CHECK_HIP(hipFree(A->vi_base));
+    if (A->vi_table) UHIP_CHECK_HIP(hipFree(A->vi_table));
+    A->vi_base = nullptr;
+    A->vi_table = nullptr;
+    A->vi_nvals = 0;
     A->dia_off = nullptr;
     A->dia_val = nullptr;
     A->dia_mask = nullptr;

@@ -334,11 +334,104 @@ inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2) {
     return (int32_t)((waves + kDiaWG / 64 - 1) / (kDiaWG / 64));
 }
 
+// VI (value-indexed, row-relative): slot d of row r holds the entry at column
+// base[r] + off[d] (bit d of mask[r]) with value table[codes[d * ld + r]]; the
+// table (at most 256 values) is staged in LDS. Lane mapping, load batching and the
+// masked, column-ordered sum are those of the DIA kernel, so each row sum equals
+// the sequential CSR sum bit for bit.
+template <typename TAcc, class Epi, typename TA, typename TX, typename TM>
+__global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
+                                                    const uint8_t *__restrict__ codes, const TA *__restrict__ table,
+                                                    int32_t nvals, const int32_t *__restrict__ vbase,
+                                                    const TM *__restrict__ mask, const TX *__restrict__ x, Epi epi,
+                                                    double *__restrict__ partials, int64_t stride) {
+    constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
+    constexpr int R = DiaRows<TA>::R;
+    __shared__ TA tab[256];
+    for (int i = threadIdx.x; i < nvals; i += kDiaWG) tab[i] = table[i];
+    __syncthreads();
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int32_t base = (b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6)) * (64 * R);
+    const int64_t ld = dia_ld(nrows);
+    double part[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) part[d] = 0.0;
+    if (base < nrows) {
+        int32_t row[R], rb[R];
+        uint32_t m[R];
+        TAcc acc[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            row[i] = base + 64 * i + lane;
+            const bool in = row[i] < nrows;
+            m[i] = in ? (uint32_t)mask[row[i]] : 0u;
+            rb[i] = in ? vbase[row[i]] : 0;
+            acc[i] = TAcc(0);
+        }
+        constexpr int G = 8;
+        for (int d = 0; d < ndiag; d += G) {
+            TA vv[G][R];
+            TX xv[G][R];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (d + u < ndiag) {
+                    const uint8_t *cd = codes + (int64_t)(d + u) * ld;
+                    const int32_t o = off[d + u];
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        vv[u][i] = tab[ld_stream(cd + min(row[i], (int32_t)ld - 1))];
+                        xv[u][i] = x[((m[i] >> (d + u)) & 1u) ? rb[i] + o : 0];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (d + u < ndiag) {
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        const TAcc p = (TAcc)vv[u][i] * (TAcc)xv[u][i];
+                        acc[i] = ((m[i] >> (d + u)) & 1u) ? acc[i] + p : acc[i];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (row[i] < nrows) epi(row[i], acc[i], part);
+    }
+    if constexpr (Epi::kDots > 0) {
+#pragma unroll
+        for (int d = 0; d < Epi::kDots; ++d) {
+            const double t = block_sum<kDiaWG>(part[d]);
+            if (threadIdx.x == 0) partials[d * stride + b] = t;
+        }
+    }
+}
+
 // Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
 // Matrix values have the accumulation type TAcc.
 template <typename TAcc, typename TX, class Epi>
 int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi) {
     using TA = TAcc;
+    if (A && A->format == UHIP_FMT_VI) {
+        UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->vi_base && A->vi_table && A->ndiag >= 1 &&
+                         A->ndiag <= 32 && A->vi_nvals >= 1 && A->vi_nvals <= 256,
+                     "spmv: VI plan missing");
+        if (A->nrows == 0) return UHIP_OK;
+        if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
+        const dim3 g(A->nparts), blk(kDiaWG);
+#define UHIP_VI_LAUNCH(TM)                                                                                          \
+    hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, A->dia_off, \
+                       (const uint8_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals, A->vi_base,            \
+                       (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
+        if (A->ndiag <= 8) UHIP_VI_LAUNCH(uint8_t);
+        else if (A->ndiag <= 16) UHIP_VI_LAUNCH(uint16_t);
+        else UHIP_VI_LAUNCH(uint32_t);
+#undef UHIP_VI_LAUNCH
+        UHIP_CHECK_LAUNCH();
+        return UHIP_OK;
+    }
     if (A && A->format == UHIP_FMT_DIA) {
         UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->ndiag >= 1 && A->ndiag <= 32,
                      "spmv: DIA plan missing");
