@@ -212,14 +212,16 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
  * offsets from their own first column and whose values take at most 256 distinct
  * bit patterns (geometric and aggregation transfer operators) as a per-row base
  * column, a per-row slot mask and one 8-bit value code per slot (the value index
- * of CSR-VI, Kourtis et al. 2008), the distinct values in a table.
+ * of CSR-VI, Kourtis et al. 2008), the distinct values in a table. dia_val then
+ * holds the codes row by row: 1, 2 or 4 64-bit words per row (ndiag <= 8, 16,
+ * 32), the code of slot d in byte d % 8 of word d / 8.
  * UHIP_FMT_AUTO takes DIA when it applies with at most 25% padding, else VI when
- * it applies with at most 1.5 slots per stored entry, else SELL-64 when its padding
+ * it applies with at most 3 slots per stored entry, else SELL-64 when its padding
  * adds at most 25% to nnz, else the row-block CSR kernel. Each y_i is the sum of exactly the row's CSR entries in column
  * order in every format. */
 int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz);
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A);
-/* Autotuner: times y = A x for every applicable plan (DIA, SELL-64, row-block CSR
+/* Autotuner: times y = A x for every applicable plan (DIA, VI, SELL-64, row-block CSR
  * with 1024 / 2048 / 4096 nonzeros per workgroup), reps launches each, and keeps
  * the fastest (format and best average ms returned). No reference analogue
  * (PETSc's MatMult has one fixed AIJ kernel). */

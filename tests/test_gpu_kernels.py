@@ -104,11 +104,16 @@ def test_auto_format_choice(ctx):
     assert ua.DeviceCsr(ctx, bt.make_batch([9, 13, 17, 21, 25, 29, 33, 37, 41, 45, 49, 53, 57, 61, 65, 69, 73],
                                            levels=1).A).format == ua.FMT_SELL64
     assert ua.DeviceCsr(ctx, MATRICES["rows_up_to_900"]()).format == ua.FMT_ROWBLOCK
-    # trilinear transfers: few distinct values, <= 8 / 27 offsets from each row's first column;
-    # the restriction fills its 27 slots (VI), the interpolation only 3.4 of 8 (row-block CSR)
+    # trilinear transfers: few distinct values, <= 8 / 27 offsets from each row's first
+    # column: the restriction fills its 27 slots, the interpolation 3 of 8 (both VI);
+    # two entries per row spread over 9 offsets (4.5 slots per entry) are not VI
     P3 = ua.gen_prolongation(16, 16, 16)
-    assert ua.DeviceCsr(ctx, P3).format == ua.FMT_ROWBLOCK
+    assert ua.DeviceCsr(ctx, P3).format == ua.FMT_VI
     assert ua.DeviceCsr(ctx, ua.Csr.from_scipy(P3.to_scipy().T)).format == ua.FMT_VI
+    r = np.arange(200)
+    S = sp.csr_matrix((np.tile([1.0, 2.0], 200), np.stack([r, r + 1 + r % 8], 1).ravel(), 2 * np.arange(201)),
+                      shape=(200, 300))
+    assert ua.DeviceCsr(ctx, ua.Csr.from_scipy(S)).format != ua.FMT_VI
 
 
 def _galerkin(n):

@@ -353,11 +353,14 @@ __global__ __launch_bounds__(kBlock) void k_vi_scan(int64_t nnz, const T *__rest
                                                     unsigned long long *__restrict__ gtable,
                                                     int32_t *__restrict__ gflags) {
     __shared__ unsigned long long t[kViSlots];
-    __shared__ int32_t bad;
+    __shared__ int32_t bad, count;
     for (int i = threadIdx.x; i < kViSlots; i += kBlock) t[i] = kViEmpty;
-    if (threadIdx.x == 0) bad = 0;
+    if (threadIdx.x == 0) bad = count = 0;
     __syncthreads();
+    // The block stops at its 257th distinct value (the table holds at most 256), so a
+    // matrix with many values costs a few probes per entry, not a walk of the table.
     for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * kBlock) {
+        if (__hip_atomic_load(&bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 4) break;
         const T v = val[e];
         if (v != v) {
             atomicOr(&bad, 8);
@@ -373,12 +376,17 @@ __global__ __launch_bounds__(kBlock) void k_vi_scan(int64_t nnz, const T *__rest
                 placed = true;
             } else if (cur == kViEmpty) {
                 const unsigned long long old = atomicCAS(&t[slot], kViEmpty, key);
+                if (old == kViEmpty && atomicAdd(&count, 1) >= 256) atomicOr(&bad, 4);
                 placed = (old == kViEmpty || old == key);
             }
         }
         if (!placed) atomicOr(&bad, 4);
     }
     __syncthreads();
+    if (bad & 4) {
+        if (threadIdx.x == 0) atomicOr(gflags, bad);
+        return;
+    }
     for (int i = threadIdx.x; i < kViSlots; i += kBlock) {
         const unsigned long long key = t[i];
         if (key == kViEmpty) continue;
@@ -400,14 +408,16 @@ __global__ __launch_bounds__(kBlock) void k_vi_scan(int64_t nnz, const T *__rest
     if (threadIdx.x == 0 && bad) atomicOr(gflags, bad);
 }
 
-// Row r: base = its first column; every slot d gets the code of the entry at column
-// base + off[d] (0 where absent) and bit d of the mask says whether it is present.
+// Row r: base = its first column; slot d gets the code of the entry at column
+// base + off[d] (0 where absent) in byte d % 8 of word r * NW + d / 8 (NW =
+// sizeof(TM)), and bit d of the mask says whether it is present.
 template <typename T, typename TM>
 __global__ __launch_bounds__(kBlock) void k_vi_fill(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
                                                     int32_t nvals, const unsigned long long *__restrict__ sorted_bits,
                                                     const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                                                     const T *__restrict__ val, int32_t *__restrict__ base,
-                                                    uint8_t *__restrict__ codes, TM *__restrict__ mask) {
+                                                    uint64_t *__restrict__ codes, TM *__restrict__ mask) {
+    constexpr int NW = (int)sizeof(TM);
     __shared__ int32_t so[32];
     __shared__ unsigned long long sv[256];
     if (threadIdx.x < ndiag) so[threadIdx.x] = off[threadIdx.x];
@@ -419,10 +429,12 @@ __global__ __launch_bounds__(kBlock) void k_vi_fill(int32_t n, int32_t ndiag, co
     const int32_t e1 = rowptr[r + 1];
     const int32_t b = e < e1 ? col[e] : 0;
     uint32_t m = 0;
-    const int64_t ld = dia_ld(n);
-    for (int d = 0; d < ndiag; ++d) {
-        uint8_t code = 0;
-        if (e < e1 && col[e] - b == so[d]) {
+    uint64_t word[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) word[w] = 0;
+#pragma unroll
+    for (int d = 0; d < 8 * NW; ++d) {
+        if (d < ndiag && e < e1 && col[e] - b == so[d]) {
             const unsigned long long key = vi_bits(val[e]);
             int lo = 0, hi = nvals - 1;  // the key is in the table (collected by k_vi_scan)
             while (lo < hi) {
@@ -430,12 +442,13 @@ __global__ __launch_bounds__(kBlock) void k_vi_fill(int32_t n, int32_t ndiag, co
                 if (sv[mid] < key) lo = mid + 1;
                 else hi = mid;
             }
-            code = (uint8_t)lo;
+            word[d / 8] |= (uint64_t)lo << (8 * (d % 8));
             m |= 1u << d;
             ++e;
         }
-        codes[(int64_t)d * ld + r] = code;
     }
+#pragma unroll
+    for (int w = 0; w < NW; ++w) codes[(int64_t)r * NW + w] = word[w];
     base[r] = b;
     mask[r] = (TM)m;
 }
@@ -494,14 +507,14 @@ static int plan_vi(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     const size_t mb = nd <= 8 ? 1 : (nd <= 16 ? 2 : 4);
     UHIP_CHECK_HIP(hipMemcpyAsync(table, offs.data(), sizeof(int32_t) * (size_t)nd, hipMemcpyHostToDevice, s));
     UHIP_CHECK_HIP(hipMemcpyAsync(vt, bits.data(), sizeof(unsigned long long) * (size_t)nv, hipMemcpyHostToDevice, s));
-    UHIP_CHECK_HIP(hipMalloc(&A->dia_val, (size_t)nd * (size_t)dia_ld(n)));
+    UHIP_CHECK_HIP(hipMalloc(&A->dia_val, sizeof(uint64_t) * mb * (size_t)n));
     UHIP_CHECK_HIP(hipMalloc(&A->dia_mask, mb * (size_t)n));
     UHIP_CHECK_HIP(hipMalloc(&A->vi_base, sizeof(int32_t) * (size_t)n));
     UHIP_CHECK_HIP(hipMalloc(&A->vi_table, vb * 256));
     const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
 #define UHIP_VI_FILL(T, TM)                                                                                      \
     hipLaunchKernelGGL((k_vi_fill<T, TM>), g, b, 0, s, n, nd, table, nv, vt, A->rowptr, A->col, (const T *)A->val, \
-                       A->vi_base, (uint8_t *)A->dia_val, (TM *)A->dia_mask)
+                       A->vi_base, (uint64_t *)A->dia_val, (TM *)A->dia_mask)
     if (A->dtype == UHIP_F64) {
         if (mb == 1) UHIP_VI_FILL(double, uint8_t);
         else if (mb == 2) UHIP_VI_FILL(double, uint16_t);
@@ -698,8 +711,8 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
                              "uhip_csr_plan: DIA needs a square matrix with sorted rows and <= 32 distinct offsets");
     }
     if (format == UHIP_FMT_VI || (format == UHIP_FMT_AUTO && n > 0 && A->nnz > 0)) {
-        // the VI kernel gathers x for every slot: automatic only with <= 1.5 slots per entry
-        const int st = plan_vi(ctx, A, format == UHIP_FMT_VI ? 1e30 : 1.5);
+        // automatic with <= 3 slots per stored entry (a trilinear interpolation has 2.4-2.7)
+        const int st = plan_vi(ctx, A, format == UHIP_FMT_VI ? 1e30 : 3.0);
         if (st == UHIP_OK) return UHIP_OK;
         if (st != UHIP_ERR_UNSUPPORTED) return st;
         if (format == UHIP_FMT_VI)

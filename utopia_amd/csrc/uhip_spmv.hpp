@@ -335,25 +335,26 @@ inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2) {
 }
 
 // VI (value-indexed, row-relative): slot d of row r holds the entry at column
-// base[r] + off[d] (bit d of mask[r]) with value table[codes[d * ld + r]]; the
-// table (at most 256 values) is staged in LDS. Lane mapping, load batching and the
-// masked, column-ordered sum are those of the DIA kernel, so each row sum equals
-// the sequential CSR sum bit for bit.
+// base[r] + off[d] (bit d of mask[r]) with value table[code], the code being byte
+// d % 8 of the 64-bit word codes[r * NW + d / 8]; the table (at most 256 values)
+// is staged in LDS. A row's codes arrive in NW wide loads, a slot that no row of
+// the wavefront holds costs nothing (wave-uniform branch), and the masked,
+// column-ordered sum makes each row sum the sequential CSR sum bit for bit.
 template <typename TAcc, class Epi, typename TA, typename TX, typename TM>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
-                                                    const uint8_t *__restrict__ codes, const TA *__restrict__ table,
+                                                    const uint64_t *__restrict__ codes, const TA *__restrict__ table,
                                                     int32_t nvals, const int32_t *__restrict__ vbase,
                                                     const TM *__restrict__ mask, const TX *__restrict__ x, Epi epi,
                                                     double *__restrict__ partials, int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
     constexpr int R = DiaRows<TA>::R;
+    constexpr int NW = (int)sizeof(TM);  // 64-bit code words per row: 8 slots each
     __shared__ TA tab[256];
     for (int i = threadIdx.x; i < nvals; i += kDiaWG) tab[i] = table[i];
     __syncthreads();
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
     const int32_t base = (b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6)) * (64 * R);
-    const int64_t ld = dia_ld(nrows);
     double part[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) part[d] = 0.0;
@@ -361,6 +362,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         int32_t row[R], rb[R];
         uint32_t m[R];
         TAcc acc[R];
+        uint32_t wm = 0;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             row[i] = base + 64 * i + lane;
@@ -368,31 +370,37 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
             m[i] = in ? (uint32_t)mask[row[i]] : 0u;
             rb[i] = in ? vbase[row[i]] : 0;
             acc[i] = TAcc(0);
+            wm |= m[i];
         }
-        constexpr int G = 8;
-        for (int d = 0; d < ndiag; d += G) {
-            TA vv[G][R];
-            TX xv[G][R];
 #pragma unroll
-            for (int u = 0; u < G; ++u) {
-                if (d + u < ndiag) {
-                    const uint8_t *cd = codes + (int64_t)(d + u) * ld;
-                    const int32_t o = off[d + u];
+        for (int sh = 1; sh < 64; sh <<= 1) wm |= (uint32_t)__shfl_xor((int)wm, sh);
+        wm = __builtin_amdgcn_readfirstlane(wm);  // slots held by some row of the wavefront
 #pragma unroll
-                    for (int i = 0; i < R; ++i) {
-                        vv[u][i] = tab[ld_stream(cd + min(row[i], (int32_t)ld - 1))];
-                        xv[u][i] = x[((m[i] >> (d + u)) & 1u) ? rb[i] + o : 0];
-                    }
+        for (int w = 0; w < NW; ++w) {
+            if (8 * w >= ndiag) break;
+            uint64_t c[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) c[i] = row[i] < nrows ? ld_stream(codes + (int64_t)row[i] * NW + w) : 0ull;
+            TX xv[8][R];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int d = 8 * w + u;
+                if (d < ndiag && ((wm >> d) & 1u)) {
+                    const int32_t o = off[d];
+#pragma unroll
+                    for (int i = 0; i < R; ++i) xv[u][i] = ((m[i] >> d) & 1u) ? x[rb[i] + o] : TX(0);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < R; ++i) xv[u][i] = TX(0);
                 }
             }
 #pragma unroll
-            for (int u = 0; u < G; ++u) {
-                if (d + u < ndiag) {
+            for (int u = 0; u < 8; ++u) {
+                const int d = 8 * w + u;
 #pragma unroll
-                    for (int i = 0; i < R; ++i) {
-                        const TAcc p = (TAcc)vv[u][i] * (TAcc)xv[u][i];
-                        acc[i] = ((m[i] >> (d + u)) & 1u) ? acc[i] + p : acc[i];
-                    }
+                for (int i = 0; i < R; ++i) {
+                    const TAcc p = (TAcc)tab[(uint32_t)(c[i] >> (8 * u)) & 0xffu] * (TAcc)xv[u][i];
+                    acc[i] = ((m[i] >> d) & 1u) ? acc[i] + p : acc[i];
                 }
             }
         }
@@ -423,7 +431,7 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         const dim3 g(A->nparts), blk(kDiaWG);
 #define UHIP_VI_LAUNCH(TM)                                                                                          \
     hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, A->dia_off, \
-                       (const uint8_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals, A->vi_base,            \
+                       (const uint64_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals, A->vi_base,           \
                        (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
         if (A->ndiag <= 8) UHIP_VI_LAUNCH(uint8_t);
         else if (A->ndiag <= 16) UHIP_VI_LAUNCH(uint16_t);
