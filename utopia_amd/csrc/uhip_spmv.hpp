@@ -195,9 +195,14 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             row[i] = base + 64 * i + lane;
-            m[i] = (row[i] < nrows) ? (uint32_t)mask[row[i]] : 0u;
+            m[i] = (uint32_t)mask[min(row[i], nrows - 1)];
             acc[i] = TAcc(0);
         }
+#pragma unroll
+        for (int i = 0; i < R; ++i) m[i] = row[i] < nrows ? m[i] : 0u;
+        // x is read at the clamped column whether or not the entry exists (the mask
+        // only selects in the sum), so no load waits for the mask
+        const int32_t xlast = nrows - 1;
         constexpr int G = DiaRows<TA>::G;
         const int32_t k = ndiag >> 1;
         for (int d = 0; d < ndiag; d += G) {
@@ -213,14 +218,14 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
 #pragma unroll
                         for (int i = 0; i < R; ++i) {
                             vv[u][i] = vd[min(max(row[i] + sh, 0), (int32_t)ld - 1)];
-                            xv[u][i] = x[((m[i] >> dd) & 1u) ? row[i] + o : 0];
+                            xv[u][i] = x[min(max(row[i] + o, 0), xlast)];
                         }
                     } else {
                         const TA *vd = val + (int64_t)dd * ld;
 #pragma unroll
                         for (int i = 0; i < R; ++i) {
                             vv[u][i] = ld_stream(vd + min(row[i], (int32_t)ld - 1));
-                            xv[u][i] = x[((m[i] >> dd) & 1u) ? row[i] + o : 0];
+                            xv[u][i] = x[min(max(row[i] + o, 0), xlast)];
                         }
                     }
                 }
@@ -283,25 +288,31 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
         uint32_t m[P];
         TA a0[P], a1[P], aS1[P], aS2[P], mS1[P];
         TX xc[P], xm1[P], xp1[P], xmS1[P], xpS1[P];
+        // x is read at clamped rows whether or not the entry exists (the mask only
+        // selects in the sum), so no load waits for the mask
+        const int32_t xlast = nrows - 1;
+        auto cl = [xlast](int32_t v) { return min(max(v, 0), xlast); };
 #pragma unroll
         for (int z = 0; z < P; ++z) {
             r[z] = (int32_t)(slab0 + (int64_t)z * S2) + seg * 64 + lane;
             const int32_t rc = min(r[z], last);
-            m[z] = r[z] < nrows ? (uint32_t)mask[r[z]] : 0u;
+            m[z] = (uint32_t)mask[min(r[z], xlast)];
             a0[z] = ld_stream(v0 + rc);
             a1[z] = ld_stream(v1 + rc);
             aS1[z] = ld_stream(vS1 + rc);
             aS2[z] = ld_stream(vS2 + rc);
             mS1[z] = ld_stream(vS1 + max(rc - S1, 0));
-            xc[z] = x[min(r[z], nrows - 1)];
-            xm1[z] = x[((m[z] >> 2) & 1u) ? r[z] - 1 : 0];
-            xp1[z] = x[((m[z] >> 4) & 1u) ? r[z] + 1 : 0];
-            xmS1[z] = x[((m[z] >> 1) & 1u) ? r[z] - S1 : 0];
-            xpS1[z] = x[((m[z] >> 5) & 1u) ? r[z] + S1 : 0];
+            xc[z] = x[cl(r[z])];
+            xm1[z] = x[cl(r[z] - 1)];
+            xp1[z] = x[cl(r[z] + 1)];
+            xmS1[z] = x[cl(r[z] - S1)];
+            xpS1[z] = x[cl(r[z] + S1)];
         }
+#pragma unroll
+        for (int z = 0; z < P; ++z) m[z] = r[z] < nrows ? m[z] : 0u;
         const TA mS2_0 = ld_stream(vS2 + max(min(r[0], last) - S2, 0));
-        const TX xmS2_0 = x[(m[0] & 1u) ? r[0] - S2 : 0];
-        const TX xpS2_P = x[((m[P - 1] >> 6) & 1u) ? r[P - 1] + S2 : 0];
+        const TX xmS2_0 = x[cl(r[0] - S2)];
+        const TX xpS2_P = x[cl(r[P - 1] + S2)];
 #pragma unroll
         for (int z = 0; z < P; ++z) {
             const TA up = __shfl_up(a1[z], 1, 64);
@@ -362,14 +373,20 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         int32_t row[R], rb[R];
         uint32_t m[R];
         TAcc acc[R];
+        // Loads from a clamped row (no branch per row, so they are all in flight
+        // together); rows past the end then get an empty mask.
         uint32_t wm = 0;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             row[i] = base + 64 * i + lane;
-            const bool in = row[i] < nrows;
-            m[i] = in ? (uint32_t)mask[row[i]] : 0u;
-            rb[i] = in ? vbase[row[i]] : 0;
+            const int32_t rc = min(row[i], nrows - 1);
+            m[i] = (uint32_t)mask[rc];
+            rb[i] = vbase[rc];
             acc[i] = TAcc(0);
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            m[i] = row[i] < nrows ? m[i] : 0u;
             wm |= m[i];
         }
 #pragma unroll
@@ -380,7 +397,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
             if (8 * w >= ndiag) break;
             uint64_t c[R];
 #pragma unroll
-            for (int i = 0; i < R; ++i) c[i] = row[i] < nrows ? ld_stream(codes + (int64_t)row[i] * NW + w) : 0ull;
+            for (int i = 0; i < R; ++i) c[i] = ld_stream(codes + (int64_t)min(row[i], nrows - 1) * NW + w);
             TX xv[8][R];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
