@@ -231,7 +231,8 @@ def test_fused_epilogues_both_formats_with_dot(ctx, fmt, dims):
     D = ua.DeviceCsr(ctx, A, fmt=fmt)
     D32 = ua.DeviceCsr(ctx, A, dtype=np.float32, fmt=fmt)
     if dims == (16, 16, 13):
-        assert D32.m.dia_sym == 2  # plane-blocked 7-point kernel, with a partial last slab
+        # plane-blocked 7-point kernel, with a partial last slab
+        assert D32.m.dia_sym == 2 and D.m.dia_sym == 1
     x32, b32 = x.astype(np.float32), b.astype(np.float32)
     x32d, b32d = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray.from_numpy(ctx, b32)
     out32 = ua.DeviceArray(ctx, n, np.float32)

@@ -314,16 +314,17 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     }
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
     // FP32 symmetric 7-point operators with the largest offset a multiple of 64 take the
-    // plane-blocked kernel (its FP64 instances hold 110+ VGPRs, occupancy 4, and lose
-    // more on the epilogue loads than they gain on the matrix)
-    if (sym && A->dtype == UHIP_F32 && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] &&
-        offs[6] % 64 == 0 && (int64_t)n + (int64_t)kDia7Planes * offs[6] < INT32_MAX)
+    // plane-blocked kernel. In FP64 (2 planes per wave) the plain product gains 4%
+    // but the fused-epilogue forms lose more, and the FP64 solve slows by 6%.
+    const int planes = dia7_planes(A->dtype);
+    if (sym && A->dtype == UHIP_F32 && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] && offs[6] % 64 == 0 &&
+        (int64_t)n + (int64_t)planes * offs[6] < INT32_MAX)
         sym = 2;
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->dia_sym = sym;
     A->format = UHIP_FMT_DIA;
-    A->nparts = sym == 2 ? dia7z_workgroups(n, offs[6])
+    A->nparts = sym == 2 ? dia7z_workgroups(n, offs[6], planes)
                          : (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
     return UHIP_OK;
 }
@@ -534,7 +535,8 @@ static int plan_vi(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     A->ndiag = nd;
     A->vi_nvals = nv;
     A->format = UHIP_FMT_VI;
-    A->nparts = (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
+    const int32_t rows_per_wg = kDiaWG * vi_rows_per_lane(A->dtype, nd);
+    A->nparts = (n + rows_per_wg - 1) / rows_per_wg;
     return UHIP_OK;
 }
 

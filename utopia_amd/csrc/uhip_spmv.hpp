@@ -163,8 +163,8 @@ constexpr int kDiaWG = 256;
 // rows per thread
 template <typename TA>
 struct DiaRows {
-    static constexpr int R = sizeof(TA) == 8 ? 2 : 4;  // rows per thread
-    static constexpr int G = sizeof(TA) == 8 ? 8 : 4;  // diagonals whose loads are in flight together
+    static constexpr int R = 2;  // rows per thread
+    static constexpr int G = 8;  // diagonals whose loads are in flight together
 };
 inline int32_t dia_rows_per_wg(int dtype) { return kDiaWG * (dtype == UHIP_F64 ? DiaRows<double>::R : DiaRows<float>::R); }
 
@@ -174,13 +174,13 @@ __host__ __device__ constexpr int32_t dia_ld(int32_t n) { return (n + 63) & ~63;
 // (offset -o) is read from the stored +o at row r - o. Those values are read twice (the
 // mirror read o rows later hits in L2), so SYM loads them through the cache; the full
 // layout reads every value once and streams it.
-template <typename TAcc, class Epi, typename TA, typename TX, typename TM, bool SYM>
+template <typename TAcc, class Epi, typename TA, typename TX, typename TM, bool SYM, int R = DiaRows<TA>::R,
+          int G = DiaRows<TA>::G>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
                                                      const TA *__restrict__ val, const TM *__restrict__ mask,
                                                      const TX *__restrict__ x, Epi epi, double *__restrict__ partials,
                                                      int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
-    constexpr int R = DiaRows<TA>::R;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
     const int32_t base = (b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6)) * (64 * R);
@@ -203,7 +203,6 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
         // x is read at the clamped column whether or not the entry exists (the mask
         // only selects in the sum), so no load waits for the mask
         const int32_t xlast = nrows - 1;
-        constexpr int G = DiaRows<TA>::G;
         const int32_t k = ndiag >> 1;
         for (int d = 0; d < ndiag; d += G) {
             TA vv[G][R];
@@ -256,12 +255,17 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
 
 // Symmetric 7-point DIA (offsets -S2, -S1, -1, 0, +1, +S1, +S2, S2 a multiple of 64;
 // stored: main, +1, +S1, +S2), blocked over the largest offset: a wave owns 64
-// consecutive rows of kDia7Planes consecutive planes (plane stride S2), lane l
-// row slab * P * S2 + z * S2 + seg * 64 + l. The -S2 mirror and x at +-S2 come from
+// consecutive rows of P consecutive planes (plane stride S2), lane l
+// row slab * P * S2 + z * S2 + seg * 64 + l (P = Dia7Planes). The -S2 mirror and x at +-S2 come from
 // the neighbouring plane in registers, the -1 mirror from a wave shuffle of the +1
 // values, so every stored value is read once and streams; the row sums keep the
 // column order (bit-identical to the CSR sum).
-constexpr int kDia7Planes = 4;
+// planes per wave: 4 for FP32; 2 for FP64, whose 4-plane form holds 110+ VGPRs
+template <typename TA>
+struct Dia7Planes {
+    static constexpr int P = sizeof(TA) == 8 ? 2 : 4;
+};
+inline int dia7_planes(int dtype) { return dtype == UHIP_F64 ? Dia7Planes<double>::P : Dia7Planes<float>::P; }
 
 template <typename TAcc, class Epi, typename TA, typename TX>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int32_t *__restrict__ off,
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
                                                        const TX *__restrict__ x, Epi epi,
                                                        double *__restrict__ partials, int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
-    constexpr int P = kDia7Planes;
+    constexpr int P = Dia7Planes<TA>::P;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
     const int64_t ld = dia_ld(nrows);
@@ -339,10 +343,20 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
     }
 }
 
-inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2) {
-    const int64_t slabs = ((int64_t)nrows + (int64_t)kDia7Planes * S2 - 1) / ((int64_t)kDia7Planes * S2);
+inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2, int planes) {
+    const int64_t slabs = ((int64_t)nrows + (int64_t)planes * S2 - 1) / ((int64_t)planes * S2);
     const int64_t waves = slabs * (S2 / 64);
     return (int32_t)((waves + kDiaWG / 64 - 1) / (kDiaWG / 64));
+}
+
+// Rows per lane of the VI kernel: for at most 8 slots 4 (FP32) or 2 (FP64), else 1
+// (a 27-slot restriction row is latency-bound: more rows per lane only lengthen it).
+template <typename TA>
+constexpr int vi_rows(int32_t ndiag) {
+    return ndiag <= 8 ? (sizeof(TA) == 8 ? 2 : 4) : 1;
+}
+inline int vi_rows_per_lane(int dtype, int32_t ndiag) {
+    return dtype == UHIP_F64 ? vi_rows<double>(ndiag) : vi_rows<float>(ndiag);
 }
 
 // VI (value-indexed, row-relative): slot d of row r holds the entry at column
@@ -351,14 +365,13 @@ inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2) {
 // is staged in LDS. A row's codes arrive in NW wide loads, a slot that no row of
 // the wavefront holds costs nothing (wave-uniform branch), and the masked,
 // column-ordered sum makes each row sum the sequential CSR sum bit for bit.
-template <typename TAcc, class Epi, typename TA, typename TX, typename TM>
+template <typename TAcc, class Epi, typename TA, typename TX, typename TM, int R>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
                                                     const uint64_t *__restrict__ codes, const TA *__restrict__ table,
                                                     int32_t nvals, const int32_t *__restrict__ vbase,
                                                     const TM *__restrict__ mask, const TX *__restrict__ x, Epi epi,
                                                     double *__restrict__ partials, int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
-    constexpr int R = DiaRows<TA>::R;
     constexpr int NW = (int)sizeof(TM);  // 64-bit code words per row: 8 slots each
     __shared__ TA tab[256];
     for (int i = threadIdx.x; i < nvals; i += kDiaWG) tab[i] = table[i];
@@ -446,13 +459,13 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         if (A->nrows == 0) return UHIP_OK;
         if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
         const dim3 g(A->nparts), blk(kDiaWG);
-#define UHIP_VI_LAUNCH(TM)                                                                                          \
-    hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, A->dia_off, \
-                       (const uint64_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals, A->vi_base,           \
-                       (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
-        if (A->ndiag <= 8) UHIP_VI_LAUNCH(uint8_t);
-        else if (A->ndiag <= 16) UHIP_VI_LAUNCH(uint16_t);
-        else UHIP_VI_LAUNCH(uint32_t);
+#define UHIP_VI_LAUNCH(TM, RR)                                                                                  \
+    hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM, RR>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag,      \
+                       A->dia_off, (const uint64_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals,         \
+                       A->vi_base, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
+        if (A->ndiag <= 8) UHIP_VI_LAUNCH(uint8_t, vi_rows<TA>(8));
+        else if (A->ndiag <= 16) UHIP_VI_LAUNCH(uint16_t, 1);
+        else UHIP_VI_LAUNCH(uint32_t, 1);
 #undef UHIP_VI_LAUNCH
         UHIP_CHECK_LAUNCH();
         return UHIP_OK;
