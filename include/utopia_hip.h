@@ -323,6 +323,10 @@ int uhip_dense_cholesky(uhip_ctx ctx, int32_t n, double *A, int32_t lda, int *in
 int uhip_dense_cholesky_inverse(uhip_ctx ctx, int32_t n, const double *L, int32_t lda, double *Ainv);
 /* y = M x (row-major n x n, FP64), x/y of dtype x_dtype / y_dtype */
 int uhip_dense_gemv(uhip_ctx ctx, int32_t n, const double *M, const void *x, int x_dtype, void *y, int y_dtype);
+/* y = M x with an FP32 copy of M (the coarse solve of the FP32 V-cycle: half the
+ * bytes of the FP64 inverse; products summed in FP64 as uhip_dense_gemv). No
+ * reference analogue (PETSc's coarse solve is a factorisation in one precision). */
+int uhip_dense_gemv_m32(uhip_ctx ctx, int32_t n, const float *M, const float *x, float *y);
 
 /* ------------------------------------------------------------------------
  * Box-constrained QP kernels (FP64).
@@ -440,7 +444,9 @@ int uhip_solver_precondition(uhip_solver s, const double *rhs, double *z);
 /* SolutionStatus (iterates, gradient_norm, reason) and solve wall time */
 int uhip_solver_get_status(uhip_solver s, int64_t *iterates, double *gradient_norm, int *reason, double *seconds);
 /* setup_seconds solve_seconds total_iterations refinements n_levels coarse_n eig_max_<l> eig_min_<l>
- * level_nnz_<l> level_rows_<l> ncolors */
+ * level_nnz_<l> level_rows_<l> ncolors transfer_{nnz,rows,cols}_<i> alpha_bar active_set_changes;
+ * format_{A,R,P}_<l>: the SpMV plan (UHIP_FMT_*) of level l's operator, restriction or
+ * interpolation in the FP64 V-cycle, format32_{A,R,P}_<l> in the FP32 one of a mixed solver */
 int uhip_solver_get_info(uhip_solver s, const char *key, double *value);
 int uhip_solver_get_level(uhip_solver s, int l, int32_t *rowptr, int32_t *col, double *val);
 int uhip_solver_get_mask(uhip_solver s, int l, double *mask);

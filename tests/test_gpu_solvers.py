@@ -546,3 +546,25 @@ def test_cg_reference_hexa_fem_system(ctx, golden, pc):
     assert abs(cg.status["iterates"] - so["iterates"]) <= 1
     assert rel(x, xo) <= 1e-10
     assert np.linalg.norm(b - K.to_scipy() @ x) < 1e-6
+
+
+def test_level_plan_formats_reported(ctx):
+    """format_{A,R,P}_<l> / format32_{A,R,P}_<l> report the SpMV plan of each level: with
+    the automatic rules the 7-point fine operator is DIA in both precisions and the
+    trilinear transfers are value-indexed."""
+    n = 24
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=4)
+    cg = ua.ConjugateGradient(ctx, "multigrid_mixed", atol=1e-8, rtol=0, stol=0, max_it=50)
+    cg.set_transfer_operators(ua.mg_hierarchy(n, n, n, 3))
+    cg.update(A)
+    f = int(cg.info("n_levels")) - 1
+    assert cg.info(f"format_A_{f}") == ua.FMT_DIA and cg.info(f"format32_A_{f}") == ua.FMT_DIA
+    for w in "RP":
+        assert cg.info(f"format_{w}_{f}") == ua.FMT_VI and cg.info(f"format32_{w}_{f}") == ua.FMT_VI
+    with pytest.raises(ua.HipError):
+        cg.info("format_R_0")  # the coarsest level has no transfers
+    plain = ua.ConjugateGradient(ctx, "multigrid", atol=1e-8, rtol=0, stol=0, max_it=50)
+    plain.set_transfer_operators(ua.mg_hierarchy(n, n, n, 3))
+    plain.update(A)
+    with pytest.raises(ua.HipError):
+        plain.info(f"format32_A_{f}")  # no FP32 V-cycle

@@ -44,8 +44,19 @@ public:
         return solve_raw(rhs.data(), UHIP_F64, sol.data(), UHIP_F64);
     }
     bool solve_raw(const void *rhs, int rhs_dt, void *sol, int sol_dt) const {
-        hip_check(uhip_dense_gemv(ctx_->get(), n_, ainv_, rhs, rhs_dt, sol, sol_dt), "coarse gemv");
+        if (ainv32_ && rhs_dt == UHIP_F32 && sol_dt == UHIP_F32)
+            hip_check(uhip_dense_gemv_m32(ctx_->get(), n_, ainv32_, (const float *)rhs, (float *)sol), "coarse gemv");
+        else
+            hip_check(uhip_dense_gemv(ctx_->get(), n_, ainv_, rhs, rhs_dt, sol, sol_dt), "coarse gemv");
         return true;
+    }
+    // FP32 copy of the inverse, used by FP32-in / FP32-out solves (the FP32 V-cycle)
+    void prepare_f32() {
+        if (ainv32_ || !ainv_) return;
+        void *p = nullptr;
+        hip_check(uhip_malloc(ctx_->get(), &p, sizeof(float) * (size_t)n_ * n_), "alloc inverse f32");
+        ainv32_ = (float *)p;
+        hip_check(uhip_vec_cast_f64_f32(ctx_->get(), (int64_t)n_ * n_, ainv_, ainv32_), "inverse f32");
     }
     int32_t n() const { return n_; }
     const double *inverse() const { return ainv_; }
@@ -53,10 +64,13 @@ public:
 private:
     void release() {
         if (ainv_ && ctx_) uhip_free(ctx_->get(), ainv_);
+        if (ainv32_ && ctx_) uhip_free(ctx_->get(), ainv32_);
         ainv_ = nullptr;
+        ainv32_ = nullptr;
     }
     HipContextPtr ctx_;
     double *ainv_ = nullptr;
+    float *ainv32_ = nullptr;
     int32_t n_ = 0;
 };
 
@@ -582,6 +596,7 @@ private:
 class MixedPrecisionMultigrid final : public LinearSolver<HipCrsMatrix, HipVector> {
 public:
     explicit MixedPrecisionMultigrid(std::shared_ptr<Multigrid> mg) : mg_(std::move(mg)) {}
+    const VCycle<float> &engine() const { return eng_; }
     void update(const std::shared_ptr<const HipCrsMatrix> &op) override {
         LinearSolver::update(op);
         mg_->update(op);
@@ -593,6 +608,7 @@ public:
         eng_.mg_type = src.mg_type;
         eng_.masks = src.masks;
         eng_.smoother = src.smoother;
+        mg_->coarse_solver()->prepare_f32();
         eng_.coarse = src.coarse;
         if (eng_.smoother != MG_CHEBYSHEV) throw HipError(UHIP_ERR_UNSUPPORTED, "mixed precision: Chebyshev smoother only");
         for (size_t l = 0; l < src.levels.size(); ++l) {

@@ -163,26 +163,26 @@ __global__ __launch_bounds__(256) void k_identity(int n, double *__restrict__ B)
 // y = M x, one wavefront per row. Lane l keeps eight chains, chain u summing columns
 // l + 64 u + 512 j in order (eight loads in flight per lane); the chains are combined
 // in a fixed tree and the lanes by the butterfly, so the result is the same on every run.
-template <typename TX, typename TY>
-__global__ __launch_bounds__(256) void k_gemv(int n, const double *__restrict__ M, const TX *__restrict__ x,
+template <typename TX, typename TY, typename TM = double>
+__global__ __launch_bounds__(256) void k_gemv(int n, const TM *__restrict__ M, const TX *__restrict__ x,
                                               TY *__restrict__ y) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
-    const double *m = M + (int64_t)row * n;
+    const TM *m = M + (int64_t)row * n;
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int c = lane;
     for (; c + 7 * 64 < n; c += 8 * 64) {
         double mv[8], xv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            mv[u] = m[c + u * 64];
+            mv[u] = (double)m[c + u * 64];
             xv[u] = (double)x[c + u * 64];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) a[u] += mv[u] * xv[u];
     }
-    for (int u = 0; c < n; c += 64, ++u) a[u] += m[c] * (double)x[c];
+    for (int u = 0; c < n; c += 64, ++u) a[u] += (double)m[c] * (double)x[c];
     double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     s = wave_sum(s);
     if (lane == 0) y[row] = (TY)s;
@@ -273,6 +273,14 @@ int uhip_dense_gemv(uhip_ctx ctx, int32_t n, const double *M, const void *x, int
         hipLaunchKernelGGL((k_gemv<float, double>), g, b, 0, s, n, M, (const float *)x, (double *)y);
     else
         hipLaunchKernelGGL((k_gemv<double, float>), g, b, 0, s, n, M, (const double *)x, (float *)y);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_dense_gemv_m32(uhip_ctx ctx, int32_t n, const float *M, const float *x, float *y) {
+    UHIP_REQUIRE(ctx && M && x && y && n >= 0, "uhip_dense_gemv_m32: bad argument");
+    if (n == 0) return UHIP_OK;
+    hipLaunchKernelGGL((k_gemv<float, float, float>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, M, x, y);
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
 }

@@ -439,7 +439,29 @@ int uhip_solver_get_info(uhip_solver s, const char *key, double *value) {
     else if (k.rfind("eig_min_", 0) == 0 && s->mg) *value = s->mg->level_eig_min(std::stoi(k.substr(8)));
     else if (k.rfind("level_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(10))).nnz();
     else if (k.rfind("level_rows_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(11))).rows();
-    else return UHIP_ERR_INVALID;
+    else if ((k.rfind("format_", 0) == 0 || k.rfind("format32_", 0) == 0) && s->mg && k.size() >= 10) {
+        // format_{A,R,P}_<l> of the FP64 V-cycle, format32_{A,R,P}_<l> of the FP32 one (mixed)
+        const bool f32 = k.rfind("format32_", 0) == 0;
+        const size_t at = f32 ? 9 : 7;
+        const char which = k[at];
+        const int l = std::stoi(k.substr(at + 2));
+        auto pick = [&](const auto &lv) -> const uhip_csr * {
+            if (which == 'A' && lv.A) return &lv.A->raw();
+            if (which == 'R' && lv.R) return &lv.R->raw();
+            if (which == 'P' && lv.P) return &lv.P->raw();
+            return nullptr;
+        };
+        const uhip_csr *m = nullptr;
+        if (f32) {
+            if (!s->mixed || l < 0 || l >= (int)s->mixed->engine().levels.size()) return UHIP_ERR_INVALID;
+            m = pick(s->mixed->engine().levels[(size_t)l]);
+        } else {
+            if (l < 0 || l >= (int)s->mg->engine().levels.size()) return UHIP_ERR_INVALID;
+            m = pick(s->mg->engine().levels[(size_t)l]);
+        }
+        if (!m) return UHIP_ERR_INVALID;
+        *value = (double)m->format;
+    } else return UHIP_ERR_INVALID;
     return UHIP_OK;
 }
 
