@@ -373,11 +373,48 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
                                                     double *__restrict__ partials, int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
     constexpr int NW = (int)sizeof(TM);  // 64-bit code words per row: 8 slots each
+    constexpr bool kFlat = R == 1 && NW >= 2;
     __shared__ TA tab[256];
+    __shared__ int32_t so[32];
     for (int i = threadIdx.x; i < nvals; i += kDiaWG) tab[i] = table[i];
+    if (kFlat && threadIdx.x < 32) so[threadIdx.x] = (int32_t)threadIdx.x < ndiag ? off[threadIdx.x] : 0;
     __syncthreads();
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
+    if constexpr (kFlat) {
+        // Wide rows (a restriction's 27 slots), one row per lane and no branch: all code
+        // words, then all 8 * NW gathers (an absent slot reads x at the row's base column,
+        // always in bounds), then the masked sum in slot order. Every load is in flight
+        // before the first use.
+        double part[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) part[d] = 0.0;
+        const int32_t row = b * kDiaWG + (int32_t)threadIdx.x;
+        const int32_t rc = min(row, nrows - 1);
+        const uint32_t m = row < nrows ? (uint32_t)mask[rc] : 0u;
+        const int32_t rb = vbase[rc];
+        uint64_t c[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) c[w] = ld_stream(codes + (int64_t)rc * NW + w);
+        TX xv[8 * NW];
+#pragma unroll
+        for (int d = 0; d < 8 * NW; ++d) xv[d] = x[((m >> d) & 1u) ? rb + so[d] : rb];
+        TAcc acc = TAcc(0);
+#pragma unroll
+        for (int d = 0; d < 8 * NW; ++d) {
+            const TAcc p = (TAcc)tab[(uint32_t)(c[d / 8] >> (8 * (d % 8))) & 0xffu] * (TAcc)xv[d];
+            acc = ((m >> d) & 1u) ? acc + p : acc;
+        }
+        if (row < nrows) epi(row, acc, part);
+        if constexpr (Epi::kDots > 0) {
+#pragma unroll
+            for (int d = 0; d < Epi::kDots; ++d) {
+                const double t = block_sum<kDiaWG>(part[d]);
+                if (threadIdx.x == 0) partials[d * stride + b] = t;
+            }
+        }
+        return;
+    }
     const int32_t base = (b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6)) * (64 * R);
     double part[ND];
 #pragma unroll
