@@ -198,6 +198,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
             m[i] = (uint32_t)mask[min(row[i], nrows - 1)];
             acc[i] = TAcc(0);
         }
+        typename Epi::In ein[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) ein[i] = epi.load(min(row[i], nrows - 1));
 #pragma unroll
         for (int i = 0; i < R; ++i) m[i] = row[i] < nrows ? m[i] : 0u;
         // x is read at the clamped column whether or not the entry exists (the mask
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
         }
 #pragma unroll
         for (int i = 0; i < R; ++i)
-            if (row[i] < nrows) epi(row[i], acc[i], part);
+            if (row[i] < nrows) epi.apply(row[i], acc[i], ein[i], part);
     }
     if constexpr (Epi::kDots > 0) {
 #pragma unroll
@@ -290,7 +293,8 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
         const int32_t last = (int32_t)ld - 1;
         int32_t r[P];
         uint32_t m[P];
-        TA a0[P], a1[P], aS1[P], aS2[P], mS1[P];
+        TA a0[P], a1[P], aS1[P], aS2[P], mS1[P], edge[P];
+        typename Epi::In ein[P];
         TX xc[P], xm1[P], xp1[P], xmS1[P], xpS1[P];
         // x is read at clamped rows whether or not the entry exists (the mask only
         // selects in the sum), so no load waits for the mask
@@ -306,6 +310,8 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
             aS1[z] = ld_stream(vS1 + rc);
             aS2[z] = ld_stream(vS2 + rc);
             mS1[z] = ld_stream(vS1 + max(rc - S1, 0));
+            edge[z] = v1[max(rc - 1, 0)];  // the -1 mirror of lane 0 (the others shuffle it)
+            ein[z] = epi.load(min(r[z], xlast));
             xc[z] = x[cl(r[z])];
             xm1[z] = x[cl(r[z] - 1)];
             xp1[z] = x[cl(r[z] + 1)];
@@ -320,8 +326,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
 #pragma unroll
         for (int z = 0; z < P; ++z) {
             const TA up = __shfl_up(a1[z], 1, 64);
-            const TA edge = v1[max(min(r[z], last) - 1, 0)];
-            const TA m1 = lane == 0 ? edge : up;
+            const TA m1 = lane == 0 ? edge[z] : up;
             const TA vals[7] = {z == 0 ? mS2_0 : aS2[z > 0 ? z - 1 : 0], mS1[z], m1, a0[z], a1[z], aS1[z], aS2[z]};
             const TX xs[7] = {z == 0 ? xmS2_0 : xc[z > 0 ? z - 1 : 0], xmS1[z], xm1[z], xc[z], xp1[z], xpS1[z],
                               z == P - 1 ? xpS2_P : xc[z < P - 1 ? z + 1 : 0]};
@@ -331,7 +336,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
                 const TAcc p = (TAcc)vals[d] * (TAcc)xs[d];
                 acc = ((m[z] >> d) & 1u) ? acc + p : acc;
             }
-            if (r[z] < nrows) epi(r[z], acc, part);
+            if (r[z] < nrows) epi.apply(r[z], acc, ein[z], part);
         }
     }
     if constexpr (Epi::kDots > 0) {
@@ -393,6 +398,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         const int32_t rc = min(row, nrows - 1);
         const uint32_t m = row < nrows ? (uint32_t)mask[rc] : 0u;
         const int32_t rb = vbase[rc];
+        const typename Epi::In ein = epi.load(rc);
         uint64_t c[NW];
 #pragma unroll
         for (int w = 0; w < NW; ++w) c[w] = ld_stream(codes + (int64_t)rc * NW + w);
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
             const TAcc p = (TAcc)tab[(uint32_t)(c[d / 8] >> (8 * (d % 8))) & 0xffu] * (TAcc)xv[d];
             acc = ((m >> d) & 1u) ? acc + p : acc;
         }
-        if (row < nrows) epi(row, acc, part);
+        if (row < nrows) epi.apply(row, acc, ein, part);
         if constexpr (Epi::kDots > 0) {
 #pragma unroll
             for (int d = 0; d < Epi::kDots; ++d) {
@@ -434,6 +440,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
             rb[i] = vbase[rc];
             acc[i] = TAcc(0);
         }
+        typename Epi::In ein[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) ein[i] = epi.load(min(row[i], nrows - 1));
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             m[i] = row[i] < nrows ? m[i] : 0u;
@@ -473,7 +482,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         }
 #pragma unroll
         for (int i = 0; i < R; ++i)
-            if (row[i] < nrows) epi(row[i], acc[i], part);
+            if (row[i] < nrows) epi.apply(row[i], acc[i], ein[i], part);
     }
     if constexpr (Epi::kDots > 0) {
 #pragma unroll
@@ -574,89 +583,133 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
 }
 
 // ---- epilogues ------------------------------------------------------------
+// Each epilogue splits into load(r) - the row's vector inputs, which the DIA, plane-
+// blocked and VI kernels issue together with the matrix loads - and apply(r, s, in,
+// part), which combines them with the row sum s and stores. operator() does both.
+
+struct EpiNoIn {};
 
 // y = s
 template <typename TAcc, typename TY>
 struct EpiStore {
     static constexpr int kDots = 0;
+    using In = EpiNoIn;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)s); }
+    __device__ In load(int) const { return In{}; }
+    __device__ void apply(int r, TAcc s, const In &, double *) const { st_stream(y + r, (TY)s); }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // y = alpha*s + beta*y   (beta == 0 never reads y)
 template <typename TAcc, typename TY>
 struct EpiAxpby {
     static constexpr int kDots = 0;
+    struct In {
+        TAcc y;
+    };
     TY *y;
     TAcc alpha, beta;
-    __device__ void operator()(int r, TAcc s, double *) const {
-        st_stream(y + r, (beta == TAcc(0)) ? (TY)(alpha * s) : (TY)(alpha * s + beta * (TAcc)y[r]));
+    __device__ In load(int r) const { return In{beta == TAcc(0) ? TAcc(0) : (TAcc)y[r]}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *) const {
+        st_stream(y + r, (beta == TAcc(0)) ? (TY)(alpha * s) : (TY)(alpha * s + beta * in.y));
     }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // y = alpha*s + beta*z
 template <typename TAcc, typename TY>
 struct EpiAxpbz {
     static constexpr int kDots = 0;
+    struct In {
+        TAcc z;
+    };
     const TY *z;
     TY *y;
     TAcc alpha, beta;
-    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)(alpha * s + beta * (TAcc)z[r])); }
+    __device__ In load(int r) const { return In{(TAcc)z[r]}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *) const {
+        st_stream(y + r, (TY)(alpha * s + beta * in.z));
+    }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // r = b - s
 template <typename TAcc, typename TB, typename TY>
 struct EpiResidual {
     static constexpr int kDots = 0;
+    struct In {
+        TAcc b;
+    };
     const TB *b;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)((TAcc)b[r] - s)); }
+    __device__ In load(int r) const { return In{(TAcc)b[r]}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *) const { st_stream(y + r, (TY)(in.b - s)); }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // y = s, part += x_r * s  (p.Ap of CG)
 template <typename TAcc, typename TX, typename TY>
 struct EpiStoreDot {
     static constexpr int kDots = 1;
+    struct In {
+        double x;
+    };
     const TX *x;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *part) const {
+    __device__ In load(int r) const { return In{(double)x[r]}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *part) const {
         st_stream(y + r, (TY)s);
-        part[0] += (double)x[r] * (double)s;
+        part[0] += in.x * (double)s;
     }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // y = s, part[0] += x_r s, part[1] += z_r x_r   (A x with x.Ax and z.x in one pass)
 template <typename TAcc, typename TY, typename TX>
 struct EpiStoreDot2 {
     static constexpr int kDots = 2;
+    struct In {
+        double x, z;
+    };
     const TX *x;
     const double *z;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *part) const {
+    __device__ In load(int r) const { return In{(double)x[r], z[r]}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *part) const {
         st_stream(y + r, (TY)s);
-        const double xr = (double)x[r];
-        part[0] += xr * (double)s;
-        part[1] += z[r] * xr;
+        part[0] += in.x * (double)s;
+        part[1] += in.z * in.x;
     }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // y = mask .* s  (restriction followed by MultiLevelMask::apply); mask may be null
 template <typename TAcc, typename TY>
 struct EpiMaskStore {
     static constexpr int kDots = 0;
+    struct In {
+        TAcc m;
+    };
     const TY *mask;
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const {
-        st_stream(y + r, mask ? (TY)((TAcc)mask[r] * s) : (TY)s);
+    __device__ In load(int r) const { return In{mask ? (TAcc)mask[r] : TAcc(1)}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *) const {
+        st_stream(y + r, mask ? (TY)(in.m * s) : (TY)s);
     }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // y = y + s  (coarse-grid correction c += I c_c)
 template <typename TAcc, typename TY>
 struct EpiAddTo {
     static constexpr int kDots = 0;
+    struct In {
+        TAcc y;
+    };
     TY *y;
-    __device__ void operator()(int r, TAcc s, double *) const { st_stream(y + r, (TY)((TAcc)y[r] + s)); }
+    __device__ In load(int r) const { return In{(TAcc)y[r]}; }
+    __device__ void apply(int r, TAcc s, const In &in, double *) const { st_stream(y + r, (TY)(in.y + s)); }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 // One step of Chebyshev3level::solve (utopia_Chebyshev3level.hpp:138-175) fused
@@ -669,6 +722,10 @@ struct EpiAddTo {
 template <typename TAcc, typename TP, typename TV, typename TXI, typename TXO, bool DOT = false>
 struct EpiCheb {
     static constexpr int kDots = DOT ? 1 : 0;
+    struct In {
+        TAcc pp, rp, x;
+        double dw;
+    };
     const TP *pprev;
     const TV *rprev;
     const TXI *xin;
@@ -678,13 +735,19 @@ struct EpiCheb {
     TAcc a_prev, beta, a_cur;
     int last;
     const double *dot_with;
-    __device__ void operator()(int r, TAcc s, double *part) const {
-        const TAcc pp = (TAcc)pprev[r];
-        const TAcc rp = rprev ? (TAcc)rprev[r] : -pp;
-        const TAcc rk = rp + a_prev * s;
-        const TAcc pk = -rk + beta * pp;
-        TAcc xv = xin ? (TAcc)xin[r] : TAcc(0);
-        xv = xv + a_prev * pp;
+    __device__ In load(int r) const {
+        In in;
+        in.pp = (TAcc)pprev[r];
+        in.rp = rprev ? (TAcc)rprev[r] : -in.pp;
+        in.x = xin ? (TAcc)xin[r] : TAcc(0);
+        in.dw = DOT ? dot_with[r] : 0.0;
+        return in;
+    }
+    __device__ void apply(int r, TAcc s, const In &in, double *part) const {
+        const TAcc rk = in.rp + a_prev * s;
+        const TAcc pk = -rk + beta * in.pp;
+        TAcc xv = in.x;
+        xv = xv + a_prev * in.pp;
         if (last) {
             xv = xv + a_cur * pk;
         } else {
@@ -693,8 +756,9 @@ struct EpiCheb {
         }
         const TXO xo = (TXO)xv;
         st_stream(xout + r, xo);
-        if constexpr (DOT) part[0] += (double)xo * dot_with[r];
+        if constexpr (DOT) part[0] += (double)xo * in.dw;
     }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
 }  // namespace uhip
