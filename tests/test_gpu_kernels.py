@@ -565,6 +565,7 @@ VI_MATRICES = {
     "P3d": lambda: ua.gen_prolongation(16, 12, 10),
     "R3d": lambda: ua.Csr.from_scipy(ua.gen_prolongation(16, 12, 10).to_scipy().T),
     "P2d": lambda: ua.gen_prolongation(34, 20),
+    "R2d": lambda: ua.Csr.from_scipy(ua.gen_prolongation(34, 20).to_scipy().T),
     "agg": lambda: _agg_interp(3001, 4),
 }
 
@@ -592,6 +593,43 @@ def test_spmv_vi_bit_exact(ctx, name, dt):
         mkd = ua.DeviceArray.from_numpy(ctx, mk)
         ua.check(L().uhip_spmv_masked(ctx.h, D.ref, P(xd), P(mkd), P(yd)))
         assert np.array_equal(yd.numpy(), mk * ref)
+
+
+def _box27(nx, ny, nz):
+    """27-point box stencil: -1 off the diagonal, 27 on it."""
+    T = lambda k: sp.diags([np.ones(k - 1), np.ones(k), np.ones(k - 1)], [-1, 0, 1])
+    B = sp.kron(sp.kron(T(nz), T(ny)), T(nx)).tocsr()
+    return ua.Csr.from_scipy((28 * sp.identity(B.shape[0]) - B).tocsr())
+
+
+VI_SQUARE = {
+    "2d9": (lambda: ua.gen_stencil(ua.STENCIL_2D9, 30, 20), 9),
+    "3d7": (lambda: ua.gen_stencil(ua.STENCIL_3D7, 10, 9, 8), 13),
+    "box27": (lambda: _box27(9, 8, 7), 27),
+}
+
+
+@pytest.mark.parametrize("name", sorted(VI_SQUARE))
+def test_vi_square_epilogues(ctx, name):
+    """Constant-coefficient stencils (two values) forced into VI: rows of 9-16 and 17-32
+    slots from their first column take the two- and four-word code paths. y = A x, the
+    residual and A p with p.Ap match the CSR sums bit for bit (the dot to rounding)."""
+    make, slots = VI_SQUARE[name]
+    A = make()
+    n = A.shape[0]
+    D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_VI)
+    assert D.format == ua.FMT_VI and D.m.ndiag == slots and D.m.vi_nvals == 2
+    x, b = ua.gen_uniform(21, n), ua.gen_uniform(22, n)
+    Ax = orc.spmv(A, x)
+    xd, bd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b), ua.DeviceArray(ctx, n)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), Ax)
+    D.residual(xd, bd, yd)
+    assert np.array_equal(yd.numpy(), b - Ax)
+    s = ctx.scalars
+    ua.check(L().uhip_spmv_dot(ctx.h, D.ref, P(xd), P(yd), ctypes.c_void_p(s)))
+    assert np.array_equal(yd.numpy(), Ax)
+    assert abs(ctx.read_scalars(s, 1)[0] - x @ Ax) <= 1e-13 * (np.abs(x) @ np.abs(Ax))
 
 
 def test_vi_rejects_many_values_and_nan(ctx):
