@@ -422,7 +422,9 @@ int uhip_solver_destroy(uhip_solver s);
  * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
  * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour) scale_max_eig scale_min_eig power_method_tol
  * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each coarse_space_it
- * active_set_tol; AlgebraicMultigrid: n_levels bmax weight. Read at the next update.
+ * active_set_tol; AlgebraicMultigrid: n_levels bmax weight block_size (2-4: BlockAgglomerate,
+ * utopia_AlgebraicMultigrid.hpp:87-103) component (block weight entry, default block_size - 1).
+ * Read at the next update.
  * autotune (reps; 0 = off) applies to the SpMV plans of matrices built after it is set. */
 int uhip_solver_set_param(uhip_solver s, const char *key, double value);
 /* Multigrid::set_transfer_operators (utopia_LinearMultiLevel.hpp:65-79), host CSR, coarse to fine */

@@ -174,13 +174,72 @@ def agglomerate(A, bmax=3, weight=1.0 / 3):
     return _from_owned(out)
 
 
-def amg_transfers(A, n_levels, bmax=3, weight=1.0 / 3):
+def agglomerate_blocks(A, B, bmax=3, weight=1.0 / 3, component=None):
+    """BlockAgglomerate::create_transfer (solvers/multilevel/agglomerate/
+    utopia_BlockAgglomerate.hpp:37-141) for block size B, as (rowptr, col, val, shape).
+    Block (I, J) of the B x B block graph exists where any entry of its B x B tile is
+    stored; its weight is the tile's (component, component) entry (0 where absent;
+    component defaults to B - 1, :160). Pure-Python loops: small inputs only."""
+    M = Mat.of(A)
+    rp, col, val = np.asarray(M.rowptr), np.asarray(M.col), np.asarray(M.val)
+    n = M.shape[0]
+    c = B - 1 if component is None else component
+    assert n % B == 0 and 0 <= c < B
+    nb = n // B
+    rows = []  # per block row: [(J, weight)] in ascending J (the block CRS row view)
+    for I in range(nb):
+        w = {}
+        for r in range(I * B, (I + 1) * B):
+            for e in range(rp[r], rp[r + 1]):
+                w.setdefault(int(col[e]) // B, 0.0)
+        r = I * B + c
+        for e in range(rp[r], rp[r + 1]):
+            if int(col[e]) % B == c:
+                w[int(col[e]) // B] = float(val[e])
+        rows.append(sorted(w.items()))
+    a_max = [max([abs(v) for J, v in rows[I] if J != I], default=0.0) for I in range(nb)]  # :55-63
+    parent = [-1] * nb
+    nc = 0
+    for I in range(nb):  # :66-118
+        if parent[I] != -1:
+            continue
+        parent[I] = nc
+        count = 0
+        for J, v in rows[I]:
+            if weight * a_max[I] < abs(v) and parent[J] == -1:
+                parent[J] = nc
+                count += 1
+            if count >= bmax:
+                break
+        if count == 0:  # not a cluster centre: join the aggregate of the strongest block
+            parent[I] = -1
+            best, arg = 0.0, -1
+            for J, v in rows[I]:
+                if abs(v) > best:
+                    best, arg = abs(v), J
+            if arg != -1:
+                parent[I] = parent[arg]
+        else:
+            nc += 1
+    for I in range(nb):  # :112-118
+        if parent[I] == -1:
+            parent[I] = nc
+            nc += 1
+    pcol = np.array([parent[i // B] * B + i % B for i in range(n)], np.int32)  # :127-137
+    return np.arange(n + 1, dtype=np.int32), pcol, np.ones(n), (n, nc * B)
+
+
+def amg_transfers(A, n_levels, bmax=3, weight=1.0 / 3, block_size=1):
     """AlgebraicMultigridBuilder::build (solvers/multilevel/utopia_AlgebraicMultigrid.hpp:23-52):
-    P_l = agglomerate(A_{l+1}), A_l = P_l^T A_{l+1} P_l from the finest level down.
+    P_l = agglomerate(A_{l+1}), A_l = P_l^T A_{l+1} P_l from the finest level down;
+    block_size 2-4 aggregates with BlockAgglomerate (:87-103).
     Returns the interpolations and the operators, both coarse to fine, as Mat."""
     Ps, As = [], [Mat.of(A)]
     for _ in range(n_levels - 1):
-        P = Mat(*agglomerate(As[0], bmax, weight))
+        if 2 <= block_size <= 4:
+            P = Mat(*agglomerate_blocks(As[0], block_size, bmax, weight))
+        else:
+            P = Mat(*agglomerate(As[0], bmax, weight))
         As.insert(0, Mat(*ptap(As[0], P)))
         Ps.insert(0, P)
     return Ps, As

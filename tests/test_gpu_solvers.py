@@ -8,6 +8,7 @@ rounding), mixed precision within 1e-8 of the FP64 solution, QP active sets
 identical up to ties at 1e-12.
 """
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import oracle as orc
@@ -475,8 +476,8 @@ def test_amg_hierarchy_matches_oracle_exactly(ctx, case):
         assert _same_csr(G, As[l].rowptr, As[l].col, As[l].val)
 
 
-def _oracle_amg(A, L, pre, post, colors):
-    Ps, As = orc.amg_transfers(A, L)
+def _oracle_amg(A, L, pre, post, colors, block_size=1):
+    Ps, As = orc.amg_transfers(A, L, block_size=block_size)
     mg = orc.MG(Ps, smoother=2, pre=pre, post=post, masks=False)
     mg.update(A)
     for l in range(1, L):
@@ -568,3 +569,44 @@ def test_level_plan_formats_reported(ctx):
     plain.update(A)
     with pytest.raises(ua.HipError):
         plain.info(f"format32_A_{f}")  # no FP32 V-cycle
+
+
+def _vector_problem(nx, ny, seed):
+    """Two unknowns per node: S (x) [[2, .5], [.5, 1]] (SPD, coupled within and across nodes)."""
+    S = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, nx, ny, seed=seed).to_scipy()
+    return ua.Csr.from_scipy(sp.kron(S, sp.csr_matrix([[2.0, 0.5], [0.5, 1.0]])).tocsr())
+
+
+def test_block_amg_hierarchy_matches_oracle_exactly(ctx):
+    """AlgebraicMultigrid with block_size 2 (BlockAgglomerate on every level): transfers
+    and Galerkin operators bit for bit against the oracle's block pass."""
+    A = _vector_problem(40, 36, 3)
+    L = 4
+    s = ua.AlgebraicMultigrid(ctx, n_levels=L, smoother=2, block_size=2)
+    s.update(A)
+    Ps, As = orc.amg_transfers(A, L, block_size=2)
+    for l in range(L - 1):
+        P = s.transfer(l)
+        assert P.shape == Ps[l].shape and _same_csr(P, Ps[l].rowptr, Ps[l].col, Ps[l].val)
+        assert P.shape[1] % 2 == 0
+    for l in range(L):
+        G = s.level(l)
+        assert _same_csr(G, As[l].rowptr, As[l].col, As[l].val)
+    odd = ua.gen_stencil(ua.STENCIL_2D5, 5, 3)  # 15 rows: not a multiple of the block size
+    with pytest.raises(ua.HipError):
+        ua.AlgebraicMultigrid(ctx, n_levels=2, block_size=2).update(odd)
+
+
+def test_cg_block_amg_matches_oracle(ctx):
+    A = _vector_problem(40, 36, 3)
+    b = ua.gen_uniform(8, A.shape[0])
+    atol = 1e-10 * np.linalg.norm(b)
+    cg = ua.ConjugateGradient(ctx, "amg", n_levels=4, smoother=2, block_size=2, pre_smoothing_steps=2,
+                              post_smoothing_steps=2, atol=atol, rtol=0, stol=0, max_it=500)
+    x = cg.solve(A, b)
+    mg = _oracle_amg(A, 4, 2, 2, cg.colors, block_size=2)
+    xo, so = mg.cg_solve(b, atol=atol, max_it=500)
+    print(f"cg-block-amg: gpu it={cg.status['iterates']} oracle it={so['iterates']} rel.err={rel(x, xo):.2e}")
+    assert abs(cg.status["iterates"] - so["iterates"]) <= 1
+    assert rel(x, xo) <= 1e-10
+    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol

@@ -666,3 +666,59 @@ def test_cg_solves_reference_hexa_fem_system(golden):
     for pc in (0, 1):
         x, st = orc.cg_solve(K, b, precond=pc, atol=1e-10, rtol=0, stol=0, max_it=2000)
         assert st["converged"] and np.linalg.norm(b - K.to_scipy() @ x) < 1e-6
+
+
+# ---- BlockAgglomerate (solvers/multilevel/agglomerate/utopia_BlockAgglomerate.hpp) ----
+
+def _block_parents(P, B):
+    """Aggregate of each block row, read from a block interpolation."""
+    rp, col, val, shape = P
+    assert np.array_equal(rp, np.arange(len(rp))) and np.all(val == 1.0)
+    assert np.array_equal(col % B, np.arange(len(col)) % B)
+    return col[::B] // B
+
+
+def _scalar_parents(S):
+    rp, col, val, shape = orc.agglomerate(S)
+    assert np.array_equal(rp, np.arange(len(rp))) and np.all(val == 1.0)  # every row aggregated, a_max > 0
+    return col
+
+
+@pytest.mark.parametrize("B", [2, 3, 4])
+def test_block_agglomerate_of_kron_is_scalar_agglomerate(B):
+    """On S (x) M the block graph is S's graph and every block weight is S_IJ M_cc, so the
+    block pass must reproduce the (pinned) scalar pass on S, expanded to B sub-rows."""
+    S = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 13, 11, seed=2)
+    M = np.diag(np.arange(1.0, B + 1)) + 0.25 * (np.ones((B, B)) - np.eye(B))
+    A = ua.Csr.from_scipy(sp.kron(S.to_scipy(), sp.csr_matrix(M)).tocsr())
+    P = orc.agglomerate_blocks(A, B)
+    assert P[3] == (A.shape[0], (max(_scalar_parents(S)) + 1) * B)
+    assert np.array_equal(_block_parents(P, B), _scalar_parents(S))
+
+
+def test_block_agglomerate_weight_component():
+    """Two components with different couplings: the pass follows the component asked for
+    (default B - 1), and the zero-weight blocks of the other component's pattern change
+    nothing. A pass reading the wrong component fails: the two scalar results differ."""
+    S0 = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 12, 10, seed=1).to_scipy()
+    S1 = ua.gen_stencil(ua.STENCIL_2D5_EDGECOEF, 12, 10, seed=7).to_scipy()
+    E0, E1 = sp.csr_matrix(np.diag([1.0, 0.0])), sp.csr_matrix(np.diag([0.0, 1.0]))
+    A = ua.Csr.from_scipy((sp.kron(S0, E0) + sp.kron(S1, E1)).tocsr())
+    p0, p1 = _scalar_parents(ua.Csr.from_scipy(S0)), _scalar_parents(ua.Csr.from_scipy(S1))
+    assert not np.array_equal(p0, p1)
+    assert np.array_equal(_block_parents(orc.agglomerate_blocks(A, 2, component=0), 2), p0)
+    assert np.array_equal(_block_parents(orc.agglomerate_blocks(A, 2), 2), p1)
+
+
+def test_block_agglomerate_hand_example():
+    """Three block nodes, weights (component 1) w00=10, w01=w10=-1, w12=w21=-0.1, wii=10.
+    a_max excludes the diagonal block (utopia_BlockAgglomerate.hpp:55-63): node 0 opens
+    aggregate 0 and takes node 1 (|-1| > 1/3); node 2 (a_max 0.1) finds node 1 taken and
+    itself the strongest block, so it stays open and becomes aggregate 1. Counting the
+    diagonal in a_max would leave all three nodes singletons."""
+    W = np.array([[10.0, -1.0, 0.0], [-1.0, 10.0, -0.1], [0.0, -0.1, 10.0]])
+    other = np.array([[3.0, 2.0, 0.0], [2.0, 3.0, 2.0], [0.0, 2.0, 3.0]])  # component 0: ignored
+    A = ua.Csr.from_scipy(sp.csr_matrix(np.kron(other, np.diag([1.0, 0.0])) + np.kron(W, np.diag([0.0, 1.0]))))
+    P = orc.agglomerate_blocks(A, 2)
+    assert P[3] == (6, 4)
+    assert list(_block_parents(P, 2)) == [0, 0, 1]

@@ -482,7 +482,9 @@ class ProjectedGradient(Solver):
 
 class AlgebraicMultigrid(Solver):
     """AlgebraicMultigrid: Agglomerate transfers built from the operator at update
-    (params n_levels, bmax, weight, and the Multigrid smoother/cycle parameters)."""
+    (params n_levels, bmax, weight; block_size 2-4 aggregates B x B blocks with
+    BlockAgglomerate, component picks the block weight entry; and the Multigrid
+    smoother/cycle parameters)."""
     TYPE = SOLVER_AMG
 
 

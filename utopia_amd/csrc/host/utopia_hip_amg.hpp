@@ -1,5 +1,6 @@
 // AlgebraicMultigrid over the HIP backend:
 //   Agglomerate         solvers/multilevel/agglomerate/utopia_Agglomerate.hpp:12-187
+//   BlockAgglomerate    solvers/multilevel/agglomerate/utopia_BlockAgglomerate.hpp:12-163
 //   AlgebraicMultigrid  solvers/multilevel/utopia_AlgebraicMultigrid.hpp:14-173
 // The aggregation is a greedy pass whose every decision depends on the rows
 // before it, so it runs on the host over the downloaded CSR arrays (setup
@@ -7,7 +8,10 @@
 // cycles are the device Multigrid.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "utopia_hip_multigrid.hpp"
 
@@ -19,6 +23,84 @@ public:
     void weight(double v) { weight_ = v; }
     int64_t bmax() const { return bmax_; }
     double weight() const { return weight_; }
+    // AlgebraicMultigrid::ensure_agglomerator (utopia_AlgebraicMultigrid.hpp:87-103):
+    // block_size 2, 3 or 4 selects BlockAgglomerate, anything else the scalar pass.
+    // component < 0 means BlockSize - 1 (BlockAgglomerate's default).
+    void block_size(int v) { block_size_ = v; }
+    int block_size() const { return (block_size_ >= 2 && block_size_ <= 4) ? block_size_ : 1; }
+    void component(int v) { component_ = v; }
+    int component() const { return component_ < 0 ? block_size() - 1 : component_; }
+
+    // BlockAgglomerate::create_transfer (:37-141) on the B x B block graph of a matrix
+    // with n = B * nb rows: block (I, J) exists where any entry of rows I*B.. and columns
+    // J*B.. is stored, its weight is the entry (I*B + c, J*B + c) (0 if absent). a_max[I]
+    // is the largest |weight| off the diagonal block; block row I not yet in an aggregate
+    // opens one and takes up to bmax blocks J not yet in one with weight * a_max[I] <
+    // |w(I, J)|, in column order; if it takes none it joins the aggregate of its largest
+    // |w| block (possibly its own, leaving it open); open blocks become singletons in
+    // order. P(I*B + s, parent[I]*B + s) = 1 for every sub-row s.
+    static void aggregate_blocks(int32_t n, const int32_t *rp, const int32_t *col, const double *val, int B, int c,
+                                 int64_t bmax, double weight, std::vector<int32_t> &parent, int32_t &n_coarse) {
+        if (B < 2 || n % B != 0 || c < 0 || c >= B)
+            throw HipError(UHIP_ERR_INVALID, "BlockAgglomerate: rows must be a multiple of the block size, 0 <= component < block size");
+        const int32_t nb = n / B;
+        std::vector<std::vector<std::pair<int32_t, double>>> brow((size_t)nb);  // (J, weight), J ascending
+        std::vector<int32_t> js;
+        for (int32_t I = 0; I < nb; ++I) {
+            js.clear();
+            for (int32_t r = I * B; r < (I + 1) * B; ++r)
+                for (int32_t e = rp[r]; e < rp[r + 1]; ++e) js.push_back(col[e] / B);
+            std::sort(js.begin(), js.end());
+            js.erase(std::unique(js.begin(), js.end()), js.end());
+            auto &row = brow[(size_t)I];
+            row.reserve(js.size());
+            for (int32_t J : js) row.emplace_back(J, 0.0);
+            const int32_t r = I * B + c;
+            for (int32_t e = rp[r]; e < rp[r + 1]; ++e) {
+                if (col[e] % B != c) continue;
+                const int32_t J = col[e] / B;
+                auto it = std::lower_bound(row.begin(), row.end(), std::make_pair(J, -HUGE_VAL));
+                it->second = val[e];
+            }
+        }
+        std::vector<double> a_max((size_t)nb, 0.0);
+        for (int32_t I = 0; I < nb; ++I)
+            for (const auto &b : brow[(size_t)I])
+                if (b.first != I) a_max[(size_t)I] = std::max(a_max[(size_t)I], std::fabs(b.second));
+        parent.assign((size_t)nb, -1);
+        int32_t nc = 0;
+        for (int32_t I = 0; I < nb; ++I) {
+            if (parent[(size_t)I] != -1) continue;
+            parent[(size_t)I] = nc;
+            const double strong = weight * a_max[(size_t)I];
+            int64_t taken = 0;
+            for (const auto &b : brow[(size_t)I]) {
+                if (strong < std::fabs(b.second) && parent[(size_t)b.first] == -1) {
+                    parent[(size_t)b.first] = nc;
+                    ++taken;
+                }
+                if (taken >= bmax) break;
+            }
+            if (taken > 0) {
+                ++nc;
+                continue;
+            }
+            parent[(size_t)I] = -1;
+            double best = 0.0;
+            int32_t arg = -1;
+            for (const auto &b : brow[(size_t)I]) {
+                const double a = std::fabs(b.second);
+                if (a > best) {
+                    best = a;
+                    arg = b.first;
+                }
+            }
+            if (arg != -1) parent[(size_t)I] = parent[(size_t)arg];
+        }
+        for (int32_t I = 0; I < nb; ++I)
+            if (parent[(size_t)I] == -1) parent[(size_t)I] = nc++;
+        n_coarse = nc;
+    }
 
     // Agglomerate::create_transfer_matrix (:50-178): parent[i] = aggregate of row i
     // (-2: a row with a single stored value joins no aggregate). Row i not yet in an
@@ -83,6 +165,20 @@ public:
         std::vector<int32_t> parent;
         std::vector<double> a_max;
         int32_t nc = 0;
+        const int B = block_size();
+        if (B > 1) {
+            aggregate_blocks(n, rp.data(), col.data(), val.data(), B, component(), bmax_, weight_, parent, nc);
+            std::vector<int32_t> prp((size_t)n + 1), pcol((size_t)n);
+            std::vector<double> pval((size_t)n, 1.0);
+            for (int32_t i = 0; i < n; ++i) {
+                prp[(size_t)i] = i;
+                pcol[(size_t)i] = parent[(size_t)(i / B)] * B + i % B;
+            }
+            prp[(size_t)n] = n;
+            auto P = std::make_shared<HipCrsMatrix>();
+            P->from_host(A.context(), n, nc * B, prp.data(), pcol.data(), pval.data());
+            return P;
+        }
         aggregate(n, rp.data(), col.data(), val.data(), bmax_, weight_, parent, a_max, nc);
         std::vector<int32_t> prp((size_t)n + 1, 0), pcol;
         std::vector<double> pval;
@@ -103,6 +199,8 @@ public:
 private:
     int64_t bmax_ = 3;
     double weight_ = 1.0 / 3;
+    int block_size_ = 1;
+    int component_ = -1;
 };
 
 class AlgebraicMultigrid final : public IterativeSolver<HipCrsMatrix, HipVector> {

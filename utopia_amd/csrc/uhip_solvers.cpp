@@ -94,12 +94,14 @@ std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s, bool with_transfers 
     return mg;
 }
 
-// AlgebraicMultigrid (n_levels, Agglomerate bmax / weight) over a Multigrid with the
-// smoother and cycle parameters of the MG types
+// AlgebraicMultigrid (n_levels, block_size, Agglomerate bmax / weight / component)
+// over a Multigrid with the smoother and cycle parameters of the MG types
 std::shared_ptr<AlgebraicMultigrid> make_amg(uhip_solver_s *s) {
     auto agg = std::make_shared<Agglomerate>();
     agg->bmax((int64_t)param(s, "bmax", 3));
     agg->weight(param(s, "weight", 1.0 / 3));
+    agg->block_size((int)param(s, "block_size", 1));
+    agg->component((int)param(s, "component", -1));
     auto amg = std::make_shared<AlgebraicMultigrid>(make_multigrid(s, false), agg);
     amg->n_levels((int)param(s, "n_levels", 2));
     return amg;
