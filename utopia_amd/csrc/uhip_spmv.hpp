@@ -261,8 +261,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
 // consecutive rows of P consecutive planes (plane stride S2), lane l
 // row slab * P * S2 + z * S2 + seg * 64 + l (P = Dia7Planes). The -S2 mirror and x at +-S2 come from
 // the neighbouring plane in registers, the -1 mirror from a wave shuffle of the +1
-// values, so every stored value is read once and streams; the row sums keep the
-// column order (bit-identical to the CSR sum).
+// values; the +S1 diagonal is read through the cache, so its -S1 mirror (the same
+// array S1 rows back, read by the wave S1/64 segments earlier) hits L2, and the other
+// diagonals stream. The row sums keep the column order (bit-identical to the CSR sum).
 // planes per wave: 4 for FP32; 2 for FP64, whose 4-plane form holds 110+ VGPRs
 template <typename TA>
 struct Dia7Planes {
@@ -307,9 +308,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
             m[z] = (uint32_t)mask[min(r[z], xlast)];
             a0[z] = ld_stream(v0 + rc);
             a1[z] = ld_stream(v1 + rc);
-            aS1[z] = ld_stream(vS1 + rc);
+            aS1[z] = vS1[rc];
             aS2[z] = ld_stream(vS2 + rc);
-            mS1[z] = ld_stream(vS1 + max(rc - S1, 0));
+            mS1[z] = vS1[max(rc - S1, 0)];
             edge[z] = v1[max(rc - 1, 0)];  // the -1 mirror of lane 0 (the others shuffle it)
             ein[z] = epi.load(min(r[z], xlast));
             xc[z] = x[cl(r[z])];
