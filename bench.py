@@ -134,12 +134,22 @@ def cpu_baseline(n_full, gpu_iters, sample_n=128, seed=3):
     per_it = t_solve / its
     scale = (n_full ** 3) / (n ** 3)
     est = per_it * scale * gpu_iters
+    # the same solve on one core: a fixed number of iterations of the same hierarchy
+    k1 = 8
+    orc.set_threads(1)
+    t0 = time.perf_counter()
+    _, st1 = mg.cg_solve(b, atol=0.0, max_it=k1)
+    its1 = st1["iterates"] + 1
+    per_it1 = (time.perf_counter() - t0) / its1
+    orc.set_threads(threads)
     return {
         "value": est, "unit": "s", "cores": threads, "kind": "port",
+        "value_1core": per_it1 * scale * gpu_iters,
         "sample": (f"oracle FP64 MG-PCG (V(2,2) Chebyshev, 5 levels) on the {n}^3 variable-coefficient "
                    f"operator: {its} iterations in {t_solve:.2f} s ({per_it * 1e3:.1f} ms/iteration, setup "
                    f"{t_setup:.1f} s); scaled by the row ratio {scale:.0f} and the {gpu_iters} iterations of the "
-                   f"{n_full}^3 solve"),
+                   f"{n_full}^3 solve; value_1core: {its1} iterations of the same solve on one core "
+                   f"({per_it1 * 1e3:.0f} ms/iteration), scaled the same way"),
     }
 
 
