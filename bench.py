@@ -300,6 +300,65 @@ def host_info():
     return info
 
 
+def bench_shard(a):
+    """One GPU's share of the 256^3 variable-coefficient problem split over 8 GPUs
+    by z-slabs: the 256 x 256 x 32 local block (2.1M rows). Times the SpMV plans and
+    the mixed MG-PCG iteration at that shape; the halo exchange a multi-GPU run would
+    add is a model (one 256^2 plane each way per SpMV over one xGMI link), not a
+    measurement."""
+    import utopia_amd as ua
+    n, nz, parts = a.n, a.n // 8, 8
+    ctx = ua.Context(0)
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, nz, seed=a.seed)
+    N = A.shape[0]
+    kern = kernel_rooflines(ua, ctx, A, N)
+    for k, v in kern.items():
+        log(f"shard kernel {k}: {v['ms'] * 1e3:.1f} us, {v['gbs']:.0f} GB/s")
+    b = ua.gen_uniform(a.seed + 100, N)
+    interps = ua.mg_hierarchy(n, n, nz, 4)
+    atol = 1e-10 * float(np.linalg.norm(b))
+    bd = ua.DeviceArray.from_numpy(ctx, b)
+    xd = ua.DeviceArray(ctx, N)
+    L = ua.lib()
+    forms = {}
+    for form, kind in (("fp64", "multigrid"), ("mixed", "multigrid_mixed")):
+        ua.srand(1)
+        cg = ua.ConjugateGradient(ctx, kind, atol=atol, rtol=0, stol=0, max_it=1000, pre_smoothing_steps=2,
+                                  post_smoothing_steps=2, autotune=a.autotune)
+        cg.set_transfer_operators(interps)
+        cg.update(A)
+        times = []
+        for k in range(a.warmup + a.steps):
+            ua.check(L.uhip_vec_set_f64(ctx.h, N, 0.0, ctypes.c_void_p(xd.ptr)))
+            ctx.sync()
+            t0 = time.perf_counter()
+            cg.apply_device(bd, xd)
+            ctx.sync()
+            if k >= a.warmup:
+                times.append(time.perf_counter() - t0)
+        its = cg.status["iterates"] + 1
+        t = float(np.mean(times))
+        forms[form] = {"s": t, "iterations": its, "ms_per_iteration": t / its * 1e3}
+        log(f"shard {form}: {t * 1e3:.1f} ms, {its} iterations, {t / its * 1e6:.0f} us/iteration")
+        del cg
+    plane = n * n
+    link = 153e9  # one xGMI link, bytes/s
+    halo = {"bytes_per_spmv_fp64": 2 * 8 * plane, "modelled_us_fp64": 8 * plane / link * 1e6,
+            "bytes_per_spmv_fp32": 2 * 4 * plane, "modelled_us_fp32": 4 * plane / link * 1e6,
+            "note": "model: one plane each way to each of 2 neighbours on separate links, overlapped with the "
+                    "interior rows; not measured (one GPU)"}
+    print(json.dumps({
+        "metric": f"shard shape: {n}x{n}x{nz} slab (1 of {parts} z-slabs of the {n}^3 problem), mixed MG-PCG "
+                  "time per iteration and SpMV GB/s", "value": forms["mixed"]["ms_per_iteration"],
+        "unit": "ms/iteration", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": forms["mixed"]["s"] * 1e3, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f32 V-cycle / f64 CG", "data": "synthetic (seeded coefficients and b)",
+        "config": {"workload": f"MG-PCG 3D 7-point variable-coefficient {n}x{n}x{nz} (local block of an "
+                               f"{parts}-way z-slab split), 4 levels, rtol 1e-10", "rows": N, "nnz": A.nnz},
+        "forms": forms, "kernels": kern, "halo": halo}))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,9 +373,10 @@ def main(argv=None):
     ap.add_argument("--cpu-sample-n", type=int, default=128)
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV / streaming kernel timings (for counter collection runs)")
-    ap.add_argument("--workload", choices=["mgpcg256", "batched", "mprgp1024"], default="mgpcg256",
+    ap.add_argument("--workload", choices=["mgpcg256", "batched", "mprgp1024", "shard"], default="mgpcg256",
                     help="mgpcg256: the 256^3 MG-PCG line (default); batched: 64 systems x 8 RHS hipGraph MG-PCG; "
-                         "mprgp1024: box-constrained QP on the 1024^2 9-point operator")
+                         "mprgp1024: box-constrained QP on the 1024^2 9-point operator; shard: one GPU's "
+                         "256x256x32 slab of the 256^3 problem")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU sharding is a design "
@@ -327,6 +387,8 @@ def main(argv=None):
         return bench_batched(a)
     if a.workload == "mprgp1024":
         return bench_mprgp(a)
+    if a.workload == "shard":
+        return bench_shard(a)
     import utopia_amd as ua
     L = ua.lib()
     n = a.n
