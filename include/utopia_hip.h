@@ -346,6 +346,13 @@ int uhip_mprgp_alpha_f_f64(uhip_ctx ctx, int64_t n, const double *x, const doubl
 /* CG step (utopia_MPRGP.hpp:167-182): x -= alpha p; g -= alpha Ap; fi = get_fi(x,g); s[i_fiAp] = fi.Ap */
 int uhip_mprgp_cg_step_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *p, const double *Ap,
                            const double *lb, const double *ub, double *fi, double alpha, double *scalars, int i_fiAp);
+/* The CG step followed by the state of the new iterate in one pass (utopia_MPRGP.hpp:167-182
+ * then :238-243): x -= alpha p; g -= alpha Ap; fi, beta, gp = fi + beta of the new x, g;
+ * dots_d[0..3] = beta.beta, fi.fi, gp.gp, fi.Ap. The sums equal those of
+ * uhip_mprgp_cg_step_f64 and uhip_mprgp_state_f64 bit for bit. */
+int uhip_mprgp_cg_step_state_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *p, const double *Ap,
+                                 const double *lb, const double *ub, double *fi, double *beta, double *gp,
+                                 double alpha, double *dots_d);
 /* p = fi - (s[i_num]/s[i_den]) p */
 int uhip_mprgp_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *fi, const double *scalars, int i_num,
                            int i_den);

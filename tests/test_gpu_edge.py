@@ -92,3 +92,37 @@ def test_qp_with_coinciding_bounds(ctx):
     xo, so = orc.mprgp_solve(A, b, lb=c, ub=c, atol=1e-9, rtol=0, max_it=1000)
     assert m.status["iterates"] == so["iterates"]
     assert np.linalg.norm(x - xo) <= 1e-12 * np.linalg.norm(xo)
+
+
+def test_mprgp_fused_cg_step_state_matches_separate_passes(ctx):
+    """uhip_mprgp_cg_step_state_f64 = uhip_mprgp_cg_step_f64 then uhip_mprgp_state_f64,
+    bit for bit: the iterate, the gradients and the four dots."""
+    n, a = 70001, 0.37
+    rng = np.random.default_rng(5)
+    x0 = rng.uniform(-0.2, 0.3, n)
+    lb = np.where(rng.random(n) < 0.3, x0, -0.1)  # some components on the bound
+    ub = np.full(n, 0.25)
+    x0 = np.clip(x0, lb, ub)
+    g0, p, Ap = rng.normal(size=n), rng.normal(size=n), rng.normal(size=n)
+    L = ua.lib()
+    d = {k: ua.DeviceArray.from_numpy(ctx, v) for k, v in (("lb", lb), ("ub", ub), ("p", p), ("Ap", Ap))}
+    s = ctx.scalars
+    out = []
+    for fused in (False, True):
+        x, g = ua.DeviceArray.from_numpy(ctx, x0), ua.DeviceArray.from_numpy(ctx, g0)
+        fi, beta, gp = ua.DeviceArray(ctx, n), ua.DeviceArray(ctx, n), ua.DeviceArray(ctx, n)
+        if fused:
+            ua.check(L.uhip_mprgp_cg_step_state_f64(ctx.h, n, P(x), P(g), P(d["p"]), P(d["Ap"]), P(d["lb"]),
+                                                    P(d["ub"]), P(fi), P(beta), P(gp), ctypes.c_double(a),
+                                                    ctypes.c_void_p(s + 8 * 40)))
+            dots = ctx.read_scalars(s + 8 * 40, 4)
+        else:
+            ua.check(L.uhip_mprgp_cg_step_f64(ctx.h, n, P(x), P(g), P(d["p"]), P(d["Ap"]), P(d["lb"]), P(d["ub"]),
+                                              P(fi), ctypes.c_double(a), ctypes.c_void_p(s), 43))
+            ua.check(L.uhip_mprgp_state_f64(ctx.h, n, P(x), P(g), P(d["lb"]), P(d["ub"]), P(fi), P(beta), P(gp),
+                                            ctypes.c_void_p(s + 8 * 40)))
+            dots = ctx.read_scalars(s + 8 * 40, 4)
+        out.append([x.numpy(), g.numpy(), fi.numpy(), beta.numpy(), gp.numpy(), dots])
+    for u, v in zip(*out):
+        assert np.array_equal(u, v)
+    assert np.allclose(out[0][0], x0 - a * p, rtol=0, atol=1e-15)

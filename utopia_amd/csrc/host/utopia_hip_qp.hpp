@@ -76,6 +76,7 @@ public:
         A.context()->read_scalars(s + S_BB, 2, st);
         double beta_beta = st[0], fi_fi = st[1];
         while (!converged) {
+            bool state_done = false;
             if (beta_beta <= (gamma * gamma * fi_fi)) {
                 static_assert(S_GP == S_PAP + 1, "p.Ap and g.p in consecutive slots");
                 hip_check(uhip_spmv_dot2_f64(c, &A.raw(), p_.data(), Ap_.data(), g_.data(), s + S_PAP), "Ap, g.p");
@@ -86,10 +87,14 @@ public:
                 if (pAp <= 0.0) return true;  // negative curvature
                 const double alpha_cg = gp_dot / pAp;
                 if (alpha_cg <= alpha_f) {
-                    hip_check(uhip_mprgp_cg_step_f64(c, n, x.data(), g_.data(), p_.data(), Ap_.data(), lb, ub,
-                                                     fi_.data(), alpha_cg, s, S_FIAP),
+                    // the state of the new iterate comes with the step (it reads x and g only, so
+                    // it may precede the p update)
+                    static_assert(S_FIAP == S_BB + 3, "state dots and fi.Ap in consecutive slots");
+                    hip_check(uhip_mprgp_cg_step_state_f64(c, n, x.data(), g_.data(), p_.data(), Ap_.data(), lb, ub,
+                                                           fi_.data(), beta_.data(), gp_.data(), alpha_cg, s + S_BB),
                               "cg step");
                     hip_check(uhip_mprgp_pupdate_f64(c, n, p_.data(), fi_.data(), s, S_FIAP, S_PAP), "p update");
+                    state_done = true;
                 } else {
                     hip_check(uhip_mprgp_expand_f64(c, n, x.data(), g_.data(), p_.data(), Ap_.data(), lb, ub, alpha_f,
                                                     alpha_bar_),
@@ -110,9 +115,10 @@ public:
                                                     p_.data(), alpha_cg),
                           "proportioning");
             }
-            hip_check(uhip_mprgp_state_f64(c, n, x.data(), g_.data(), lb, ub, fi_.data(), beta_.data(), gp_.data(),
-                                           s + S_BB),
-                      "state");
+            if (!state_done)
+                hip_check(uhip_mprgp_state_f64(c, n, x.data(), g_.data(), lb, ub, fi_.data(), beta_.data(),
+                                               gp_.data(), s + S_BB),
+                          "state");
             A.context()->read_scalars(s + S_BB, 3, st);
             beta_beta = st[0];
             fi_fi = st[1];
@@ -126,7 +132,7 @@ public:
     }
 
 private:
-    enum { S_BB = 8, S_FF = 9, S_GG = 10, S_PAP = 11, S_GP = 12, S_AF = 13, S_FIAP = 14, S_BAB = 15, S_GB = 16 };
+    enum { S_BB = 8, S_FF = 9, S_GG = 10, S_FIAP = 11, S_PAP = 12, S_GP = 13, S_AF = 14, S_BAB = 15, S_GB = 16 };
 
     // MPRGP::power_method (utopia_MPRGP.hpp:351-387)
     double power_method(const HipCrsMatrix &A) {

@@ -150,6 +150,49 @@ __global__ __launch_bounds__(kBlock) void k_mprgp_cg_step(int64_t n, double *__r
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
+// CG step fused with the state that follows it (utopia_MPRGP.hpp:168-183 and the
+// get_fi / chopped-gradient pass at the end of the iteration): x -= a p, g -= a Ap,
+// then fi, beta, gp = fi + beta from the new x, g, and the dots beta.beta, fi.fi,
+// gp.gp, fi.Ap. The grid and each thread's entries are those of k_mprgp_cg_step and
+// k_mprgp_state, so the four sums are bit-identical to the two kernels'.
+__global__ __launch_bounds__(kBlock) void k_mprgp_cg_step_state(int64_t n, double *__restrict__ x,
+                                                                double *__restrict__ g, const double *__restrict__ p,
+                                                                const double *__restrict__ Ap,
+                                                                const double *__restrict__ lb,
+                                                                const double *__restrict__ ub, double *__restrict__ fi,
+                                                                double *__restrict__ beta, double *__restrict__ gp,
+                                                                double a, double *__restrict__ partials,
+                                                                int64_t stride) {
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double api = Ap[i];
+        const double xi = x[i] - a * p[i];
+        const double gi = g[i] - a * api;
+        x[i] = xi;
+        g[i] = gi;
+        const double li = lb[i], ui = ub[i];
+        const double f = free_grad(xi, gi, li, ui), b = chopped_grad(xi, gi, li, ui);
+        const double q = f + b;
+        fi[i] = f;
+        beta[i] = b;
+        gp[i] = q;
+        s0 += b * b;
+        s1 += f * f;
+        s2 += q * q;
+        s3 += f * api;
+    }
+    s0 = block_sum<kBlock>(s0);
+    s1 = block_sum<kBlock>(s1);
+    s2 = block_sum<kBlock>(s2);
+    s3 = block_sum<kBlock>(s3);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s0;
+        partials[stride + blockIdx.x] = s1;
+        partials[2 * stride + blockIdx.x] = s2;
+        partials[3 * stride + blockIdx.x] = s3;
+    }
+}
+
 // p = fi - (s[inum]/s[iden]) p
 __global__ __launch_bounds__(kBlock) void k_mprgp_pupdate(int64_t n, double *__restrict__ p,
                                                           const double *__restrict__ fi, const double *__restrict__ s,
@@ -409,6 +452,19 @@ int uhip_mprgp_cg_step_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const 
                        ctx->partials);
     UHIP_CHECK_LAUNCH();
     return finish_sums(ctx, 1, gr, ctx->max_partials, scalars + i_fiAp, 0u);
+}
+
+int uhip_mprgp_cg_step_state_f64(uhip_ctx ctx, int64_t n, double *x, double *g, const double *p, const double *Ap,
+                                 const double *lb, const double *ub, double *fi, double *beta, double *gp,
+                                 double alpha, double *dots_d) {
+    UHIP_REQUIRE(ctx && x && g && p && Ap && lb && ub && fi && beta && gp && dots_d,
+                 "uhip_mprgp_cg_step_state_f64: null argument");
+    const int gr = grid_for(n, 4, kMaxRedBlocks);
+    UHIP_TRY(ensure_partials(ctx, gr));
+    hipLaunchKernelGGL(k_mprgp_cg_step_state, dim3(gr), dim3(kBlock), 0, ctx->stream, n, x, g, p, Ap, lb, ub, fi, beta,
+                       gp, alpha, ctx->partials, ctx->max_partials);
+    UHIP_CHECK_LAUNCH();
+    return finish_sums(ctx, 4, gr, ctx->max_partials, dots_d, 0u);
 }
 
 int uhip_mprgp_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *fi, const double *scalars, int i_num,
