@@ -1,7 +1,7 @@
 """Solver-level parity on the GPU (through the solver C ABI) against the CPU
 oracle on the same seeded inputs.
 
-Bars (BASELINE workloads): iteration counts within +-1 of the oracle, FP64
+Bars (the benchmark workloads): iteration counts within +-1 of the oracle, FP64
 solution relative error <= 1e-10 for MG-PCG at rtol 1e-10 (1e-6 for the
 Jacobi-PCG run at rtol 1e-8, whose ~400 iterations amplify dot-product
 rounding), mixed precision within 1e-8 of the FP64 solution, QP active sets
