@@ -564,6 +564,9 @@ def test_level_plan_formats_reported(ctx):
         assert cg.info(f"format_{w}_{f}") == ua.FMT_VI and cg.info(f"format32_{w}_{f}") == ua.FMT_VI
     with pytest.raises(ua.HipError):
         cg.info("format_R_0")  # the coarsest level has no transfers
+    for bad in ("level_nnz_99", "level_rows_-1", "eig_max_x", "transfer_nnz_7", "format_A_"):
+        with pytest.raises(ua.HipError):
+            cg.info(bad)  # an error code, not an exception through the C ABI
     plain = ua.ConjugateGradient(ctx, "multigrid", atol=1e-8, rtol=0, stol=0, max_it=50)
     plain.set_transfer_operators(ua.mg_hierarchy(n, n, n, 3))
     plain.update(A)

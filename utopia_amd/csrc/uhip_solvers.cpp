@@ -422,9 +422,24 @@ int uhip_solver_get_status(uhip_solver s, int64_t *iterates, double *gradient_no
     return UHIP_OK;
 }
 
+static int get_info(uhip_solver s, const std::string &k, double *value);
+
 int uhip_solver_get_info(uhip_solver s, const char *key, double *value) {
     if (!s || !key || !value) return UHIP_ERR_INVALID;
-    const std::string k(key);
+    try {
+        return get_info(s, std::string(key), value);
+    } catch (const std::exception &e) {  // malformed level index, transfer index out of range
+        return fail(e);
+    }
+}
+
+static int get_info(uhip_solver s, const std::string &k, double *value) {
+    auto level = [&](size_t at) {
+        const int l = std::stoi(k.substr(at));
+        if (!s->mg || l < 0 || l >= s->mg->n_levels())
+            throw HipError(UHIP_ERR_INVALID, "get_info " + k + ": level out of range");
+        return l;
+    };
     if (k == "setup_seconds") *value = s->setup_seconds;
     else if (k == "solve_seconds") *value = s->seconds;
     else if (k == "total_iterations") *value = (double)s->total_iterations;
@@ -437,10 +452,10 @@ int uhip_solver_get_info(uhip_solver s, const char *key, double *value) {
     else if (k.rfind("transfer_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->interpolations().at((size_t)std::stoi(k.substr(13)))->nnz();
     else if (k.rfind("transfer_rows_", 0) == 0 && s->mg) *value = (double)s->mg->interpolations().at((size_t)std::stoi(k.substr(14)))->rows();
     else if (k.rfind("transfer_cols_", 0) == 0 && s->mg) *value = (double)s->mg->interpolations().at((size_t)std::stoi(k.substr(14)))->cols();
-    else if (k.rfind("eig_max_", 0) == 0 && s->mg) *value = s->mg->level_eig_max(std::stoi(k.substr(8)));
-    else if (k.rfind("eig_min_", 0) == 0 && s->mg) *value = s->mg->level_eig_min(std::stoi(k.substr(8)));
-    else if (k.rfind("level_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(10))).nnz();
-    else if (k.rfind("level_rows_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(std::stoi(k.substr(11))).rows();
+    else if (k.rfind("eig_max_", 0) == 0 && s->mg) *value = s->mg->level_eig_max(level(8));
+    else if (k.rfind("eig_min_", 0) == 0 && s->mg) *value = s->mg->level_eig_min(level(8));
+    else if (k.rfind("level_nnz_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(level(10)).nnz();
+    else if (k.rfind("level_rows_", 0) == 0 && s->mg) *value = (double)s->mg->level_operator(level(11)).rows();
     else if ((k.rfind("format_", 0) == 0 || k.rfind("format32_", 0) == 0) && s->mg && k.size() >= 10) {
         // format_{A,R,P}_<l> of the FP64 V-cycle, format32_{A,R,P}_<l> of the FP32 one (mixed)
         const bool f32 = k.rfind("format32_", 0) == 0;
