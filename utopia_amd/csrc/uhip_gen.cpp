@@ -3,6 +3,7 @@
 // consumes exactly these arrays.
 #include <cmath>
 #include <cstdint>
+#include <new>
 #include <vector>
 
 #include "../../include/utopia_hip.h"
@@ -48,8 +49,20 @@ int64_t uhip_gen_nnz(int kind, int32_t nx, int32_t ny, int32_t nz) {
     }
 }
 
+static int gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift, int32_t *rowptr,
+                       int32_t *col, double *val);
+
 int uhip_gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift, int32_t *rowptr,
                      int32_t *col, double *val) {
+    try {
+        return gen_stencil(kind, nx, ny, nz, seed, shift, rowptr, col, val);
+    } catch (const std::bad_alloc &) {  // edge-coefficient arrays of a huge grid
+        return UHIP_ERR_ALLOC;
+    }
+}
+
+static int gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift, int32_t *rowptr,
+                       int32_t *col, double *val) {
     if (nx <= 0 || ny <= 0 || nz <= 0 || !rowptr || !col || !val) return UHIP_ERR_INVALID;
     int64_t e = 0;
     rowptr[0] = 0;
