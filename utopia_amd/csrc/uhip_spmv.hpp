@@ -311,7 +311,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
             aS1[z] = vS1[rc];
             aS2[z] = ld_stream(vS2 + rc);
             mS1[z] = vS1[max(rc - S1, 0)];
-            edge[z] = v1[max(rc - 1, 0)];  // the -1 mirror of lane 0 (the others shuffle it)
+            // the -1 mirror of lane 0 (the others shuffle it); loaded by lane 0 alone, since a
+            // whole-wave load would fetch again the lines the +1 stream just read
+            edge[z] = lane == 0 ? v1[max(rc - 1, 0)] : TA(0);
             ein[z] = epi.load(min(r[z], xlast));
             xc[z] = x[cl(r[z])];
             xm1[z] = x[cl(r[z] - 1)];
