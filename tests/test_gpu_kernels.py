@@ -167,8 +167,8 @@ def test_spmv_dia_bit_exact(ctx, name, dt):
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.format == ua.FMT_DIA
     if name in DIA_SYMMETRIC:
-        # the plane-blocked 7-point kernel (2) serves FP32 operators; FP64 ones keep 1
-        assert D.m.dia_sym == (min(DIA_SYMMETRIC[name], 1) if dt is np.float64 else DIA_SYMMETRIC[name])
+        # 2: plane-blocked storage (in FP64 the plain product runs the symmetric kernel on it)
+        assert D.m.dia_sym == DIA_SYMMETRIC[name]
     x = ua.gen_uniform(17, n)
     if dt is np.float32:
         x = x.astype(np.float32)
@@ -232,7 +232,9 @@ def test_fused_epilogues_both_formats_with_dot(ctx, fmt, dims):
     D32 = ua.DeviceCsr(ctx, A, dtype=np.float32, fmt=fmt)
     if dims == (16, 16, 13):
         # plane-blocked 7-point kernel, with a partial last slab
-        assert D32.m.dia_sym == 2 and D.m.dia_sym == 1
+        # plane-blocked storage in both precisions: FP64 runs the dot products with the
+        # plane-blocked kernel and the other epilogues with the symmetric one
+        assert D32.m.dia_sym == 2 and D.m.dia_sym == 2
     x32, b32 = x.astype(np.float32), b.astype(np.float32)
     x32d, b32d = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray.from_numpy(ctx, b32)
     out32 = ua.DeviceArray(ctx, n, np.float32)

@@ -313,11 +313,11 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
         }
     }
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
-    // FP32 symmetric 7-point operators with the largest offset a multiple of 64 take the
-    // plane-blocked kernel. In FP64 (2 planes per wave) the plain product gains 4%
-    // but the fused-epilogue forms lose more, and the FP64 solve slows by 6%.
+    // symmetric 7-point operators with the largest offset a multiple of 64 take the
+    // plane-blocked storage (the same arrays; in FP64 only the dot-carrying products run
+    // the plane-blocked kernel, see launch_rowblock)
     const int planes = dia7_planes(A->dtype);
-    if (sym && A->dtype == UHIP_F32 && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] && offs[6] % 64 == 0 &&
+    if (sym && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] && offs[6] % 64 == 0 &&
         (int64_t)n + (int64_t)planes * offs[6] < INT32_MAX)
         sym = 2;
     A->dia_off = table;  // first nd slots hold the sorted offsets

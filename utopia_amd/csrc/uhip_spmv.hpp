@@ -529,9 +529,19 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
 #define UHIP_DIA_LAUNCH(TM, SYM)                                                                                  \
     hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, TM, SYM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, \
                        A->dia_off, v, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
-        if (A->dia_sym == 2) {
+        // FP64 plane-blocked storage: every product with dots runs the plane-blocked kernel
+        // (its grid is nparts, the partial sums the finish reads); the others (smoothing
+        // steps, residuals, transfers) run the symmetric kernel over the same arrays, which
+        // is faster for them in FP64 and writes no partial sums.
+        const bool blocked = A->dia_sym == 2 && (sizeof(TA) == 4 || Epi::kDots > 0);
+        if (blocked) {
             hipLaunchKernelGGL((k_spmv_dia7z<TAcc, Epi, TA, TX>), g, blk, 0, stream_of(ctx), A->nrows, A->dia_off, v,
                                (const uint8_t *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials);
+        } else if (A->dia_sym == 2) {
+            const dim3 gs((A->nrows + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype));
+            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint8_t, true>), gs, blk, 0, stream_of(ctx), A->nrows,
+                               A->ndiag, A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, ctx->partials,
+                               ctx->max_partials);
         } else if (A->ndiag <= 8) {
             if (A->dia_sym) UHIP_DIA_LAUNCH(uint8_t, true);
             else UHIP_DIA_LAUNCH(uint8_t, false);
