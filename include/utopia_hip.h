@@ -183,7 +183,8 @@ typedef struct uhip_csr {
     int32_t dia_sym;    /* DIA of a bitwise-symmetric matrix: dia_val holds only the offsets >= 0
                          * (ndiag/2 + 1 arrays); offset -o of row r is read as offset +o of row r - o.
                          * 2: the same storage of a 7-point operator whose largest offset is a multiple
-                         * of 64, run by the plane-blocked kernel */
+                         * of 64, run by the plane-blocked kernel (FP64: the products with dots; its
+                         * other products run the symmetric kernel on the same arrays) */
     int32_t spmm_kernel; /* multi-RHS products (uhip_spmm*): 0 by row length (wavefront per row when
                           * nnz >= 32 rows), 1 one thread per row, 2 one wavefront per row */
     int32_t *vi_base;   /* VI: per-row base column (the row's first column; 0 for an empty row) */
