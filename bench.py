@@ -81,8 +81,7 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
             nbytes = dia_bytes(n_rows, D.m.ndiag, vb, D.m.dia_sym)
             stored = f"{D.m.ndiag // 2 + 1} of {D.m.ndiag} offsets stored (symmetric)" if D.m.dia_sym else \
                 f"{D.m.ndiag} offsets"
-            # FP64 plane-blocked storage runs the plain product with the symmetric kernel
-            pb = D.m.dia_sym == 2 and dt is np.float32
+            pb = D.m.dia_sym == 2
             kname = "k_spmv_dia7z" if pb else "k_spmv_dia"
             blocked = ", plane-blocked" if pb else ""
             label = f"{kname}<{ty}, EpiStore<{ty}>> (DIA, {stored}{blocked}, per-row mask)"

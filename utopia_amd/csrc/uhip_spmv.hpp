@@ -496,6 +496,17 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
     }
 }
 
+template <typename TAcc, typename TY>
+struct EpiStore;
+template <class Epi>
+struct IsEpiStore {
+    static constexpr bool value = false;
+};
+template <typename TAcc, typename TY>
+struct IsEpiStore<EpiStore<TAcc, TY>> {
+    static constexpr bool value = true;
+};
+
 // Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
 // Matrix values have the accumulation type TAcc.
 template <typename TAcc, typename TX, class Epi>
@@ -529,11 +540,11 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
 #define UHIP_DIA_LAUNCH(TM, SYM)                                                                                  \
     hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, TM, SYM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, \
                        A->dia_off, v, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
-        // FP64 plane-blocked storage: every product with dots runs the plane-blocked kernel
-        // (its grid is nparts, the partial sums the finish reads); the others (smoothing
-        // steps, residuals, transfers) run the symmetric kernel over the same arrays, which
-        // is faster for them in FP64 and writes no partial sums.
-        const bool blocked = A->dia_sym == 2 && (sizeof(TA) == 4 || Epi::kDots > 0);
+        // FP64 plane-blocked storage: every product with dots and the plain product run the
+        // plane-blocked kernel (its grid is nparts, the partial sums the finish reads); the
+        // others (smoothing steps, residuals) run the symmetric kernel over the same arrays,
+        // which is faster for them in FP64 and writes no partial sums.
+        const bool blocked = A->dia_sym == 2 && (sizeof(TA) == 4 || Epi::kDots > 0 || IsEpiStore<Epi>::value);
         if (blocked) {
             hipLaunchKernelGGL((k_spmv_dia7z<TAcc, Epi, TA, TX>), g, blk, 0, stream_of(ctx), A->nrows, A->dia_off, v,
                                (const uint8_t *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials);
