@@ -70,6 +70,16 @@ int uhip_event_timer_stop(uhip_ctx ctx, void *ev_pair, float *ms);
 /* Pre-size the reduction workspace (per_slot partial sums per dot) so that a
  * later captured sequence allocates nothing. */
 int uhip_ctx_reserve(uhip_ctx ctx, int64_t per_slot);
+/* Generation of the reduction workspace: changes whenever it is reallocated (a reservation
+ * or an operation needing more partial sums). A graph captured at one generation holds the
+ * workspace address of that generation and must be captured again after a change. */
+int64_t uhip_ctx_workspace_generation(uhip_ctx ctx);
+/* gate_d (device int32, NULL = none): while set, the multi-RHS / batched kernels launched on
+ * this context (uhip_spmm, uhip_spmm_cheb_step, uhip_axpy_seg, uhip_segdot, uhip_bcg_step)
+ * return without work whenever *gate_d reads 0 when they start. The batched CG points it at
+ * its active-pair count, so a captured run of iterations past convergence costs only its
+ * launches. No reference analogue. */
+int uhip_ctx_set_gate(uhip_ctx ctx, const int32_t *gate_d);
 /* hipGraph capture of the work issued on the context stream between begin and
  * end; launch replays it on the same stream. No reference analogue (the
  * reference drives every iteration from the host). */
@@ -92,6 +102,9 @@ int uhip_graph_destroy(void *exec);
  *   emul     VecPointwiseMult  (.cpp:588)
  *   ediv     VecPointwiseDivide
  *   pseudo_inv  ElementWisePseudoInverse (utopia_core/utopia_ElementWisePseudoInverse.hpp:20-56)
+ *   emin/emax   VecPointwiseMin/Max z = a < b ? a : b / a < b ? b : a  (e_min/e_max, .cpp:597-603)
+ *   reciprocal  VecReciprocal (1/x where x != 0, zeros kept), then *num unless num == 1 (.hpp:711-720)
+ *   replace     x_i = to where x_i == from (PointJacobi::check_indef, solvers/smoothers/utopia_PointJacobi.hpp:116-127)
  *   dot      VecDot            (.cpp:59)     result in device memory
  *   norm2    VecNorm NORM_2    (.hpp:301-305) result (already sqrt) in device memory
  *   dots     EvalDots split-phase VecDotBegin/End (backend/petsc/eval/utopia_petsc_EvalDotVecVecs.cpp:57-92),
@@ -112,6 +125,10 @@ int uhip_graph_destroy(void *exec);
     int uhip_vec_emul_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z);                   \
     int uhip_vec_ediv_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z);                   \
     int uhip_vec_pseudo_inv_##SUF(uhip_ctx ctx, int64_t n, const T *x, T *y, T tol);                  \
+    int uhip_vec_emin_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z);                   \
+    int uhip_vec_emax_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z);                   \
+    int uhip_vec_reciprocal_##SUF(uhip_ctx ctx, int64_t n, T num, const T *x, T *y);                  \
+    int uhip_vec_replace_##SUF(uhip_ctx ctx, int64_t n, T from, T to, T *x);                          \
     int uhip_vec_dots_##SUF(uhip_ctx ctx, int64_t n, int k, const T *const *xs, const T *const *ys,  \
                             double *result_d);                                                        \
     int uhip_vec_dot_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, double *result_d);       \
@@ -122,7 +139,8 @@ UHIP_DECLARE_BLAS1(f64, double)
 UHIP_DECLARE_BLAS1(f32, float)
 int uhip_vec_cast_f64_f32(uhip_ctx ctx, int64_t n, const double *x, float *y);
 int uhip_vec_cast_f32_f64(uhip_ctx ctx, int64_t n, const float *x, double *y);
-/* out = xin + a*p with dtype conversion (xin NULL = 0): the first Chebyshev step
+/* out = xin + a*p with dtype conversion (xin NULL = 0; a == 0 adds nothing, as VecAXPY with
+ * alpha 0, so the call copies / casts xin exactly): the first Chebyshev step
  * x = x + alpha*p without a following SpMV (Chebyshev3level.hpp:154), and the
  * FP32<->FP64 boundary of the mixed-precision V-cycle. */
 int uhip_vec_axpy_cast(uhip_ctx ctx, int64_t n, double a, const void *p, int p_dtype, const void *xin, int x_dtype,
@@ -257,6 +275,14 @@ int uhip_spmv_masked(uhip_ctx ctx, const uhip_csr *A, const void *x, const void 
 /* y += A x                         IPRTransfer::interpolate + c += c_I (utopia_Multigrid.hpp:317,343) */
 int uhip_spmv_add(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y);
 
+/* One Jacobi sweep fused with its SpMV, r = b - A xin rounded, then
+ * add != 0: out = xin + d .* r   Jacobi::sweep (solvers/smoothers/utopia_Jacobi.hpp:132-138), d = pseudo_inv(diag A, 0)
+ * add == 0: out = d .* r         PointJacobi::sweep (solvers/smoothers/utopia_PointJacobi.hpp:132-142), A = LU
+ * xin and d have A->dtype; b (b_dtype) and out (out_dtype) may be FP64 on an FP32 level. out must
+ * not alias xin, b or d (other rows still read xin). */
+int uhip_spmv_jacobi(uhip_ctx ctx, const uhip_csr *A, const void *xin, const void *b, int b_dtype, const void *d,
+                     void *out, int out_dtype, int add);
+
 /* One fused step of Chebyshev3level::solve (solvers/smoothers/utopia_Chebyshev3level.hpp:138-175):
  * with s = A pprev:  r_k = rprev + a_prev*s ; p_k = -r_k + beta*pprev ;
  * xout = xin + a_prev*pprev (+ a_cur*p_k when last). rprev NULL: rprev = -pprev.
@@ -283,6 +309,11 @@ int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *
 
 /* Diagonal:  d_i = A_ii (0 if absent)   MatGetDiagonal (utopia_petsc_Matrix.cpp:744-756) */
 int uhip_csr_diag(uhip_ctx ctx, const uhip_csr *A, void *d);
+/* A_ii += alpha * d_i where row i stores its diagonal (values only; re-plan a DIA / SELL-64 / VI
+ * matrix after the call)   PetscMatrix::shift_diag -> MatDiagonalSet ADD_VALUES
+ * (backend/petsc/utopia_petsc_Matrix.cpp:1783-1788); with alpha = -1 and d = diag(A) it forms
+ * PointJacobi's LU = A - D (solvers/smoothers/utopia_PointJacobi.hpp:89-96) */
+int uhip_csr_shift_diag(uhip_ctx ctx, uhip_csr *A, const void *d, double alpha);
 /* A_ii += 1 where |A_ii| < 1e-12  LinearMultiLevel::fix_semidefinite_operator
  * (solvers/multilevel/utopia_LinearMultiLevel.hpp:89-113) */
 int uhip_csr_fix_semidefinite(uhip_ctx ctx, uhip_csr *A);
@@ -324,10 +355,6 @@ int uhip_dense_cholesky(uhip_ctx ctx, int32_t n, double *A, int32_t lda, int *in
 int uhip_dense_cholesky_inverse(uhip_ctx ctx, int32_t n, const double *L, int32_t lda, double *Ainv);
 /* y = M x (row-major n x n, FP64), x/y of dtype x_dtype / y_dtype */
 int uhip_dense_gemv(uhip_ctx ctx, int32_t n, const double *M, const void *x, int x_dtype, void *y, int y_dtype);
-/* y = M x with an FP32 copy of M (the coarse solve of the FP32 V-cycle: half the
- * bytes of the FP64 inverse; products summed in FP64 as uhip_dense_gemv). No
- * reference analogue (PETSc's coarse solve is a factorisation in one precision). */
-int uhip_dense_gemv_m32(uhip_ctx ctx, int32_t n, const float *M, const float *x, float *y);
 
 /* ------------------------------------------------------------------------
  * Box-constrained QP kernels (FP64).
@@ -428,7 +455,9 @@ int uhip_solver_create_by_name(uhip_ctx ctx, const char *backend, const char *ty
 int uhip_solver_destroy(uhip_solver s);
 /* read(Input&)-style parameters: atol rtol stol max_it verbose reset_initial_guess refine
  * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
- * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour) scale_max_eig scale_min_eig power_method_tol
+ * cycle_type (1 MULTIPLICATIVE_CYCLE, 3 FULL_CYCLE; utopia_MultiLevelBase.hpp:12-14) v_cycle_repetition
+ * use_line_search (utopia_Multigrid.hpp:324-341)
+ * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour, 3 PointJacobi) scale_max_eig scale_min_eig power_method_tol
  * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each coarse_space_it
  * active_set_tol; AlgebraicMultigrid: n_levels bmax weight block_size (2-4: BlockAgglomerate,
  * utopia_AlgebraicMultigrid.hpp:87-103) component (block weight entry, default block_size - 1).
@@ -532,7 +561,9 @@ typedef struct uhip_bsolver_s *uhip_bsolver;
 int uhip_bsolver_create(uhip_ctx ctx, int k, int mixed, uhip_bsolver *out);
 int uhip_bsolver_destroy(uhip_bsolver s);
 /* rtol max_it pre_smoothing_steps post_smoothing_steps scale_max_eig scale_min_eig
- * power_method_tol power_method_max_it use_graph graph_iterations; autotune (reps > 0: at setup
+ * power_method_tol power_method_max_it use_graph graph_iterations (CG iterations per captured graph,
+ * default 16; kernels gated on the active count, see uhip_ctx_set_gate) workspace_reserve (pre-size
+ * the reduction workspace; a captured graph is captured again after it grows); autotune (reps > 0: at setup
  * the multi-RHS kernel of every level operator, transfer and coarse block inverse is chosen by
  * uhip_csr_autotune_spmm) */
 int uhip_bsolver_set_param(uhip_bsolver s, const char *key, double value);
@@ -547,7 +578,7 @@ int uhip_bsolver_solve(uhip_bsolver s, const double *B_d, double *X_d, int64_t *
 int uhip_bsolver_get_conv(uhip_bsolver s, int32_t *conv_it, int32_t *flags);
 /* dL/dA on the pattern of the finest operator from lambda and x (device, [n x k]) */
 int uhip_bsolver_pattern_grad(uhip_bsolver s, const double *lambda_d, const double *x_d, double *grad_d);
-/* setup_seconds solve_seconds iterations graph_replays n_levels eig_max_<l>_<s> */
+/* setup_seconds solve_seconds iterations graph_replays graph_captures n_levels eig_max_<l>_<s> */
 int uhip_bsolver_get_info(uhip_bsolver s, const char *key, double *value);
 int uhip_bsolver_set_level_eig(uhip_bsolver s, int l, int32_t sys, double eig_max);
 

@@ -472,7 +472,10 @@ struct or_mg {
     double *dinv[OR_MAXL];
     double *chol;
     int smoother, pre, post, mg_type, masks, fix_semi;
-    double *r[OR_MAXL], *c[OR_MAXL], *cI[OR_MAXL], *rR[OR_MAXL];
+    int cycle_type, line_search; /* MultiLevelBase::cycle_type (1 or 3), Multigrid::use_line_search */
+    int64_t v_cycle_repetition;
+    or_csr LU[OR_MAXL];          /* PointJacobi: A - D (smoother 3) */
+    double *r[OR_MAXL], *c[OR_MAXL], *cI[OR_MAXL], *rR[OR_MAXL], *f[OR_MAXL];
     int32_t *color[OR_MAXL]; /* multicolour Gauss-Seidel: colour of each row (smoother 2) */
     int32_t ncolors[OR_MAXL];
 };
@@ -492,7 +495,16 @@ or_mg *or_mg_create(int nlevels, const or_csr *const *interp, int smoother, int 
     mg->mg_type = mg_type;
     mg->masks = masks;
     mg->fix_semi = fix_semidefinite;
+    mg->cycle_type = 1;          /* MULTIPLICATIVE_CYCLE (utopia_MultiLevelBase.hpp:39) */
+    mg->v_cycle_repetition = 10; /* (utopia_MultiLevelBase.hpp:40) */
+    mg->line_search = 0;         /* (utopia_Multigrid.hpp:74) */
     return mg;
+}
+
+void or_mg_set_cycle(or_mg *mg, int cycle_type, int64_t v_cycle_repetition, int line_search) {
+    mg->cycle_type = cycle_type;
+    mg->v_cycle_repetition = v_cycle_repetition;
+    mg->line_search = line_search;
 }
 
 int or_mg_nlevels(const or_mg *mg) { return mg->L; }
@@ -665,6 +677,24 @@ int or_mg_update(or_mg *mg, const or_csr *A) {
             mg->dinv[l] = vnew(n);
             diag_of(&mg->A[l], mg->dinv[l]);
             for (int64_t i = 0; i < n; ++i) mg->dinv[l][i] = mg->dinv[l][i] != 0.0 ? 1.0 / mg->dinv[l][i] : 0.0;
+        } else if (mg->smoother == 3) {
+            /* PointJacobi::update (solvers/smoothers/utopia_PointJacobi.hpp:84-101): d = diag(A);
+             * LU = A - diag(d) (MatAXPY: the diagonal entries stay, as A_ii - d_i = 0);
+             * d_inv = 1. / d evaluated as VecReciprocal (zeros kept,
+             * backend/petsc/utopia_petsc_Vector.hpp:711-720); check_indef: zeros become ones */
+            free(mg->dinv[l]);
+            mg->dinv[l] = vnew(n);
+            diag_of(&mg->A[l], mg->dinv[l]);
+            or_csr_free(&mg->LU[l]);
+            csr_copy(&mg->A[l], &mg->LU[l]);
+            for (int32_t i = 0; i < mg->LU[l].nrows; ++i)
+                for (int32_t j = mg->LU[l].rowptr[i]; j < mg->LU[l].rowptr[i + 1]; ++j)
+                    if (mg->LU[l].col[j] == i) mg->LU[l].val[j] = mg->LU[l].val[j] + (-1.0 * mg->dinv[l][i]);
+            for (int64_t i = 0; i < n; ++i) {
+                const double d = mg->dinv[l][i];
+                mg->dinv[l][i] = d != 0.0 ? 1.0 / d : d;
+                if (mg->dinv[l][i] == 0) mg->dinv[l][i] = 1;
+            }
         } else {
             /* Jacobi::update (solvers/smoothers/utopia_Jacobi.hpp:99-110) */
             free(mg->dinv[l]);
@@ -686,12 +716,27 @@ int or_mg_update(or_mg *mg, const or_csr *A) {
         free(mg->c[l]);
         free(mg->cI[l]);
         free(mg->rR[l]);
+        free(mg->f[l]);
         mg->r[l] = vnew(n);
         mg->c[l] = vnew(n);
         mg->cI[l] = vnew(n);
         mg->rR[l] = vnew(n);
+        mg->f[l] = vnew(n);
     }
     return info;
+}
+
+/* PointJacobi::smooth -> sweep (solvers/smoothers/utopia_PointJacobi.hpp:61-66,132-142):
+ * r = rhs - LU x; x = d_inv .* r */
+static void point_jacobi_smooth(const or_csr *LU, const double *dinv, const double *rhs, double *x, int sweeps) {
+    const int64_t n = LU->nrows;
+    double *r = vnew(n);
+    for (int s = 0; s < sweeps; ++s) {
+        or_spmv(LU, x, r);
+        for (int64_t i = 0; i < n; ++i) r[i] = rhs[i] - r[i];
+        for (int64_t i = 0; i < n; ++i) x[i] = dinv[i] * r[i];
+    }
+    free(r);
 }
 
 /* Jacobi::smooth (solvers/smoothers/utopia_Jacobi.hpp:89-140): r = rhs - A x; x += d_inv .* r */
@@ -706,11 +751,42 @@ static void jacobi_smooth(const or_csr *A, const double *dinv, const double *rhs
     free(r);
 }
 
+/* Jacobi / PointJacobi smoothing on a matrix (update + smooth), for pins */
+void or_jacobi_smooth(const or_csr *A, const double *rhs, double *x, int sweeps) {
+    const int64_t n = A->nrows;
+    double *d = vnew(n);
+    diag_of(A, d);
+    pseudo_inv(n, d, 0.0);
+    jacobi_smooth(A, d, rhs, x, sweeps);
+    free(d);
+}
+
+void or_point_jacobi_smooth(const or_csr *A, const double *rhs, double *x, int sweeps) {
+    const int64_t n = A->nrows;
+    double *d = vnew(n);
+    or_csr LU;
+    diag_of(A, d);
+    csr_copy(A, &LU);
+    for (int32_t i = 0; i < LU.nrows; ++i)
+        for (int32_t j = LU.rowptr[i]; j < LU.rowptr[i + 1]; ++j)
+            if (LU.col[j] == i) LU.val[j] = LU.val[j] + (-1.0 * d[i]);
+    for (int64_t i = 0; i < n; ++i) {
+        const double v = d[i];
+        d[i] = v != 0.0 ? 1.0 / v : v;
+        if (d[i] == 0) d[i] = 1;
+    }
+    point_jacobi_smooth(&LU, d, rhs, x, sweeps);
+    or_csr_free(&LU);
+    free(d);
+}
+
 static void smoothing(or_mg *mg, int l, const double *rhs, double *x, int nu) {
     if (mg->smoother == 0)
         or_chebyshev_smooth(&mg->A[l], rhs, x, nu, mg->eig_max[l], mg->eig_min[l]);
     else if (mg->smoother == 2)
         or_gs_smooth(&mg->A[l], mg->dinv[l], mg->color[l], mg->ncolors[l], rhs, x, nu, 1);
+    else if (mg->smoother == 3)
+        point_jacobi_smooth(&mg->LU[l], mg->dinv[l], rhs, x, nu);
     else
         jacobi_smooth(&mg->A[l], mg->dinv[l], rhs, x, nu);
 }
@@ -735,9 +811,47 @@ static void standard_cycle(or_mg *mg, int l) {
         }
         standard_cycle(mg, l - 1);
         or_spmv(&mg->I[l - 1], mg->c[l - 1], cI);
-        for (int64_t i = 0; i < n; ++i) c[i] = c[i] + cI[i];
+        if (mg->line_search) {
+            /* use_line_search, symmetric form (utopia_Multigrid.hpp:324-341):
+             * alpha = dot(c_I, r_R) / dot(A c_I, c_I); c += alpha c_I, or c += c_I if alpha <= 0 */
+            double *Ac = vnew(n);
+            or_spmv(&mg->A[l], cI, Ac);
+            const double alpha = or_dot(n, cI, rR) / or_dot(n, Ac, cI);
+            free(Ac);
+            if (alpha <= 0.)
+                for (int64_t i = 0; i < n; ++i) c[i] = c[i] + cI[i];
+            else
+                for (int64_t i = 0; i < n; ++i) c[i] = c[i] + alpha * cI[i];
+        } else {
+            for (int64_t i = 0; i < n; ++i) c[i] = c[i] + cI[i];
+        }
         smoothing(mg, l, r, c, mg->post);
     }
+}
+
+/* Multigrid::full_cycle (solvers/multilevel/utopia_Multigrid.hpp:381-415): rhss[l] = r; restrict
+ * and mask down to level 0; coarse solve; c[1] = I c[0]; per level i = 1..l-1: v_cycle_repetition
+ * times (r[i] = rhss[i]; standard_cycle(i)), then c[i+1] = I c[i]; finally v_cycle_repetition
+ * standard cycles on level l. */
+static void full_cycle(or_mg *mg, int l) {
+    double **rhss = mg->f;
+    memcpy(rhss[l], mg->r[l], sizeof(double) * (size_t)mg->A[l].nrows);
+    for (int i = l - 1; i >= 0; --i) {
+        or_spmv(&mg->R[i], rhss[i + 1], rhss[i]);
+        if (mg->masks)
+            for (int64_t k = 0; k < mg->A[i].nrows; ++k) rhss[i][k] = mg->mask[i][k] * rhss[i][k];
+    }
+    memset(mg->c[0], 0, sizeof(double) * (size_t)mg->A[0].nrows);
+    or_cholesky_solve(mg->A[0].nrows, mg->chol, rhss[0], mg->c[0]);
+    or_spmv(&mg->I[0], mg->c[0], mg->c[1]);
+    for (int i = 1; i < l; ++i) {
+        for (int64_t j = 0; j < mg->v_cycle_repetition; ++j) {
+            memcpy(mg->r[i], rhss[i], sizeof(double) * (size_t)mg->A[i].nrows);
+            standard_cycle(mg, i);
+        }
+        or_spmv(&mg->I[i], mg->c[i], mg->c[i + 1]);
+    }
+    for (int64_t j = 0; j < mg->v_cycle_repetition; ++j) standard_cycle(mg, l);
 }
 
 /* Multigrid::apply (solvers/multilevel/utopia_Multigrid.hpp:162-238), MULTIPLICATIVE_CYCLE */
@@ -751,8 +865,12 @@ int or_mg_apply(or_mg *mg, const double *rhs, double *x, int64_t max_it, double 
     const double r0_norm = r_norm;
     int64_t it = 1;
     int converged = 0;
+    if (mg->cycle_type == 3) max_it = 1; /* FULL_CYCLE sets max_it(1) (utopia_Multigrid.hpp:191-193) */
     while (!converged) {
-        standard_cycle(mg, l);
+        if (mg->cycle_type == 3)
+            full_cycle(mg, l);
+        else
+            standard_cycle(mg, l);
         for (int64_t i = 0; i < n; ++i) x[i] = x[i] + mg->c[l][i];
         or_spmv(&mg->A[l], x, r);
         for (int64_t i = 0; i < n; ++i) r[i] = rhs[i] - r[i];
@@ -791,7 +909,9 @@ void or_mg_destroy(or_mg *mg) {
         free(mg->c[l]);
         free(mg->cI[l]);
         free(mg->rR[l]);
+        free(mg->f[l]);
         free(mg->color[l]);
+        or_csr_free(&mg->LU[l]);
     }
     free(mg->chol);
     free(mg);

@@ -52,11 +52,15 @@ typedef struct or_status {
 int or_cg_solve(const or_csr *A, const double *b, double *x, int precond, double atol, double rtol, double stol,
                 int64_t max_it, or_status *st);
 
-/* Multigrid with Chebyshev3level (smoother=0) or Jacobi (smoother=1), dense
- * Cholesky coarse solve, IPRTransfer, masks, Galerkin assembly. */
+/* Multigrid with Chebyshev3level (smoother=0), Jacobi (smoother=1), multicolour
+ * GaussSeidel (smoother=2) or PointJacobi (smoother=3), dense Cholesky coarse solve,
+ * IPRTransfer, masks, Galerkin assembly. */
 typedef struct or_mg or_mg;
 or_mg *or_mg_create(int nlevels, const or_csr *const *interp /* coarse->fine, nlevels-1 */, int smoother,
                     int pre_steps, int post_steps, int mg_type, int masks, int fix_semidefinite);
+/* cycle_type 1 (MULTIPLICATIVE_CYCLE) or 3 (FULL_CYCLE), v_cycle_repetition, use_line_search
+ * (defaults 1, 10, 0; utopia_MultiLevelBase.hpp:38-40, utopia_Multigrid.hpp:74) */
+void or_mg_set_cycle(or_mg *mg, int cycle_type, int64_t v_cycle_repetition, int line_search);
 /* Galerkin assembly + smoother/coarse setup (PowerMethod uses rand()). */
 int or_mg_update(or_mg *mg, const or_csr *A);
 /* Multigrid::apply with max_it iterations, atol/rtol */
@@ -77,6 +81,10 @@ void or_gs_smooth(const or_csr *A, const double *dinv, const int32_t *color, int
                   double *x, int sweeps, int symmetric);
 void or_mg_destroy(or_mg *mg);
 
+/* Jacobi::update + smooth (utopia_Jacobi.hpp:99-140) and PointJacobi::update + smooth
+ * (utopia_PointJacobi.hpp:61-142): sweeps from the given x */
+void or_jacobi_smooth(const or_csr *A, const double *rhs, double *x, int sweeps);
+void or_point_jacobi_smooth(const or_csr *A, const double *rhs, double *x, int sweeps);
 /* PowerMethod::get_max_eig (rand() initial vector) */
 double or_power_method(const or_csr *A, double tol, int64_t max_it);
 /* Chebyshev3level::solve with explicit bounds, nu sweeps (smooth mode) */

@@ -53,6 +53,7 @@ def lib():
             "or_cg_solve": (I, [P, P, P, I, D, D, D, I64, P]),
             "or_mg_create": (P, [I, P, I, I, I, I, I, I]),
             "or_mg_update": (I, [P, P]),
+            "or_mg_set_cycle": (None, [P, I, I64, I]),
             "or_mg_apply": (I, [P, P, P, I64, D, D]),
             "or_cg_mg_solve": (I, [P, P, P, D, I64, P]),
             "or_mg_nlevels": (I, [P]),
@@ -65,6 +66,8 @@ def lib():
             "or_jp_color": (None, [P, ctypes.c_uint64, P, P]),
             "or_mg_destroy": (None, [P]),
             "or_power_method": (D, [P, D, I64]),
+            "or_jacobi_smooth": (None, [P, P, P, I]),
+            "or_point_jacobi_smooth": (None, [P, P, P, I]),
             "or_chebyshev_smooth": (None, [P, P, P, I, D, D]),
             "or_mprgp_solve": (I, [P, P, P, P, P, D, D, I64, D, I64, P]),
             "or_pgs_solve": (I, [P, P, P, P, P, D, I64, I, I64, P]),
@@ -285,18 +288,31 @@ def chebyshev_smooth(A, b, x, nu, eig_max, eig_min):
     return x
 
 
+def jacobi_smooth(A, b, x, sweeps, point=False):
+    """Jacobi (point=False) or PointJacobi (point=True) sweeps from x."""
+    M = Mat.of(A)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.ascontiguousarray(x, np.float64).copy()
+    f = lib().or_point_jacobi_smooth if point else lib().or_jacobi_smooth
+    f(M.ref, _p(b), _p(x), int(sweeps))
+    return x
+
+
 class MG:
     """Oracle Multigrid (+ CG-MG)."""
 
-    SMOOTHER_CHEBYSHEV, SMOOTHER_JACOBI = 0, 1
+    SMOOTHER_CHEBYSHEV, SMOOTHER_JACOBI, SMOOTHER_GAUSS_SEIDEL, SMOOTHER_POINT_JACOBI = 0, 1, 2, 3
+    MULTIPLICATIVE_CYCLE, FULL_CYCLE = 1, 3
 
-    def __init__(self, interps, smoother=0, pre=3, post=3, mg_type=1, masks=True, fix_semidefinite=False):
+    def __init__(self, interps, smoother=0, pre=3, post=3, mg_type=1, masks=True, fix_semidefinite=False,
+                 cycle_type=1, v_cycle_repetition=10, use_line_search=False):
         self._I = [Mat.of(I) for I in interps]
         arr = (ctypes.c_void_p * len(self._I))(*[ctypes.addressof(m.s) for m in self._I])
         self.h = lib().or_mg_create(len(interps) + 1, arr, smoother, pre, post, mg_type, 1 if masks else 0,
                                     1 if fix_semidefinite else 0)
         if not self.h:
             raise ValueError("or_mg_create failed")
+        lib().or_mg_set_cycle(self.h, int(cycle_type), int(v_cycle_repetition), 1 if use_line_search else 0)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -90,3 +90,68 @@ def test_rejects_non_cuda_tensors(setup):
     p, k, B, Xs, op = setup
     with pytest.raises(TypeError):
         op(torch.tensor(p.A.val), torch.tensor(B))
+
+
+def test_two_forwards_then_backward_on_the_first(setup):
+    """Two solves with different operator values before one backward: each gradient is the
+    transposed-solve gradient of its own operator (the solver is set up again for it)."""
+    p, k, B, Xs, op = setup
+    dev = torch.device("cuda:0")
+    vals1 = torch.tensor(p.A.val, dtype=torch.float64, device=dev, requires_grad=True)
+    v2 = p.A.val.copy()
+    diag = p.A.col == np.repeat(np.arange(p.n), np.diff(p.A.rowptr))
+    v2[diag] *= 1.5  # stays SPD (diagonally dominant)
+    vals2 = torch.tensor(v2, dtype=torch.float64, device=dev, requires_grad=True)
+    Bt = torch.tensor(B, dtype=torch.float64, device=dev)
+    Xst = torch.tensor(Xs, dtype=torch.float64, device=dev)
+    X1 = op(vals1, Bt)
+    X2 = op(vals2, Bt)
+    L1 = 0.5 * ((X1 - Xst) ** 2).sum()
+    L1.backward()
+    rows = np.repeat(np.arange(p.n), np.diff(p.A.rowptr))
+    for vals, v in ((vals1, p.A.val), (vals2, v2)):
+        A = ua.Csr(p.A.rowptr, p.A.col, v, p.A.shape).to_scipy()
+        Xd = spla.spsolve(A.tocsc(), B).reshape(B.shape)
+        lam = spla.spsolve(A.T.tocsc(), Xd - Xs).reshape(B.shape)
+        ref = -(lam[rows] * Xd[p.A.col]).sum(axis=1)
+        if vals is vals2:
+            L2 = 0.5 * ((X2 - Xst) ** 2).sum()
+            L2.backward()
+        g = vals.grad.cpu().numpy()
+        assert np.linalg.norm(g - ref) <= 1e-10 * np.linalg.norm(ref)
+
+
+def test_repeated_forward_on_same_values_sets_up_once():
+    from utopia_amd.autograd import BatchedSolve
+    p = bt.make_batch([10, 14], seed=2, levels=2)
+    op = BatchedSolve(p, 2, mixed=False, rtol=1e-12, max_it=300)
+    dev = torch.device("cuda:0")
+    vals = torch.tensor(p.A.val, dtype=torch.float64, device=dev)
+    Bt = torch.tensor(ua.gen_uniform(3, p.n * 2).reshape(p.n, 2), dtype=torch.float64, device=dev)
+    X1 = op(vals, Bt)
+    X2 = op(vals, Bt)
+    assert op.setups == 1 and torch.equal(X1, X2)
+    vals.mul_(2.0)  # in place: a new version, so a new setup
+    X3 = op(vals, Bt)
+    assert op.setups == 2
+    assert torch.allclose(X3, X1 / 2, rtol=1e-9, atol=0)
+
+
+def test_mixed_forward_gradient_matches_transposed_solve():
+    """FP32 V-cycle inside the FP64 CG (config 5's form): forward to rtol 1e-13 and the adjoint
+    gradient against the FP64 transposed-solve gradient."""
+    from utopia_amd.autograd import BatchedSolve
+    p = bt.make_batch([20, 13, 27], seed=8, levels=2)
+    k = 2
+    B = ua.gen_uniform(41, p.n * k).reshape(p.n, k)
+    Xs = ua.gen_normal(42, p.n * k).reshape(p.n, k)
+    op = BatchedSolve(p, k, mixed=True, rtol=1e-13, max_it=500)
+    X, _, gA, gB = _grads(p, k, B, Xs, op)
+    A = p.A.to_scipy()
+    Xd = spla.spsolve(A.tocsc(), B).reshape(B.shape)
+    lam = spla.spsolve(A.T.tocsc(), Xd - Xs).reshape(B.shape)
+    rows = np.repeat(np.arange(p.n), np.diff(p.A.rowptr))
+    ref = -(lam[rows] * Xd[p.A.col]).sum(axis=1)
+    assert np.linalg.norm(X - Xd) <= 1e-10 * np.linalg.norm(Xd)
+    assert np.linalg.norm(gB - lam) <= 1e-9 * np.linalg.norm(lam)
+    assert np.linalg.norm(gA - ref) <= 1e-9 * np.linalg.norm(ref)

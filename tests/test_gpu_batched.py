@@ -175,8 +175,9 @@ def test_batched_fp64_against_oracle_one_system(ctx):
 
 
 def test_graph_replays_match_eager(ctx):
-    """Graph replays (1 and 3 iterations per replay) equal the eager loop bit for bit,
-    on repeated solves with the host doing eager work between them."""
+    """Graph replays (1, 3 and the default 16 iterations per replay, the kernels gated on the
+    active-pair count) equal the eager loop bit for bit, with the same iteration count, on
+    repeated solves with the host doing eager work between them."""
     p = bt.make_batch([26, 40, 18], seed=4, levels=3)
     k = 8
     B = ua.gen_uniform(11, p.n * k).reshape(p.n, k)
@@ -184,8 +185,11 @@ def test_graph_replays_match_eager(ctx):
     Xe = eager.solve(B)
     ite, _ = eager.convergence()
     assert eager.info("graph_replays") == 0
-    for gi in (1, 3):
-        g = bt.BatchedSolver(ctx, k, mixed=True, rtol=1e-8, use_graph=1, graph_iterations=gi)
+    n_it = eager.info("iterations")
+    assert n_it == ite.max()
+    for gi in (1, 3, None):
+        kw = {} if gi is None else {"graph_iterations": gi}
+        g = bt.BatchedSolver(ctx, k, mixed=True, rtol=1e-8, use_graph=1, **kw)
         g.setup(p)
         for l in range(1, p.levels):  # identical Chebyshev bounds
             for s in range(p.nseg):
@@ -194,11 +198,35 @@ def test_graph_replays_match_eager(ctx):
             Xg = g.solve(B)
             itg, fl = g.convergence()
             assert g.info("graph_replays") >= 1
+            assert g.info("iterations") == n_it
+            assert g.info("graph_replays") == -(-n_it // (gi or 16))
             assert np.array_equal(Xg, Xe), (gi, rep)
             assert np.array_equal(itg, ite)
             assert (fl == 1).all()
             # eager work between replays
             ua.DeviceCsr(ctx, p.A).spmv(ua.DeviceArray.from_numpy(ctx, Xg[:, 0].copy()), ua.DeviceArray(ctx, p.n))
+
+
+def test_graph_recaptured_after_workspace_grows(ctx):
+    """A captured graph holds the reduction workspace's address: after the workspace is
+    reallocated (here by a reservation) the next solve captures again, and its result
+    equals the eager solve bit for bit."""
+    p = bt.make_batch([22, 31, 15], seed=9, levels=3)
+    k = 4
+    B = ua.gen_uniform(12, p.n * k).reshape(p.n, k)
+    eager = bt.BatchedSolver(ctx, k, mixed=True, rtol=1e-8, use_graph=0).setup(p)
+    Xe = eager.solve(B)
+    g = bt.BatchedSolver(ctx, k, mixed=True, rtol=1e-8, use_graph=1, graph_iterations=4).setup(p)
+    for l in range(1, p.levels):
+        for s in range(p.nseg):
+            g.set_level_eig(l, s, eager.eig_max(l)[s])
+    assert np.array_equal(g.solve(B), Xe)
+    assert g.info("graph_captures") == 1
+    assert np.array_equal(g.solve(B), Xe)
+    assert g.info("graph_captures") == 1
+    g.set_param("workspace_reserve", 1 << 22)  # reallocates the workspace
+    assert np.array_equal(g.solve(B), Xe)
+    assert g.info("graph_captures") == 2
 
 
 def test_batched_uneven_levels_config_shape(ctx):

@@ -34,7 +34,7 @@ class _SolveFn(torch.autograd.Function):
         B = B.contiguous()
         X = torch.zeros_like(B)
         torch.cuda.synchronize(B.device)  # B, X and values are complete before the solver stream reads them
-        op.solver.setup(op.problem, values=values)
+        op._ensure_setup(values)
         op.solver.solve_device(B, X)  # returns after its stream finished
         ctx.save_for_backward(values, X)
         ctx.op = op
@@ -47,6 +47,8 @@ class _SolveFn(torch.autograd.Function):
         G = G.contiguous()
         lam = torch.zeros_like(G)
         torch.cuda.synchronize(G.device)
+        # the operator of this forward: another forward on other values may have set up since
+        op._ensure_setup(values)
         op.solver.solve_device(G, lam)  # A^T lam = G, A = A^T
         grad_vals = torch.empty_like(values)
         op.solver.pattern_grad_device(lam, X, grad_vals)
@@ -65,6 +67,20 @@ class BatchedSolve:
         self.problem = problem
         self.ctx = ctx if ctx is not None else Context(torch.cuda.current_device())
         self.solver = BatchedSolver(self.ctx, k, mixed=mixed, **params)
+        self._values = None  # the values the solver's hierarchy was built from (kept alive: read in place)
+        self._version = None
+        self.setups = 0
+
+    def _ensure_setup(self, values: torch.Tensor):
+        """Build the hierarchy (Galerkin levels, eigenvalue estimates, coarse inverses)
+        from `values` unless it was built from these very values, unchanged since."""
+        v = self._values
+        if (v is not None and v.data_ptr() == values.data_ptr() and v.shape == values.shape
+                and self._version == values._version):
+            return
+        self.solver.setup(self.problem, values=values)
+        self._values, self._version = values, values._version
+        self.setups += 1
 
     def __call__(self, values: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
         return _SolveFn.apply(values, B, self)

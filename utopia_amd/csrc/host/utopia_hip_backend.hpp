@@ -161,6 +161,11 @@ public:
     void shift(T a);
     void abs();
     void e_mul(const HipVectorT &x);               // this = this .* x
+    void e_div(const HipVectorT &x);               // this = this ./ x
+    void e_min(const HipVectorT &x);               // this = min(this, x)  (VecPointwiseMin)
+    void e_max(const HipVectorT &x);               // this = max(this, x)  (VecPointwiseMax)
+    void reciprocal(T num);                        // VecReciprocal, then * num unless num == 1
+    void replace(T from, T to);                    // entries equal to from become to
     double dot(const HipVectorT &y) const;
     double norm2() const;
     double norm_infty() const;
@@ -243,6 +248,26 @@ using HipVector32 = HipVectorT<float>;
     template <>                                                                                               \
     inline void HipVectorT<T>::e_mul(const HipVectorT<T> &x) {                                                \
         hip_check(uhip_vec_emul_##SUF(ctx(), n_, d_, x.d_, d_), "e_mul");                                     \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::e_div(const HipVectorT<T> &x) {                                                \
+        hip_check(uhip_vec_ediv_##SUF(ctx(), n_, d_, x.d_, d_), "e_div");                                     \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::e_min(const HipVectorT<T> &x) {                                                \
+        hip_check(uhip_vec_emin_##SUF(ctx(), n_, d_, x.d_, d_), "e_min");                                     \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::e_max(const HipVectorT<T> &x) {                                                \
+        hip_check(uhip_vec_emax_##SUF(ctx(), n_, d_, x.d_, d_), "e_max");                                     \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::reciprocal(T num) {                                                            \
+        hip_check(uhip_vec_reciprocal_##SUF(ctx(), n_, num, d_, d_), "reciprocal");                           \
+    }                                                                                                         \
+    template <>                                                                                               \
+    inline void HipVectorT<T>::replace(T from, T to) {                                                        \
+        hip_check(uhip_vec_replace_##SUF(ctx(), n_, from, to, d_), "replace");                                \
     }                                                                                                         \
     template <>                                                                                               \
     inline double HipVectorT<T>::dot(const HipVectorT<T> &y) const {                                          \
@@ -416,6 +441,13 @@ public:
     void build_diag(Vector &d) const {
         if (d.size() != rows() || d.empty()) d = Vector(ctx_, rows());
         hip_check(uhip_csr_diag(c(), &m_, d.data()), "diag");
+    }
+
+    // A_ii += alpha d_i (PetscMatrix::shift_diag -> MatDiagonalSet ADD_VALUES,
+    // backend/petsc/utopia_petsc_Matrix.cpp:1783-1788); the plan is rebuilt
+    void shift_diag(const Vector &d, T alpha = T(1)) {
+        hip_check(uhip_csr_shift_diag(c(), &m_, d.data(), (double)alpha), "shift_diag");
+        replan();
     }
 
     // B = A^T (PetscMatrix::transpose, utopia_petsc_Matrix.hpp:601)

@@ -111,7 +111,11 @@ public:
     BatchedMultigridCG(HipContextPtr ctx, int K, bool mixed) : ctx_(std::move(ctx)), K_(K), mixed_(mixed) {}
     ~BatchedMultigridCG() { drop_graph(); }
 
-    void set_param(const std::string &k, double v) { params_[k] = v; }
+    void set_param(const std::string &k, double v) {
+        params_[k] = v;
+        // pre-size the reduction workspace (partial sums per dot slot) of this solver's context
+        if (k == "workspace_reserve") hip_check(uhip_ctx_reserve(ctx_->get(), (int64_t)v), "workspace_reserve");
+    }
     double param(const std::string &k, double def) const {
         auto it = params_.find(k);
         return it == params_.end() ? def : it->second;
@@ -184,27 +188,29 @@ public:
         p_.copy_from(z_);
         top.dot(c, K_, r_.data(), z_.data(), UHIP_F64, rz_.data(), false);
         const bool use_graph = param("use_graph", 1) != 0;
-        // iterations per replay: converged pairs take zero steps, so a replay
-        // may run past convergence without changing their results
-        const int gi = std::max(1, (int)param("graph_iterations", 1));
+        // iterations per replay: the captured kernels are gated on the active-pair count
+        // (uhip_ctx_set_gate), so the iterations of a replay past convergence do no work,
+        // and converged pairs take zero steps in the ones before
+        const int gi = std::max(1, (int)param("graph_iterations", 16));
         if (gi != graph_iters_) {
             drop_graph();
             graph_iters_ = gi;
         }
         int64_t it = 0;
-        int32_t active = m;
+        int32_t cnt[2] = {m, 0};  // active pairs, iterations done (device counters)
         graph_replays_ = 0;
-        while (it < max_it && active > 0) {
-            if (use_graph) {
-                if (!graph_exec_) capture();
+        while (it < max_it && cnt[0] > 0) {
+            if (use_graph && max_it - it >= graph_iters_) {
+                // the graph holds the reduction workspace's address: capture again when an
+                // operation on this context has reallocated it since
+                if (!graph_exec_ || uhip_ctx_workspace_generation(c) != graph_gen_) capture();
                 hip_check(uhip_graph_launch(c, graph_exec_), "graph launch");
                 ++graph_replays_;
-                it += graph_iters_;
-            } else {
+            } else {  // eager (or the last iterations before max_it)
                 iteration();
-                ++it;
             }
-            hip_check(uhip_memcpy_dtoh(c, &active, counters_.data(), sizeof(int32_t)), "active count");
+            hip_check(uhip_memcpy_dtoh(c, cnt, counters_.data(), sizeof(cnt)), "counters");
+            it = cnt[1];
         }
         iterations_ = it;
         hip_check(uhip_memcpy_dtod(c, X, x_.data(), sizeof(double) * (size_t)nk), "copy X back");
@@ -224,6 +230,7 @@ public:
     double solve_seconds() const { return solve_s_; }
     int64_t iterations() const { return iterations_; }
     int64_t graph_replays() const { return graph_replays_; }
+    int64_t graph_captures() const { return graph_captures_; }
     int levels() const { return (int)A64_.size(); }
     const HipCrsMatrix &level_operator(int l) const { return *A64_[(size_t)l]; }
     void set_eig_max(int l, const std::vector<double> &v) {
@@ -498,16 +505,25 @@ private:
         int64_t maxparts = 0;
         for (auto &lay : layout_) maxparts = std::max<int64_t>(maxparts, (int64_t)lay.nchunks * K_);
         hip_check(uhip_ctx_reserve(ctx_->get(), maxparts), "reserve");
+        graph_gen_ = uhip_ctx_workspace_generation(ctx_->get());
         hip_check(uhip_graph_begin(ctx_->get()), "capture begin");
+        hip_check(uhip_ctx_set_gate(ctx_->get(), counters_.data()), "gate");
         try {
             for (int i = 0; i < graph_iters_; ++i) iteration();
         } catch (...) {
+            uhip_ctx_set_gate(ctx_->get(), nullptr);
             void *g = nullptr;
             uhip_graph_end(ctx_->get(), &g);
             uhip_graph_destroy(g);
             throw;
         }
+        hip_check(uhip_ctx_set_gate(ctx_->get(), nullptr), "gate");
         hip_check(uhip_graph_end(ctx_->get(), &graph_exec_), "capture end");
+        ++graph_captures_;
+        if (uhip_ctx_workspace_generation(ctx_->get()) != graph_gen_) {
+            drop_graph();
+            throw HipError(UHIP_ERR_INVALID, "batched CG: reduction workspace reallocated during capture");
+        }
     }
     void drop_graph() {
         if (graph_exec_) uhip_graph_destroy(graph_exec_);
@@ -529,8 +545,9 @@ private:
     HipVector rz_, pap_, alpha_, beta_, rznew_, rr_, atol_;
     DeviceInts flags_, conv_it_, counters_;
     void *graph_exec_ = nullptr;
+    int64_t graph_gen_ = -1;  // workspace generation the graph was captured at
     int graph_iters_ = 1;
-    int64_t iterations_ = 0, graph_replays_ = 0;
+    int64_t iterations_ = 0, graph_replays_ = 0, graph_captures_ = 0;
     double setup_s_ = 0, solve_s_ = 0;
 };
 

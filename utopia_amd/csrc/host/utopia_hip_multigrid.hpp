@@ -43,20 +43,11 @@ public:
         if (sol.size() != rhs.size() || sol.empty()) sol.zeros_like(rhs);
         return solve_raw(rhs.data(), UHIP_F64, sol.data(), UHIP_F64);
     }
+    // the coarsest solve stays FP64 in every form: the FP64 inverse, FP64 sums; an
+    // FP32 V-cycle hands FP32 vectors in and out (converted exactly on load)
     bool solve_raw(const void *rhs, int rhs_dt, void *sol, int sol_dt) const {
-        if (ainv32_ && rhs_dt == UHIP_F32 && sol_dt == UHIP_F32)
-            hip_check(uhip_dense_gemv_m32(ctx_->get(), n_, ainv32_, (const float *)rhs, (float *)sol), "coarse gemv");
-        else
-            hip_check(uhip_dense_gemv(ctx_->get(), n_, ainv_, rhs, rhs_dt, sol, sol_dt), "coarse gemv");
+        hip_check(uhip_dense_gemv(ctx_->get(), n_, ainv_, rhs, rhs_dt, sol, sol_dt), "coarse gemv");
         return true;
-    }
-    // FP32 copy of the inverse, used by FP32-in / FP32-out solves (the FP32 V-cycle)
-    void prepare_f32() {
-        if (ainv32_ || !ainv_) return;
-        void *p = nullptr;
-        hip_check(uhip_malloc(ctx_->get(), &p, sizeof(float) * (size_t)n_ * n_), "alloc inverse f32");
-        ainv32_ = (float *)p;
-        hip_check(uhip_vec_cast_f64_f32(ctx_->get(), (int64_t)n_ * n_, ainv_, ainv32_), "inverse f32");
     }
     int32_t n() const { return n_; }
     const double *inverse() const { return ainv_; }
@@ -64,13 +55,10 @@ public:
 private:
     void release() {
         if (ainv_ && ctx_) uhip_free(ctx_->get(), ainv_);
-        if (ainv32_ && ctx_) uhip_free(ctx_->get(), ainv32_);
         ainv_ = nullptr;
-        ainv32_ = nullptr;
     }
     HipContextPtr ctx_;
     double *ainv_ = nullptr;
-    float *ainv32_ = nullptr;
     int32_t n_ = 0;
 };
 
@@ -142,6 +130,48 @@ public:
     const Vector &d_inv() const { return d_inv_; }
 
 private:
+    Vector d_inv_, r_;
+};
+
+// PointJacobi (solvers/smoothers/utopia_PointJacobi.hpp:70-145): LU = A - D kept as a matrix
+// (the diagonal entries stay in the pattern as zeros), d_inv = 1./diag(A) in the form of
+// VecReciprocal (zeros kept) with check_indef turning zeros into ones; a sweep is
+// x = d_inv .* (rhs - LU x).
+template <class Matrix, class Vector>
+class PointJacobi final : public IterativeSolver<Matrix, Vector> {
+public:
+    using T = typename Vector::Scalar;
+    PointJacobi() { this->max_it_ = 1; }
+    void update(const std::shared_ptr<const Matrix> &op) override {
+        IterativeSolver<Matrix, Vector>::update(op);
+        make_point_jacobi(*op, LU_, d_inv_);
+        r_ = Vector(op->context(), op->rows(), 0);
+    }
+    // LU = A - diag(A) (pattern kept) and d_inv = check_indef(1 ./ diag(A))
+    static void make_point_jacobi(const Matrix &A, std::shared_ptr<const Matrix> &LU, Vector &d_inv) {
+        A.build_diag(d_inv);
+        auto m = std::make_shared<Matrix>(A.copy());
+        m->shift_diag(d_inv, T(-1));
+        LU = m;
+        d_inv.reciprocal(T(1));
+        d_inv.replace(T(0), T(1));
+    }
+    // smooth: sweeps from the given x
+    bool apply(const Vector &b, Vector &x) override {
+        if (x.empty() || x.size() != b.size()) x.zeros_like(b);
+        for (int64_t s = 0; s < this->max_it_; ++s) {
+            hip_check(uhip_spmv_jacobi(x.ctx(), &LU_->raw(), x.data(), b.data(), Vector::dtype, d_inv_.data(),
+                                       r_.data(), Vector::dtype, 0),
+                      "point jacobi sweep");
+            x.copy_from(r_);
+        }
+        return true;
+    }
+    const Vector &d_inv() const { return d_inv_; }
+    const Matrix &lu() const { return *LU_; }
+
+private:
+    std::shared_ptr<const Matrix> LU_;
     Vector d_inv_, r_;
 };
 
@@ -243,15 +273,19 @@ struct MGLevel {
     std::shared_ptr<const HipCrsMatrixT<T>> A;  // level operator
     std::shared_ptr<const HipCrsMatrixT<T>> R;  // restriction to level l-1 (l >= 1)
     std::shared_ptr<const HipCrsMatrixT<T>> P;  // interpolation from level l-1 (l >= 1)
+    std::shared_ptr<const HipCrsMatrixT<T>> LU; // PointJacobi: A - D (l >= 1)
     HipVectorT<T> mask;                        // mask of this level (empty: masks off)
     HipVectorT<T> r, c, rR, rk, pa, pb, dinv;
+    HipVectorT<T> cI, f;                       // line search: I c_{l-1}; full cycle: restricted rhs
     std::shared_ptr<ColorGroups> groups;        // Gauss-Seidel colour groups (FP64 levels)
     std::vector<double> alpha, beta;            // Chebyshev coefficients for pre / post
     std::vector<double> alpha_post, beta_post;
     double eig_max = 0, eig_min = 0;
 };
 
-enum MGSmoother { MG_CHEBYSHEV = 0, MG_JACOBI = 1, MG_GAUSS_SEIDEL = 2 };
+enum MGSmoother { MG_CHEBYSHEV = 0, MG_JACOBI = 1, MG_GAUSS_SEIDEL = 2, MG_POINT_JACOBI = 3 };
+// MultiLevelBase cycle types (solvers/multilevel/utopia_MultiLevelBase.hpp:12-14)
+enum MGCycle { MG_MULTIPLICATIVE_CYCLE = 1, MG_ADDITIVE_CYCLE = 2, MG_FULL_CYCLE = 3 };
 
 template <typename T>
 class VCycle {
@@ -260,7 +294,7 @@ public:
     std::vector<MGLevel<T>> levels;
     const Factorization *coarse = nullptr;
     int pre = 3, post = 3, mg_type = 1, smoother = MG_CHEBYSHEV;
-    bool masks = true, gs_symmetric = true;
+    bool masks = true, gs_symmetric = true, line_search = false;
 
     void allocate_work() {
         for (size_t l = 0; l < levels.size(); ++l) {
@@ -268,11 +302,13 @@ public:
             const int64_t n = L.A->rows();
             auto ctx = L.A->context();
             for (HipVectorT<T> *v : {&L.r, &L.c, &L.rR, &L.rk, &L.pa, &L.pb}) *v = HipVectorT<T>(ctx, n, T(0));
+            if (line_search && l > 0) L.cI = HipVectorT<T>(ctx, n, T(0));
             if (smoother == MG_CHEBYSHEV && l > 0) {
                 chebyshev_coefficients(L.eig_max, L.eig_min, pre, L.alpha, L.beta);
                 chebyshev_coefficients(L.eig_max, L.eig_min, post, L.alpha_post, L.beta_post);
             }
         }
+        if (line_search && !levels.empty()) ls_ = HipVectorT<double>(levels.back().A->context(), 2, 0.0);
     }
 
     // Multigrid::standard_cycle (utopia_Multigrid.hpp:252-367). r: level rhs
@@ -303,7 +339,8 @@ public:
                       "mg restrict");
             cycle(l - 1, Lc.r.data(), dt, Lc.c.data(), dt);
             // c += I c_{l-1}
-            hip_check(uhip_spmv_add(c, &L.P->raw(), Lc.c.data(), L.c.data()), "mg interpolate");
+            if (line_search) line_search_correct(L, Lc);
+            else hip_check(uhip_spmv_add(c, &L.P->raw(), Lc.c.data(), L.c.data()), "mg interpolate");
             void *o = final_pass ? out : (void *)L.c.data();
             const int odt = final_pass ? out_dt : dt;
             dot_done = smooth(L, r, r_dt, false, o, odt, post, L.alpha_post, L.beta_post,
@@ -313,11 +350,76 @@ public:
         return dot_done;
     }
 
+    // Multigrid::full_cycle (utopia_Multigrid.hpp:381-415), FP64 levels: the right-hand side
+    // r of level l restricted (and masked) to every level, the coarse solution interpolated
+    // upwards with v_cycle_repetition standard cycles on each intermediate level, then
+    // v_cycle_repetition standard cycles on level l into out. The steps are the reference's;
+    // every standard cycle starts from c = 0 (:264-268), so the result equals one standard
+    // cycle of r.
+    void full_cycle(int l, const double *r, double *out, int64_t repetitions) {
+        if constexpr (!std::is_same<T, double>::value) {
+            throw HipError(UHIP_ERR_UNSUPPORTED, "full cycle: FP64 levels only");
+        } else {
+            if (l == 0) {
+                coarse->solve_raw(r, UHIP_F64, out, UHIP_F64);
+                return;
+            }
+            uhip_ctx c = levels[(size_t)l].A->c();
+            for (int i = l - 1; i >= 0; --i) {
+                auto &Li = levels[(size_t)i];
+                auto &Lf = levels[(size_t)i + 1];
+                if (Li.f.size() != Li.A->rows() || Li.f.empty()) Li.f = HipVectorT<T>(Li.A->context(), Li.A->rows());
+                hip_check(uhip_spmv_masked(c, &Lf.R->raw(), i == l - 1 ? (const void *)r : (const void *)Lf.f.data(),
+                                           masks && !Li.mask.empty() ? Li.mask.data() : nullptr, Li.f.data()),
+                          "full cycle restrict");
+            }
+            coarse->solve_raw(levels[0].f.data(), UHIP_F64, levels[0].c.data(), UHIP_F64);
+            hip_check(uhip_spmv(c, &levels[1].P->raw(), levels[0].c.data(), levels[1].c.data()), "full cycle interpolate");
+            for (int i = 1; i < l; ++i) {
+                auto &Li = levels[(size_t)i];
+                for (int64_t j = 0; j < repetitions; ++j) cycle(i, Li.f.data(), dt, Li.c.data(), dt);
+                hip_check(uhip_spmv(c, &levels[(size_t)i + 1].P->raw(), Li.c.data(), levels[(size_t)i + 1].c.data()),
+                          "full cycle interpolate");
+            }
+            for (int64_t j = 0; j < repetitions; ++j) cycle(l, r, UHIP_F64, out, UHIP_F64);
+        }
+    }
+
 private:
+    HipVectorT<double> ls_;  // line search: c_I.r_R, c_I.A c_I
+
+    static int vdot(uhip_ctx c, int64_t n, const T *x, const T *y, double *d) {
+        if constexpr (std::is_same<T, double>::value) return uhip_vec_dot_f64(c, n, x, y, d);
+        else return uhip_vec_dot_f32(c, n, x, y, d);
+    }
+    static int vaxpy(uhip_ctx c, int64_t n, T a, const T *x, T *y) {
+        if constexpr (std::is_same<T, double>::value) return uhip_vec_axpy_f64(c, n, a, x, y);
+        else return uhip_vec_axpy_f32(c, n, a, x, y);
+    }
+
+    // Multigrid::standard_cycle with use_line_search (utopia_Multigrid.hpp:324-341, the
+    // symmetric form): c_I = I c_{l-1}; alpha = (c_I . r_R) / ((A c_I) . c_I);
+    // c += alpha c_I, or c += c_I when alpha <= 0
+    void line_search_correct(MGLevel<T> &L, MGLevel<T> &Lc) {
+        uhip_ctx c = L.A->c();
+        const int64_t n = L.A->rows();
+        hip_check(uhip_spmv(c, &L.P->raw(), Lc.c.data(), L.cI.data()), "mg interpolate");
+        hip_check(vdot(c, n, L.cI.data(), L.rR.data(), ls_.data()), "line search c_I.r_R");
+        hip_check(uhip_spmv_dot(c, &L.A->raw(), L.cI.data(), L.pb.data(), ls_.data() + 1), "line search A c_I");
+        double h[2] = {0, 0};
+        L.A->context()->read_scalars(ls_.data(), 2, h);
+        const double alpha = h[0] / h[1];
+        hip_check(vaxpy(c, n, alpha <= 0. ? T(1) : T(alpha), L.cI.data(), L.c.data()), "c += alpha c_I");
+    }
+
     bool smooth(MGLevel<T> &L, const void *b, int b_dt, bool zero_guess, void *out, int out_dt, int nu,
                 const std::vector<double> &a, const std::vector<double> &be, const double *dot_with = nullptr,
                 double *dot_d = nullptr) {
-        if (nu <= 0) {
+        if (nu <= 0 && smoother != MG_CHEBYSHEV) {
+            // no sweeps: the result is the guess itself, zero on a zero guess (the
+            // standard cycle starts every pass from c = 0). Chebyshev3level takes one
+            // step even for sweeps(0) (chebyshev_coefficients).
+            if (zero_guess) L.c.set(T(0));
             if (out != (void *)L.c.data())
                 hip_check(uhip_vec_axpy_cast(L.A->c(), L.A->rows(), 0.0, L.c.data(), dt, L.c.data(), dt, out, out_dt),
                           "copy");
@@ -353,15 +455,24 @@ private:
                 throw HipError(UHIP_ERR_UNSUPPORTED, "Gauss-Seidel smoothing runs on FP64 levels");
             }
         }
-        // Jacobi sweeps: r = b - A x; x += d_inv .* r (x = 0 on a zero guess)
+        // Jacobi sweeps x = x + d_inv .* (b - A x), or PointJacobi sweeps x = d_inv .* (b - LU x),
+        // each one fused SpMV pass, alternating between c and rk (x = 0 on a zero guess)
+        const bool point = smoother == MG_POINT_JACOBI;
+        const HipCrsMatrixT<T> &M = point ? *L.LU : *L.A;
         if (zero_guess) L.c.set(T(0));
+        T *cur = L.c.data(), *other = L.rk.data();
         for (int s = 0; s < nu; ++s) {
-            hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), b, b_dt, L.rk.data()), "jacobi r");
-            L.rk.e_mul(L.dinv);
             const bool last = (s == nu - 1);
-            hip_check(uhip_vec_axpy_cast(c, n, 1.0, L.rk.data(), dt, L.c.data(), dt, last ? out : (void *)L.c.data(),
-                                         last ? out_dt : dt),
-                      "jacobi x");
+            if (last && out != (void *)cur) {
+                hip_check(uhip_spmv_jacobi(c, &M.raw(), cur, b, b_dt, L.dinv.data(), out, out_dt, point ? 0 : 1),
+                          "jacobi sweep");
+                break;
+            }
+            hip_check(uhip_spmv_jacobi(c, &M.raw(), cur, b, b_dt, L.dinv.data(), other, dt, point ? 0 : 1),
+                      "jacobi sweep");
+            std::swap(cur, other);
+            if (last)  // out is the buffer the sweep read: the result is in the other one
+                hip_check(uhip_vec_axpy_cast(c, n, 0.0, cur, dt, cur, dt, out, out_dt), "copy");
         }
         return false;
     }
@@ -413,6 +524,20 @@ public:
     int pre_smoothing_steps() const { return pre_; }
     int post_smoothing_steps() const { return post_; }
     void mg_type(int t) { mg_type_ = t; }
+    // MultiLevelBase::cycle_type / v_cycle_repetition (utopia_MultiLevelBase.hpp:79-90),
+    // Multigrid::use_line_search (utopia_Multigrid.hpp:480)
+    void cycle_type(int t) { cycle_type_ = t; }
+    int cycle_type() const { return cycle_type_; }
+    void v_cycle_repetition(int64_t n) { v_cycle_repetition_ = n; }
+    int64_t v_cycle_repetition() const { return v_cycle_repetition_; }
+    void use_line_search(bool v) {
+        line_search_ = v;
+        if (eng_.line_search != v && !eng_.levels.empty()) {
+            eng_.line_search = v;
+            eng_.allocate_work();
+        }
+    }
+    bool use_line_search() const { return line_search_; }
     void must_generate_masks(bool v) { masks_ = v; }
     bool must_generate_masks() const { return masks_; }
     void fix_semidefinite_operators(bool v) { fix_semi_ = v; }
@@ -484,16 +609,21 @@ public:
         auto cheb = std::dynamic_pointer_cast<Chebyshev3level<HipCrsMatrix, HipVector>>(smoother_);
         auto jac = std::dynamic_pointer_cast<Jacobi<HipCrsMatrix, HipVector>>(smoother_);
         auto gs = std::dynamic_pointer_cast<GaussSeidel>(smoother_);
-        if (!cheb && !jac && !gs)
-            throw HipError(UHIP_ERR_UNSUPPORTED, "Multigrid: smoother must be Chebyshev3level, Jacobi or GaussSeidel");
+        auto pj = std::dynamic_pointer_cast<PointJacobi<HipCrsMatrix, HipVector>>(smoother_);
+        if (!cheb && !jac && !gs && !pj)
+            throw HipError(UHIP_ERR_UNSUPPORTED,
+                           "Multigrid: smoother must be Chebyshev3level, Jacobi, PointJacobi or GaussSeidel");
+        if (cycle_type_ != MG_MULTIPLICATIVE_CYCLE && cycle_type_ != MG_FULL_CYCLE)
+            throw HipError(UHIP_ERR_UNSUPPORTED, "Multigrid: unknown MG type (cycle_type 1 or 3)");
         eng_.levels.clear();
         eng_.levels.resize((size_t)L);
         eng_.pre = pre_;
         eng_.post = post_;
         eng_.mg_type = mg_type_;
         eng_.masks = masks;
-        eng_.smoother = cheb ? MG_CHEBYSHEV : (gs ? MG_GAUSS_SEIDEL : MG_JACOBI);
+        eng_.smoother = cheb ? MG_CHEBYSHEV : (gs ? MG_GAUSS_SEIDEL : (pj ? MG_POINT_JACOBI : MG_JACOBI));
         eng_.gs_symmetric = gs ? gs->symmetric() : true;
+        eng_.line_search = line_search_;
         for (int l = 0; l < L; ++l) {
             auto &lv = eng_.levels[(size_t)l];
             lv.A = A_[(size_t)l];
@@ -516,6 +646,8 @@ public:
                 hip_check(uhip_pgs_dinv(A_[(size_t)l]->c(), &A_[(size_t)l]->raw(), lv.dinv.data()), "gs d_inv");
                 lv.groups = std::make_shared<ColorGroups>();
                 lv.groups->build(*A_[(size_t)l]);
+            } else if (pj) {
+                PointJacobi<HipCrsMatrix, HipVector>::make_point_jacobi(*A_[(size_t)l], lv.LU, lv.dinv);
             } else {
                 A_[(size_t)l]->build_diag(lv.dinv);
                 InvDiagPreconditioner<HipCrsMatrix, HipVector>::pseudo_inv(lv.dinv, 0.0);
@@ -541,8 +673,10 @@ public:
         int64_t it = 1;
         bool converged = false;
         this->status_.clear();
+        if (cycle_type_ == MG_FULL_CYCLE) this->max_it(1);  // utopia_Multigrid.hpp:191-193
         while (!converged) {
-            eng_.cycle(l, r_top_.data(), UHIP_F64, c_top_.data(), UHIP_F64);
+            if (cycle_type_ == MG_FULL_CYCLE) eng_.full_cycle(l, r_top_.data(), c_top_.data(), v_cycle_repetition_);
+            else eng_.cycle(l, r_top_.data(), UHIP_F64, c_top_.data(), UHIP_F64);
             x.axpy(1.0, c_top_);
             A.residual(x, rhs, r_top_);
             r_norm = r_top_.norm2();
@@ -558,13 +692,15 @@ public:
     // result is the answer: r = rhs - A*0 = rhs, x = 0 + c = c, and the closing
     // residual only feeds check_convergence, which stops at it >= max_it anyway.
     bool apply_zero_initial_guess(const HipVector &rhs, HipVector &z) override {
-        if (this->max_it_ != 1 || this->verbose_) return LinearSolver::apply_zero_initial_guess(rhs, z);
+        if (this->max_it_ != 1 || this->verbose_ || cycle_type_ != MG_MULTIPLICATIVE_CYCLE)
+            return LinearSolver::apply_zero_initial_guess(rhs, z);
         if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
         eng_.cycle(n_levels_ - 1, rhs.data(), UHIP_F64, z.data(), UHIP_F64);
         return true;
     }
     bool apply_zero_initial_guess_dot(const HipVector &rhs, HipVector &z, double *dot_d) override {
-        if (this->max_it_ != 1 || this->verbose_) return LinearSolver::apply_zero_initial_guess_dot(rhs, z, dot_d);
+        if (this->max_it_ != 1 || this->verbose_ || cycle_type_ != MG_MULTIPLICATIVE_CYCLE)
+            return LinearSolver::apply_zero_initial_guess_dot(rhs, z, dot_d);
         if (z.empty() || z.size() != rhs.size()) z.zeros_like(rhs);
         vcycle_apply_dot(eng_, rhs, z, dot_d);
         return true;
@@ -580,8 +716,9 @@ private:
     std::vector<std::shared_ptr<HipCrsMatrix>> I_, R_;
     std::vector<std::shared_ptr<const HipCrsMatrix>> A_, given_;
     std::vector<HipVector> mask_;
-    int n_levels_ = 0, pre_ = 3, post_ = 3, mg_type_ = 1;
-    bool masks_ = true, fix_semi_ = false, galerkin_ = true;
+    int n_levels_ = 0, pre_ = 3, post_ = 3, mg_type_ = 1, cycle_type_ = MG_MULTIPLICATIVE_CYCLE;
+    int64_t v_cycle_repetition_ = 10;
+    bool masks_ = true, fix_semi_ = false, galerkin_ = true, line_search_ = false;
     VCycle<double> eng_;
     HipVector r_top_, c_top_;
     double setup_s_ = 0;
@@ -608,9 +745,12 @@ public:
         eng_.mg_type = src.mg_type;
         eng_.masks = src.masks;
         eng_.smoother = src.smoother;
-        mg_->coarse_solver()->prepare_f32();
+        eng_.line_search = src.line_search;
         eng_.coarse = src.coarse;
-        if (eng_.smoother != MG_CHEBYSHEV) throw HipError(UHIP_ERR_UNSUPPORTED, "mixed precision: Chebyshev smoother only");
+        if (eng_.smoother == MG_GAUSS_SEIDEL)
+            throw HipError(UHIP_ERR_UNSUPPORTED, "mixed precision: Chebyshev, Jacobi or PointJacobi smoothing");
+        if (mg_->cycle_type() != MG_MULTIPLICATIVE_CYCLE)
+            throw HipError(UHIP_ERR_UNSUPPORTED, "mixed precision: multiplicative cycle only");
         for (size_t l = 0; l < src.levels.size(); ++l) {
             auto &d = eng_.levels[l];
             const auto &s = src.levels[l];
@@ -623,6 +763,12 @@ public:
                 d.mask = HipVector32(op->context(), s.mask.size());
                 hip_check(uhip_vec_cast_f64_f32(op->c(), s.mask.size(), s.mask.data(), d.mask.data()), "mask cast");
             }
+            // Jacobi / PointJacobi: FP32 copies of the FP64 d_inv and LU
+            if (!s.dinv.empty()) {
+                d.dinv = HipVector32(op->context(), s.dinv.size());
+                hip_check(uhip_vec_cast_f64_f32(op->c(), s.dinv.size(), s.dinv.data(), d.dinv.data()), "d_inv cast");
+            }
+            if (s.LU) d.LU = std::make_shared<HipCrsMatrix32>(s.LU->cast_f32());
             d.eig_max = s.eig_max;
             d.eig_min = s.eig_min;
         }

@@ -365,8 +365,11 @@ private:
 
 // Chebyshev three-term coefficients of Chebyshev3level::solve
 // (utopia_Chebyshev3level.hpp:138-152) for nu steps.
+// Chebyshev3level::solve updates x before its first convergence check
+// (utopia_Chebyshev3level.hpp:137-160), so sweeps(0) still takes one step: nu < 1 counts as 1.
 inline void chebyshev_coefficients(double eig_max, double eig_min, int nu, std::vector<double> &alpha,
                                    std::vector<double> &beta) {
+    nu = std::max(nu, 1);
     const double avg_eig = (eig_max + eig_min) / 2.0;
     const double diff_eig = (eig_max - eig_min) / 2.0;
     alpha.assign((size_t)std::max(nu, 1), 0.0);
@@ -398,6 +401,7 @@ bool chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, 
                             int out_dt, int nu, const std::vector<double> &alpha, const std::vector<double> &beta,
                             T *rk, T *pa, T *pb, const double *dot_with = nullptr, double *dot_d = nullptr) {
     constexpr int dt = HipDtype<T>::value;
+    nu = std::max(nu, 1);  // sweeps(0) takes one step, as chebyshev_coefficients
     uhip_ctx c = A.c();
     const int64_t n = A.rows();
     const void *pprev;

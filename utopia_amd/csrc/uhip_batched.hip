@@ -13,6 +13,12 @@
 
 namespace uhip {
 
+// A kernel of the batched family returns at once when the context's gate (a device
+// counter, uhip_ctx_set_gate) reads 0: a captured run of CG iterations then costs
+// only its launches once every (system, rhs) pair has converged.
+#define UHIP_GATE() \
+    if (gate && *gate == 0) return
+
 // SpMM: one thread per row holds all K sums; each stored entry's value and
 // column are read once and the K x values of that column are one contiguous
 // K*sizeof(TX)-byte segment. EB entries' loads are issued before their
@@ -26,9 +32,10 @@ __device__ __forceinline__ const TX *xrow(const TX *X, int32_t c) {
 }
 
 template <int K, typename TAcc, class Epi, typename TA, typename TX>
-__global__ __launch_bounds__(256) void k_spmm_csr(int32_t n, const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(256) void k_spmm_csr(const int32_t *__restrict__ gate, int32_t n, const int32_t *__restrict__ rowptr,
                                                   const int32_t *__restrict__ col, const TA *__restrict__ val,
                                                   const TX *__restrict__ X, Epi epi) {
+    UHIP_GATE();
     constexpr int EB = K >= 16 ? 2 : 4;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int32_t r = b * 256 + (int32_t)threadIdx.x;
@@ -174,9 +181,10 @@ struct EpiKCheb {
 // Long rows (dense coarse blocks): one wavefront per row, lane l takes entries
 // l, l+64, ... in order, and a fixed butterfly combines the lanes.
 template <int K, typename TAcc, class Epi, typename TA, typename TX>
-__global__ __launch_bounds__(256) void k_spmm_wave(int32_t n, const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(256) void k_spmm_wave(const int32_t *__restrict__ gate, int32_t n, const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col, const TA *__restrict__ val,
                                                    const TX *__restrict__ X, Epi epi) {
+    UHIP_GATE();
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int32_t r = b * 4 + (int32_t)(threadIdx.x >> 6);
     const int l = threadIdx.x & 63;
@@ -205,11 +213,11 @@ int launch_spmm(uhip_ctx ctx, const uhip_csr *A, const TX *X, const Epi &e) {
     if (n == 0) return UHIP_OK;
     const bool wave = A->spmm_kernel == 2 || (A->spmm_kernel != 1 && A->nnz >= 32LL * n);
     if (wave)
-        hipLaunchKernelGGL((k_spmm_wave<K, TAcc, Epi, TA, TX>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n,
-                           A->rowptr, A->col, (const TA *)A->val, X, e);
+        hipLaunchKernelGGL((k_spmm_wave<K, TAcc, Epi, TA, TX>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream,
+                           ctx->gate, n, A->rowptr, A->col, (const TA *)A->val, X, e);
     else
-        hipLaunchKernelGGL((k_spmm_csr<K, TAcc, Epi, TA, TX>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
-                           A->rowptr, A->col, (const TA *)A->val, X, e);
+        hipLaunchKernelGGL((k_spmm_csr<K, TAcc, Epi, TA, TX>), dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                           ctx->gate, n, A->rowptr, A->col, (const TA *)A->val, X, e);
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
 }
@@ -217,9 +225,10 @@ int launch_spmm(uhip_ctx ctx, const uhip_csr *A, const TX *X, const Epi &e) {
 // ---- segmented dots: out[s*K + j] = sum_{i in seg s} x[i,j] * y[i,j] ----
 // Chunks of rows never straddle a segment; chunk c covers [c_row0[c], c_row1[c]).
 template <int K, typename TX, typename TY>
-__global__ __launch_bounds__(256) void k_segdot_chunk(const int32_t *__restrict__ c_row0,
+__global__ __launch_bounds__(256) void k_segdot_chunk(const int32_t *__restrict__ gate, const int32_t *__restrict__ c_row0,
                                                       const int32_t *__restrict__ c_row1, const TX *__restrict__ x,
                                                       const TY *__restrict__ y, double *__restrict__ partials) {
+    UHIP_GATE();
     constexpr int RPB = 256 / K;
     __shared__ double red[256];
     const int c = blockIdx.x;
@@ -242,9 +251,10 @@ __global__ __launch_bounds__(256) void k_segdot_chunk(const int32_t *__restrict_
 template <int K>
 // One wavefront per segment: lane l sums the segment's chunks l, l+64, ... in
 // order, then a fixed butterfly combines the lanes (deterministic).
-__global__ __launch_bounds__(64) void k_segdot_finish(int32_t nseg, const int32_t *__restrict__ seg_c0,
+__global__ __launch_bounds__(64) void k_segdot_finish(const int32_t *__restrict__ gate, int32_t nseg, const int32_t *__restrict__ seg_c0,
                                                       const double *__restrict__ partials, double *__restrict__ out,
                                                       int take_sqrt) {
+    UHIP_GATE();
     const int s = blockIdx.x;
     const int l = threadIdx.x;
     const int32_t c0 = seg_c0[s], c1 = seg_c0[s + 1];
@@ -270,7 +280,8 @@ struct BCGState {
     int32_t *flags, *conv_it;
 };
 
-__global__ __launch_bounds__(256) void k_bcg_alpha(int32_t m, BCGState st) {
+__global__ __launch_bounds__(256) void k_bcg_alpha(const int32_t *__restrict__ gate, int32_t m, BCGState st) {
+    UHIP_GATE();
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= m) return;
     double a = st.rz[t] / st.pap[t];
@@ -284,10 +295,11 @@ __global__ __launch_bounds__(256) void k_bcg_alpha(int32_t m, BCGState st) {
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_bcg_update(int32_t n, const int32_t *__restrict__ rowseg,
+__global__ __launch_bounds__(256) void k_bcg_update(const int32_t *__restrict__ gate, int32_t n, const int32_t *__restrict__ rowseg,
                                                     const double *__restrict__ alpha, double *__restrict__ x,
                                                     const double *__restrict__ p, double *__restrict__ r,
                                                     const double *__restrict__ Ap) {
+    UHIP_GATE();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)n * K) return;
     const int32_t row = (int32_t)(i / K);
@@ -300,8 +312,9 @@ __global__ __launch_bounds__(256) void k_bcg_update(int32_t n, const int32_t *__
 // after |r| is known: converge (flag 1) where |r| <= atol; count the still active ones
 // iter (device) counts the iterations done; it is advanced here so that a
 // captured iteration replays with the right count.
-__global__ __launch_bounds__(256) void k_bcg_check(int32_t m, BCGState st, int32_t *__restrict__ iter,
+__global__ __launch_bounds__(256) void k_bcg_check(const int32_t *__restrict__ gate, int32_t m, BCGState st, int32_t *__restrict__ iter,
                                                    int32_t *__restrict__ active_count) {
+    UHIP_GATE();
     __shared__ int32_t cnt[256];
     const int32_t it = *iter + 1;
     int32_t c = 0;
@@ -327,7 +340,8 @@ __global__ __launch_bounds__(256) void k_bcg_check(int32_t m, BCGState st, int32
     }
 }
 
-__global__ __launch_bounds__(256) void k_bcg_beta(int32_t m, BCGState st) {
+__global__ __launch_bounds__(256) void k_bcg_beta(const int32_t *__restrict__ gate, int32_t m, BCGState st) {
+    UHIP_GATE();
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= m) return;
     st.beta[t] = (st.flags[t] == 0) ? st.rznew[t] / st.rz[t] : 0.0;
@@ -335,9 +349,10 @@ __global__ __launch_bounds__(256) void k_bcg_beta(int32_t m, BCGState st) {
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_bcg_pupdate(int32_t n, const int32_t *__restrict__ rowseg,
+__global__ __launch_bounds__(256) void k_bcg_pupdate(const int32_t *__restrict__ gate, int32_t n, const int32_t *__restrict__ rowseg,
                                                      const double *__restrict__ beta, double *__restrict__ p,
                                                      const double *__restrict__ z) {
+    UHIP_GATE();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)n * K) return;
     const int32_t row = (int32_t)(i / K);
@@ -346,9 +361,10 @@ __global__ __launch_bounds__(256) void k_bcg_pupdate(int32_t n, const int32_t *_
 
 // x_i = xin_i + a[seg]*p_i (Chebyshev with one sweep), dtype conversions
 template <int K, typename TP, typename TX, typename TO, typename TAcc>
-__global__ __launch_bounds__(256) void k_axpy_seg(int32_t n, const int32_t *__restrict__ rowseg,
+__global__ __launch_bounds__(256) void k_axpy_seg(const int32_t *__restrict__ gate, int32_t n, const int32_t *__restrict__ rowseg,
                                                   const double *__restrict__ coef, const TP *__restrict__ p,
                                                   const TX *__restrict__ xin, TO *__restrict__ out) {
+    UHIP_GATE();
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)n * K) return;
     const TAcc a = (TAcc)coef[3 * rowseg[i / K]];
@@ -518,7 +534,7 @@ int uhip_axpy_seg(uhip_ctx ctx, int32_t n, int k, const int32_t *rowseg, const d
     const int64_t m = (int64_t)n * k;
     dim3 g((unsigned)((m + 255) / 256)), b(256);
 #define UHIP_AXS(TP, TX, TO, TA)                                                                              \
-    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_axpy_seg<KK, TP, TX, TO, TA>), g, b, 0, ctx->stream, n, rowseg, coef, \
+    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_axpy_seg<KK, TP, TX, TO, TA>), g, b, 0, ctx->stream, ctx->gate, n, rowseg, coef, \
                                          (const TP *)p, (const TX *)xin, (TO *)out))
     if (level_dtype == UHIP_F64) {
         UHIP_AXS(double, double, double, double);
@@ -543,16 +559,16 @@ int uhip_segdot(uhip_ctx ctx, int k, int32_t nchunks, const int32_t *c_row0, con
     UHIP_TRY(ensure_partials(ctx, (int64_t)nchunks * k));
     if (dtype == UHIP_F64) {
         UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_chunk<KK, double, double>), dim3(nchunks), dim3(256), 0,
-                                             ctx->stream, c_row0, c_row1, (const double *)x, (const double *)y,
+                                             ctx->stream, ctx->gate, c_row0, c_row1, (const double *)x, (const double *)y,
                                              ctx->partials));
     } else {
         UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_chunk<KK, float, float>), dim3(nchunks), dim3(256), 0,
-                                             ctx->stream, c_row0, c_row1, (const float *)x, (const float *)y,
+                                             ctx->stream, ctx->gate, c_row0, c_row1, (const float *)x, (const float *)y,
                                              ctx->partials));
     }
     UHIP_CHECK_LAUNCH();
     const int m = nseg * k;
-    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_finish<KK>), dim3(nseg), dim3(64), 0, ctx->stream, nseg, seg_c0,
+    UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_segdot_finish<KK>), dim3(nseg), dim3(64), 0, ctx->stream, ctx->gate, nseg, seg_c0,
                                          ctx->partials, out, take_sqrt));
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
@@ -568,23 +584,23 @@ int uhip_bcg_step(uhip_ctx ctx, int stage, int32_t n, int k, int32_t m, const in
     dim3 gm((m + 255) / 256), gv((unsigned)((nk + 255) / 256)), b(256);
     switch (stage) {
         case 0:  // alpha and x, r update
-            hipLaunchKernelGGL(k_bcg_alpha, gm, b, 0, ctx->stream, m, st);
+            hipLaunchKernelGGL(k_bcg_alpha, gm, b, 0, ctx->stream, ctx->gate, m, st);
             UHIP_CHECK_LAUNCH();
             if (nk > 0) {
-                UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_bcg_update<KK>), gv, b, 0, ctx->stream, n, rowseg, st.alpha, x,
+                UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_bcg_update<KK>), gv, b, 0, ctx->stream, ctx->gate, n, rowseg, st.alpha, x,
                                                      (const double *)p, r, Ap));
                 UHIP_CHECK_LAUNCH();
             }
             return UHIP_OK;
         case 1:  // convergence check (rr holds |r|)
-            hipLaunchKernelGGL(k_bcg_check, dim3(1), b, 0, ctx->stream, m, st, iter, active_count);
+            hipLaunchKernelGGL(k_bcg_check, dim3(1), b, 0, ctx->stream, ctx->gate, m, st, iter, active_count);
             UHIP_CHECK_LAUNCH();
             return UHIP_OK;
         case 2:  // beta and p update
-            hipLaunchKernelGGL(k_bcg_beta, gm, b, 0, ctx->stream, m, st);
+            hipLaunchKernelGGL(k_bcg_beta, gm, b, 0, ctx->stream, ctx->gate, m, st);
             UHIP_CHECK_LAUNCH();
             if (nk > 0) {
-                UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_bcg_pupdate<KK>), gv, b, 0, ctx->stream, n, rowseg, st.beta, p,
+                UHIP_K_SWITCH(k, hipLaunchKernelGGL((k_bcg_pupdate<KK>), gv, b, 0, ctx->stream, ctx->gate, n, rowseg, st.beta, p,
                                                      z));
                 UHIP_CHECK_LAUNCH();
             }

@@ -70,6 +70,48 @@ __global__ __launch_bounds__(kBlock) void k_ediv(int64_t n, const T *__restrict_
     UHIP_FOR4(i, n) { z[i] = x[i] / y[i]; }
 }
 
+// z = min(x, y) / max(x, y) entry by entry in the form of PETSc's VecPointwiseMin /
+// VecPointwiseMax (PetscMin(a, b) = a < b ? a : b, PetscMax(a, b) = a < b ? b : a),
+// called by PetscVector::e_min / e_max (backend/petsc/utopia_petsc_Vector.cpp:597-603).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_emin(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
+                                                 T *__restrict__ z) {
+    UHIP_FOR4(i, n) {
+        const T a = x[i], b = y[i];
+        z[i] = a < b ? a : b;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_emax(int64_t n, const T *__restrict__ x, const T *__restrict__ y,
+                                                 T *__restrict__ z) {
+    UHIP_FOR4(i, n) {
+        const T a = x[i], b = y[i];
+        z[i] = a < b ? b : a;
+    }
+}
+
+// PetscVector::reciprocal(num) (backend/petsc/utopia_petsc_Vector.hpp:711-720): VecReciprocal
+// (1/x where x != 0, zeros kept), then a scale by num unless num == 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_reciprocal(int64_t n, T num, const T *__restrict__ x,
+                                                       T *__restrict__ y) {
+    UHIP_FOR4(i, n) {
+        const T v = x[i];
+        const T r = v != T(0) ? T(1) / v : v;
+        y[i] = num == T(1) ? r : num * r;
+    }
+}
+
+// x_i = to where x_i == from (PointJacobi::check_indef turns zeros into ones,
+// solvers/smoothers/utopia_PointJacobi.hpp:116-127)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_replace(int64_t n, T from, T to, T *__restrict__ x) {
+    UHIP_FOR4(i, n) {
+        if (x[i] == from) x[i] = to;
+    }
+}
+
 // y_i = 1/x_i where |x_i| > tol, else 0 (utopia_ElementWisePseudoInverse.hpp:20-56).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pseudo_inv(int64_t n, const T *__restrict__ x, T *__restrict__ y,
@@ -230,6 +272,18 @@ using namespace uhip;
     }                                                                                                            \
     int uhip_vec_ediv_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z) {                             \
         UHIP_LAUNCH1(k_ediv<T>, n, x, y, z);                                                                     \
+    }                                                                                                            \
+    int uhip_vec_emin_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z) {                             \
+        UHIP_LAUNCH1(k_emin<T>, n, x, y, z);                                                                     \
+    }                                                                                                            \
+    int uhip_vec_emax_##SUF(uhip_ctx ctx, int64_t n, const T *x, const T *y, T *z) {                             \
+        UHIP_LAUNCH1(k_emax<T>, n, x, y, z);                                                                     \
+    }                                                                                                            \
+    int uhip_vec_reciprocal_##SUF(uhip_ctx ctx, int64_t n, T num, const T *x, T *y) {                            \
+        UHIP_LAUNCH1(k_reciprocal<T>, n, num, x, y);                                                             \
+    }                                                                                                            \
+    int uhip_vec_replace_##SUF(uhip_ctx ctx, int64_t n, T from, T to, T *x) {                                    \
+        UHIP_LAUNCH1(k_replace<T>, n, from, to, x);                                                              \
     }                                                                                                            \
     int uhip_vec_pseudo_inv_##SUF(uhip_ctx ctx, int64_t n, const T *x, T *y, T tol) {                            \
         UHIP_LAUNCH1(k_pseudo_inv<T>, n, x, y, tol);                                                             \

@@ -54,7 +54,11 @@ int uhip_bsolver_destroy(uhip_bsolver s) {
 
 int uhip_bsolver_set_param(uhip_bsolver s, const char *key, double value) {
     if (!s || !key) return UHIP_ERR_INVALID;
-    s->solver->set_param(key, value);
+    try {
+        s->solver->set_param(key, value);
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
     return UHIP_OK;
 }
 
@@ -131,6 +135,7 @@ int uhip_bsolver_get_info(uhip_bsolver s, const char *key, double *value) {
     else if (k == "solve_seconds") *value = b.solve_seconds();
     else if (k == "iterations") *value = (double)b.iterations();
     else if (k == "graph_replays") *value = (double)b.graph_replays();
+    else if (k == "graph_captures") *value = (double)b.graph_captures();
     else if (k == "n_levels") *value = (double)b.levels();
     else if (k.rfind("eig_max_", 0) == 0) {
         int l = -1, sys = -1;

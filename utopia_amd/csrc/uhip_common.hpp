@@ -53,6 +53,8 @@ struct uhip_context_s {
     int num_cus = 256;
     double *partials = nullptr;     // kMaxDots * max_partials doubles
     int64_t max_partials = 0;       // capacity per dot slot
+    int64_t workspace_gen = 0;      // bumped whenever partials is (re)allocated
+    const int32_t *gate = nullptr;  // device counter: batched kernels skip their work while it reads 0
     double *dev_scalars = nullptr;  // UHIP_NUM_SCALARS device scalar slots
     double *host_scalars = nullptr; // pinned mirror used for read-back
     unsigned int *counters = nullptr;

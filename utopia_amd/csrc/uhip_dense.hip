@@ -277,12 +277,4 @@ int uhip_dense_gemv(uhip_ctx ctx, int32_t n, const double *M, const void *x, int
     return UHIP_OK;
 }
 
-int uhip_dense_gemv_m32(uhip_ctx ctx, int32_t n, const float *M, const float *x, float *y) {
-    UHIP_REQUIRE(ctx && M && x && y && n >= 0, "uhip_dense_gemv_m32: bad argument");
-    if (n == 0) return UHIP_OK;
-    hipLaunchKernelGGL((k_gemv<float, float, float>), dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, M, x, y);
-    UHIP_CHECK_LAUNCH();
-    return UHIP_OK;
-}
-
 }  // extern "C"

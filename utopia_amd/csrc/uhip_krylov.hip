@@ -5,13 +5,14 @@
 
 namespace uhip {
 
-// out = xin + a * p   (xin may be null = 0), with dtype conversion.
+// out = xin + a * p   (xin may be null = 0), with dtype conversion. a == 0 adds nothing
+// (VecAXPY with alpha 0 is a no-op), so the call is an exact copy / cast of xin.
 template <typename TP, typename TX, typename TO, typename TAcc>
 __global__ __launch_bounds__(kBlock) void k_axpy_cast(int64_t n, TAcc a, const TP *__restrict__ p,
                                                       const TX *__restrict__ xin, TO *__restrict__ out) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         TAcc v = xin ? (TAcc)xin[i] : TAcc(0);
-        v = v + a * (TAcc)p[i];
+        if (a != TAcc(0)) v = v + a * (TAcc)p[i];
         out[i] = (TO)v;
     }
 }

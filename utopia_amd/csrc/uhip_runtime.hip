@@ -58,6 +58,7 @@ int ensure_partials(uhip_ctx ctx, int64_t per_slot) {
     }
     UHIP_CHECK_HIP(hipMalloc(&ctx->partials, sizeof(double) * kMaxDots * cap));
     ctx->max_partials = cap;
+    ++ctx->workspace_gen;
     return UHIP_OK;
 }
 
@@ -133,6 +134,14 @@ int uhip_ctx_destroy(uhip_ctx ctx) {
     if (ctx->counters) hipFree(ctx->counters);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
+    return UHIP_OK;
+}
+
+int64_t uhip_ctx_workspace_generation(uhip_ctx ctx) { return ctx ? ctx->workspace_gen : -1; }
+
+int uhip_ctx_set_gate(uhip_ctx ctx, const int32_t *gate_d) {
+    UHIP_REQUIRE(ctx, "uhip_ctx_set_gate: null context");
+    ctx->gate = gate_d;
     return UHIP_OK;
 }
 

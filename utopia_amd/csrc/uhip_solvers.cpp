@@ -72,6 +72,8 @@ std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s, bool with_transfers 
     std::shared_ptr<IterativeSolver<HipCrsMatrix, HipVector>> sm;
     if ((int)param(s, "smoother", 0) == 1) {
         sm = std::make_shared<Jacobi<HipCrsMatrix, HipVector>>();
+    } else if ((int)param(s, "smoother", 0) == 3) {
+        sm = std::make_shared<PointJacobi<HipCrsMatrix, HipVector>>();
     } else if ((int)param(s, "smoother", 0) == 2) {
         auto gs = std::make_shared<GaussSeidel>();
         gs->use_symmetric_sweep(param(s, "use_symmetric_sweep", 1) != 0);
@@ -88,6 +90,9 @@ std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s, bool with_transfers 
     mg->pre_smoothing_steps((int)param(s, "pre_smoothing_steps", 3));
     mg->post_smoothing_steps((int)param(s, "post_smoothing_steps", 3));
     mg->mg_type((int)param(s, "mg_type", 1));
+    mg->cycle_type((int)param(s, "cycle_type", MG_MULTIPLICATIVE_CYCLE));
+    mg->v_cycle_repetition((int64_t)param(s, "v_cycle_repetition", 10));
+    mg->use_line_search(param(s, "use_line_search", 0) != 0);
     mg->must_generate_masks(param(s, "must_generate_masks", 1) != 0);
     mg->fix_semidefinite_operators(param(s, "fix_semidefinite_operators", 0) != 0);
     if (with_transfers) mg->set_transfer_operators(s->transfers);

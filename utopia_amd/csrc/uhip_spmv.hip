@@ -561,6 +561,18 @@ __global__ __launch_bounds__(kBlock) void k_fix_semidef(int32_t n, const int32_t
         if (col[j] == r && fabs((double)val[j]) < 1e-12) val[j] = val[j] + T(1);
 }
 
+// A_rr += alpha*d_r where row r stores its diagonal (MatDiagonalSet with ADD_VALUES,
+// PetscMatrix::shift_diag, backend/petsc/utopia_petsc_Matrix.cpp:1783-1788)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_shift_diag(int32_t n, const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col, T *__restrict__ val,
+                                                       const T *__restrict__ d, T alpha) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    for (int32_t j = rowptr[r]; j < rowptr[r + 1]; ++j)
+        if (col[j] == r) val[j] = val[j] + (alpha == T(1) ? d[r] : alpha * d[r]);
+}
+
 // ind_r = 1 if row r has an off-diagonal entry with |a| > tol, else 0.
 template <typename TA, typename TV>
 __global__ __launch_bounds__(kBlock) void k_offdiag_ind(int32_t n, const int32_t *__restrict__ rowptr,
@@ -883,6 +895,32 @@ int uhip_spmv_add(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
     return launch_rowblock<float>(ctx, A, (const float *)x, e);
 }
 
+int uhip_spmv_jacobi(uhip_ctx ctx, const uhip_csr *A, const void *xin, const void *b, int b_dtype, const void *d,
+                     void *out, int out_dtype, int add) {
+    UHIP_REQUIRE(ctx && A && b && d && out, "uhip_spmv_jacobi: null argument");
+    UHIP_REQUIRE(A->nrows == A->ncols, "uhip_spmv_jacobi: matrix must be square");
+    UHIP_REQUIRE(out != xin && out != b && out != d, "uhip_spmv_jacobi: out must not alias xin, b or d");
+    // a zero guess is a zero vector of the caller's: the reference forms A*0 too
+    UHIP_REQUIRE(xin, "uhip_spmv_jacobi: xin required (a zero vector for a zero guess)");
+    if (A->dtype == UHIP_F64)
+        UHIP_REQUIRE(b_dtype == UHIP_F64 && out_dtype == UHIP_F64, "uhip_spmv_jacobi: FP64 matrix needs FP64 b, out");
+#define UHIP_JAC(TACC, TB, TXO)                                                                                 \
+    do {                                                                                                        \
+        if (add) {                                                                                              \
+            EpiJacobi<TACC, TB, TXO, true> e{(const TB *)b, (const TACC *)d, (const TACC *)xin, (TXO *)out};   \
+            return launch_rowblock<TACC>(ctx, A, (const TACC *)xin, e);                                         \
+        }                                                                                                       \
+        EpiJacobi<TACC, TB, TXO, false> e{(const TB *)b, (const TACC *)d, (const TACC *)xin, (TXO *)out};      \
+        return launch_rowblock<TACC>(ctx, A, (const TACC *)xin, e);                                             \
+    } while (0)
+    if (A->dtype == UHIP_F64) UHIP_JAC(double, double, double);
+    if (b_dtype == UHIP_F64 && out_dtype == UHIP_F64) UHIP_JAC(float, double, double);
+    if (b_dtype == UHIP_F64) UHIP_JAC(float, double, float);
+    if (out_dtype == UHIP_F64) UHIP_JAC(float, float, double);
+    UHIP_JAC(float, float, float);
+#undef UHIP_JAC
+}
+
 }  // extern "C"
 
 namespace uhip {
@@ -936,6 +974,21 @@ int uhip_csr_diag(uhip_ctx ctx, const uhip_csr *A, void *d) {
     else
         hipLaunchKernelGGL(k_diag<float>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
                            (const float *)A->val, (float *)d);
+    UHIP_CHECK_LAUNCH();
+    return UHIP_OK;
+}
+
+int uhip_csr_shift_diag(uhip_ctx ctx, uhip_csr *A, const void *d, double alpha) {
+    UHIP_REQUIRE(ctx && A && d, "uhip_csr_shift_diag: null argument");
+    const int32_t n = A->nrows < A->ncols ? A->nrows : A->ncols;
+    if (n == 0) return UHIP_OK;
+    dim3 g((n + kBlock - 1) / kBlock);
+    if (A->dtype == UHIP_F64)
+        hipLaunchKernelGGL(k_shift_diag<double>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (double *)A->val, (const double *)d, alpha);
+    else
+        hipLaunchKernelGGL(k_shift_diag<float>, g, dim3(kBlock), 0, ctx->stream, n, A->rowptr, A->col,
+                           (float *)A->val, (const float *)d, (float)alpha);
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
 }

@@ -736,6 +736,32 @@ struct EpiAddTo {
     __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
+// One Jacobi sweep fused with its SpMV (Jacobi::sweep, solvers/smoothers/utopia_Jacobi.hpp:132-138):
+// with s = (A x)_r, r = b_r - s, then out_r = x_r + d_r*r (ADD, x read from xin; null xin is
+// x = 0); without ADD out_r = d_r*r (PointJacobi::sweep with A = LU,
+// solvers/smoothers/utopia_PointJacobi.hpp:132-142). The residual, the product and the sum
+// are rounded one by one as the reference's three vector operations round them.
+template <typename TAcc, typename TB, typename TXO, bool ADD>
+struct EpiJacobi {
+    static constexpr int kDots = 0;
+    struct In {
+        TAcc b, d, x;
+    };
+    const TB *b;
+    const TAcc *d;
+    const TAcc *xin;
+    TXO *out;
+    __device__ In load(int r) const {
+        return In{(TAcc)b[r], d[r], (ADD && xin) ? xin[r] : TAcc(0)};
+    }
+    __device__ void apply(int r, TAcc s, const In &in, double *) const {
+        const TAcc res = in.b - s;
+        const TAcc t = in.d * res;
+        st_stream(out + r, (TXO)(ADD ? in.x + t : t));
+    }
+    __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
+};
+
 // One step of Chebyshev3level::solve (utopia_Chebyshev3level.hpp:138-175) fused
 // with the SpMV that feeds it. With s = (A p_{k-1})_r:
 //   r_k = r_{k-1} + a_prev*s ; p_k = -r_k + beta*p_{k-1} ; x += a_prev*p_{k-1}
