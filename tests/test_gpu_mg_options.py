@@ -83,8 +83,13 @@ def test_mg_pcg_jacobi_smoothers_and_zero_pre_match_oracle(ctx, smoother, pre):
     print(f"cg-mg smoother={smoother} pre={pre}: gpu it={st['iterates']} oracle it={so['iterates']} "
           f"rel.err={rel(x, xo):.2e}")
     assert abs(st["iterates"] - so["iterates"]) <= 1
-    assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol
     assert rel(x, xo) <= 1e-10
+    if smoother == 0 and pre == 0:
+        # Chebyshev3level takes one step for sweeps(0): one pre- against two post-steps make
+        # the V-cycle non-symmetric, and CG stalls in the reference's arithmetic as well
+        assert so["iterates"] == 800 and st["iterates"] == 800
+    else:
+        assert np.linalg.norm(b - A.to_scipy() @ x) <= 1.01 * atol
 
 
 @pytest.mark.parametrize("smoother", [1, 3])
