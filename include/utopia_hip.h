@@ -264,6 +264,14 @@ int uhip_spmv_residual(uhip_ctx ctx, const uhip_csr *A, const void *x, const voi
 /* y = A x and dot_d[0] = x.y      A.apply(p,Ap) + dot(p,Ap) of ConjugateGradient
  * (solvers/smoothers/utopia_ConjugateGradient_impl.hpp:237-238) */
 int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, double *dot_d);
+/* uhip_spmv_dot allocating nothing (no reference counterpart): for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops). Same kernels and results
+ * bit for bit; the per-workgroup partial sums go to a file-scope device array of 2^18 doubles
+ * (2 MiB, one per GPU, held until the process ends) instead of the context's workspace, which
+ * uhip_spmv_dot grows with hipMalloc on demand. A must already carry its plan; a plan with more
+ * than 2^18 workgroups is rejected (UHIP_ERR_INVALID). Calls on one GPU share the array:
+ * overlapping calls (other streams, other contexts) race. */
+int uhip_spmv_dot_noalloc(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, double *dot_d);
 /* y = A x, dots_d[0] = x.y, dots_d[1] = z.x: MPRGP's A p, p.Ap and g.p (or A beta, beta.A beta,
  * g.beta) in one pass (solvers/nonlinear/constrained/quadratic_programming/utopia_MPRGP.hpp:126-128,
  * 211-213) */

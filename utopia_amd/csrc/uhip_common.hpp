@@ -112,6 +112,12 @@ inline int grid_for(int64_t n, int per_thread = 4, int cap = 1 << 20) {
 // summed in a fixed order (thread-strided, then a fixed tree); slot k is
 // replaced by its square root when bit k of sqrt_mask is set.
 int finish_sums(uhip_ctx ctx, int k, int64_t nparts, int64_t stride, double *result_d, unsigned sqrt_mask = 0);
+// The same second stage over partial sums held elsewhere than ctx->partials.
+int finish_sums_from(uhip_ctx ctx, const double *partials, int k, int64_t nparts, int64_t stride, double *result_d,
+                     unsigned sqrt_mask = 0);
+// Device address of the SpMV library's file-scope partial-sum scratch on `device` (the device
+// must be current): looked up once per device at context creation.
+int spmv_scratch_init(int device);
 
 // Grow the per-slot capacity of ctx->partials to at least per_slot entries.
 int ensure_partials(uhip_ctx ctx, int64_t per_slot);

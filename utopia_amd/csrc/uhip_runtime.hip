@@ -40,11 +40,16 @@ __global__ __launch_bounds__(1024) void k_finish_sums(const double *__restrict__
     if (threadIdx.x == 0) result[k] = ((sqrt_mask >> k) & 1u) ? sqrt(s) : s;
 }
 
-int finish_sums(uhip_ctx ctx, int k, int64_t nparts, int64_t stride, double *result_d, unsigned sqrt_mask) {
-    hipLaunchKernelGGL(k_finish_sums, dim3(k), dim3(1024), 0, stream_of(ctx), ctx->partials, nparts, stride,
-                       result_d, sqrt_mask);
+int finish_sums_from(uhip_ctx ctx, const double *partials, int k, int64_t nparts, int64_t stride, double *result_d,
+                     unsigned sqrt_mask) {
+    hipLaunchKernelGGL(k_finish_sums, dim3(k), dim3(1024), 0, stream_of(ctx), partials, nparts, stride, result_d,
+                       sqrt_mask);
     UHIP_CHECK_LAUNCH();
     return UHIP_OK;
+}
+
+int finish_sums(uhip_ctx ctx, int k, int64_t nparts, int64_t stride, double *result_d, unsigned sqrt_mask) {
+    return finish_sums_from(ctx, ctx->partials, k, nparts, stride, result_d, sqrt_mask);
 }
 
 int ensure_partials(uhip_ctx ctx, int64_t per_slot) {
@@ -104,7 +109,8 @@ int uhip_ctx_create(int device, void *stream, uhip_ctx *out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
-    int st = ensure_partials(c, 1 << 16);
+    int st = spmv_scratch_init(device);
+    if (st == UHIP_OK) st = ensure_partials(c, 1 << 16);
     if (st == UHIP_OK && hipMalloc(&c->dev_scalars, sizeof(double) * UHIP_NUM_SCALARS) != hipSuccess)
         st = set_error(UHIP_ERR_ALLOC, "hipMalloc scalars");
     if (st == UHIP_OK && hipHostMalloc(&c->host_scalars, sizeof(double) * UHIP_NUM_SCALARS) != hipSuccess)

@@ -5,9 +5,41 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "uhip_spmv.hpp"
 
 namespace uhip {
+
+// ---- partial-sum scratch of uhip_spmv_dot_noalloc ---------------------------------
+// File-scope device array (each GPU the library runs on holds its own copy until the
+// process ends): kNoAllocParts per-workgroup partial sums of one dot. In host code its
+// name is a stand-in, not a device address: the address is looked up once per device
+// (spmv_scratch_init, from uhip_ctx_create with the device current) and only that
+// address is handed to kernels.
+constexpr int64_t kNoAllocParts = 1 << 18;  // 2 MiB
+static __device__ __align__(256) double scratch[kNoAllocParts];
+constexpr int kMaxScratchDevices = 64;
+static double *g_scratch_ptr[kMaxScratchDevices] = {};
+static std::mutex g_scratch_mu;
+
+int spmv_scratch_init(int device) {
+    UHIP_REQUIRE(device >= 0 && device < kMaxScratchDevices, "spmv scratch: device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return UHIP_OK;
+    double *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_error(UHIP_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return UHIP_OK;
+}
+
+static double *scratch_of(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 // Row r starts a block when its cost window (rowptr[r] + r) / B differs from the
 // previous row's, or when r or r-1 is a long row (> B/4 nonzeros). A multi-row
@@ -864,6 +896,26 @@ int uhip_spmv_dot(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, doubl
         UHIP_TRY(launch_rowblock<float>(ctx, A, (const float *)x, e));
     }
     return finish_sums(ctx, 1, A->nparts, ctx->max_partials, dot_d);
+}
+
+int uhip_spmv_dot_noalloc(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y, double *dot_d) {
+    UHIP_REQUIRE(ctx && A && x && y && dot_d, "uhip_spmv_dot_noalloc: null argument");
+    UHIP_REQUIRE(A->nrows == A->ncols, "uhip_spmv_dot_noalloc: matrix must be square");
+    const int64_t need = A->format == UHIP_FMT_ROWBLOCK ? A->nblocks : A->nparts;
+    UHIP_REQUIRE(need <= kNoAllocParts,
+                 "uhip_spmv_dot_noalloc: the plan has %lld workgroups, the scratch holds %lld partial sums "
+                 "(at most about 2^18 workgroups; use uhip_spmv_dot)",
+                 (long long)need, (long long)kNoAllocParts);
+    double *parts = scratch_of(ctx->device);
+    UHIP_REQUIRE(parts, "uhip_spmv_dot_noalloc: scratch not initialised on device %d", ctx->device);
+    if (A->dtype == UHIP_F64) {
+        EpiStoreDot<double, double, double> e{(const double *)x, (double *)y};
+        UHIP_TRY(launch_rowblock<double>(ctx, A, (const double *)x, e, parts, kNoAllocParts));
+    } else {
+        EpiStoreDot<float, float, float> e{(const float *)x, (float *)y};
+        UHIP_TRY(launch_rowblock<float>(ctx, A, (const float *)x, e, parts, kNoAllocParts));
+    }
+    return finish_sums_from(ctx, parts, 1, A->nparts, kNoAllocParts, dot_d);
 }
 
 int uhip_spmv_dot2_f64(uhip_ctx ctx, const uhip_csr *A, const double *x, double *y, const double *z,

@@ -509,20 +509,35 @@ struct IsEpiStore<EpiStore<TAcc, TY>> {
 
 // Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
 // Matrix values have the accumulation type TAcc.
+// With parts, the per-workgroup partial sums of a dot-carrying epilogue go to parts (cap
+// entries per dot) instead of the context's workspace, which is then neither read nor grown.
 template <typename TAcc, typename TX, class Epi>
-int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi) {
+int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi, double *parts_in = nullptr,
+                    int64_t cap = 0) {
     using TA = TAcc;
+    if (parts_in && Epi::kDots > 0) {
+        const int64_t need = A->format == UHIP_FMT_ROWBLOCK ? A->nblocks : A->nparts;
+        UHIP_REQUIRE(need <= cap, "spmv: the plan needs %lld partial sums, the buffer holds %lld", (long long)need,
+                     (long long)cap);
+    }
+    // partial sums: the caller's buffer, else the context's (grown on demand)
+    auto partials_for = [&](int64_t need) -> int {
+        if (parts_in || Epi::kDots == 0) return UHIP_OK;
+        return ensure_partials(ctx, need);
+    };
     if (A && A->format == UHIP_FMT_VI) {
         UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->vi_base && A->vi_table && A->ndiag >= 1 &&
                          A->ndiag <= 32 && A->vi_nvals >= 1 && A->vi_nvals <= 256,
                      "spmv: VI plan missing");
         if (A->nrows == 0) return UHIP_OK;
-        if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
+        UHIP_TRY(partials_for(A->nparts));
+        double *parts = parts_in ? parts_in : ctx->partials;
+        const int64_t pstride = parts_in ? cap : ctx->max_partials;
         const dim3 g(A->nparts), blk(kDiaWG);
 #define UHIP_VI_LAUNCH(TM, RR)                                                                                  \
     hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM, RR>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag,      \
                        A->dia_off, (const uint64_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals,         \
-                       A->vi_base, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
+                       A->vi_base, (const TM *)A->dia_mask, x, epi, parts, pstride)
         if (A->ndiag <= 8) UHIP_VI_LAUNCH(uint8_t, vi_rows<TA>(8));
         else if (A->ndiag <= 16) UHIP_VI_LAUNCH(uint16_t, 1);
         else UHIP_VI_LAUNCH(uint32_t, 1);
@@ -534,12 +549,14 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->ndiag >= 1 && A->ndiag <= 32,
                      "spmv: DIA plan missing");
         if (A->nrows == 0) return UHIP_OK;
-        if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
+        UHIP_TRY(partials_for(A->nparts));
+        double *parts = parts_in ? parts_in : ctx->partials;
+        const int64_t pstride = parts_in ? cap : ctx->max_partials;
         const dim3 g(A->nparts), blk(kDiaWG);
         const TA *v = (const TA *)A->dia_val;
 #define UHIP_DIA_LAUNCH(TM, SYM)                                                                                  \
     hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, TM, SYM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, \
-                       A->dia_off, v, (const TM *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials)
+                       A->dia_off, v, (const TM *)A->dia_mask, x, epi, parts, pstride)
         // FP64 plane-blocked storage: every product with dots and the plain product run the
         // plane-blocked kernel (its grid is nparts, the partial sums the finish reads); the
         // others (smoothing steps, residuals) run the symmetric kernel over the same arrays,
@@ -547,12 +564,11 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         const bool blocked = A->dia_sym == 2 && (sizeof(TA) == 4 || Epi::kDots > 0 || IsEpiStore<Epi>::value);
         if (blocked) {
             hipLaunchKernelGGL((k_spmv_dia7z<TAcc, Epi, TA, TX>), g, blk, 0, stream_of(ctx), A->nrows, A->dia_off, v,
-                               (const uint8_t *)A->dia_mask, x, epi, ctx->partials, ctx->max_partials);
+                               (const uint8_t *)A->dia_mask, x, epi, parts, pstride);
         } else if (A->dia_sym == 2) {
             const dim3 gs((A->nrows + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype));
             hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint8_t, true>), gs, blk, 0, stream_of(ctx), A->nrows,
-                               A->ndiag, A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, ctx->partials,
-                               ctx->max_partials);
+                               A->ndiag, A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, parts, pstride);
         } else if (A->ndiag <= 8) {
             if (A->dia_sym) UHIP_DIA_LAUNCH(uint8_t, true);
             else UHIP_DIA_LAUNCH(uint8_t, false);
@@ -570,34 +586,36 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
     if (A && A->format == UHIP_FMT_SELL64) {
         UHIP_REQUIRE(A->slice_off && A->sell_col && A->sell_val, "spmv: SELL-64 plan missing");
         if (A->nslices == 0) return UHIP_OK;
-        if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nparts));
+        UHIP_TRY(partials_for(A->nparts));
+        double *parts = parts_in ? parts_in : ctx->partials;
+        const int64_t pstride = parts_in ? cap : ctx->max_partials;
         hipLaunchKernelGGL((k_spmv_sell<TAcc, Epi, TA, TX>), dim3(A->nparts), dim3(64 * kSellWaves), 0,
                            stream_of(ctx), A->nrows, A->nslices, A->rowptr, A->slice_off, A->sell_col,
-                           (const TA *)A->sell_val, x, epi, ctx->partials, ctx->max_partials);
+                           (const TA *)A->sell_val, x, epi, parts, pstride);
         UHIP_CHECK_LAUNCH();
         return UHIP_OK;
     }
     UHIP_REQUIRE(A && A->blocks && A->nblocks >= 0, "spmv: matrix has no row-block plan");
     if (A->nblocks == 0) return UHIP_OK;
-    if (Epi::kDots > 0) UHIP_TRY(ensure_partials(ctx, A->nblocks));
+    UHIP_TRY(partials_for(A->nblocks));
     const int capk = A->block_capk;
     const TA *val = (const TA *)A->val;
     hipStream_t s = stream_of(ctx);
     dim3 g(A->nblocks), blk(kSpmvWG);
-    double *parts = ctx->partials;
-    const int64_t st = ctx->max_partials;
+    double *parts_rb = parts_in ? parts_in : ctx->partials;
+    const int64_t st = parts_in ? cap : ctx->max_partials;
     switch (capk) {
         case 5:
             hipLaunchKernelGGL((k_spmv_rowblock<TAcc, 5, Epi, TA, TX>), g, blk, 0, s, A->blocks, A->rowptr, A->col,
-                               val, x, epi, parts, st);
+                               val, x, epi, parts_rb, st);
             break;
         case 10:
             hipLaunchKernelGGL((k_spmv_rowblock<TAcc, 10, Epi, TA, TX>), g, blk, 0, s, A->blocks, A->rowptr, A->col,
-                               val, x, epi, parts, st);
+                               val, x, epi, parts_rb, st);
             break;
         case 20:
             hipLaunchKernelGGL((k_spmv_rowblock<TAcc, 20, Epi, TA, TX>), g, blk, 0, s, A->blocks, A->rowptr, A->col,
-                               val, x, epi, parts, st);
+                               val, x, epi, parts_rb, st);
             break;
         default:
             return set_error(UHIP_ERR_INVALID, "spmv: unsupported plan capk=%d", capk);
