@@ -38,13 +38,25 @@ def spmv_bytes(n, nnz, val_bytes):
     return (val_bytes + 4) * nnz + (4 + 2 * val_bytes) * n + 4
 
 
-def dia_bytes(n, ndiag, val_bytes, sym=0):
-    # DIA: ndiag value slots per row (absent entries stored as zeros; symmetric
-    # storage keeps the ndiag // 2 + 1 offsets >= 0), the per-row mask (1/2/4 B for
-    # <= 8/16/32 offsets), x read once, y written; offsets table
+def dia_bytes(n, ndiag, val_bytes, sym=0, nan_select=False):
+    # DIA: ndiag value slots per row (absent entries NaN - or zero - in the value arrays;
+    # symmetric storage keeps the ndiag // 2 + 1 offsets >= 0), the per-row mask (1/2/4 B for
+    # <= 8/16/32 offsets; not read by the 7-point kernel when it selects on NaN-encoded
+    # absent entries), x read once, y written; offsets table
     mb = 1 if ndiag <= 8 else (2 if ndiag <= 16 else 4)
     stored = ndiag // 2 + 1 if sym else ndiag
-    return val_bytes * stored * n + mb * n + 2 * val_bytes * n + 4 * ndiag
+    mask = 0 if (sym == 2 and nan_select) else mb * n
+    return val_bytes * stored * n + mask + 2 * val_bytes * n + 4 * ndiag
+
+
+def vi_bytes(D, val_bytes, add=False):
+    # value-indexed transfer: per row the base column (4 B), slot mask (1/2/4 B) and 8-bit codes
+    # (8 B per 8 slots), the value table, x read once (ncols), y written (and read by y += A x)
+    m = D.m
+    nw = (m.ndiag + 7) // 8
+    mb = 1 if m.ndiag <= 8 else (2 if m.ndiag <= 16 else 4)
+    rows, cols = m.nrows, m.ncols
+    return (4 + mb + 8 * nw) * rows + val_bytes * m.vi_nvals + val_bytes * cols + val_bytes * rows * (2 if add else 1)
 
 
 def time_kernel(ua, ctx, launch, reps):
@@ -78,13 +90,14 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
         csr_b = spmv_bytes(n_rows, A.nnz, vb)
         ty = "double" if dt is np.float64 else "float"
         if D.format == ua.FMT_DIA:
-            nbytes = dia_bytes(n_rows, D.m.ndiag, vb, D.m.dia_sym)
+            nbytes = dia_bytes(n_rows, D.m.ndiag, vb, D.m.dia_sym, D.m.dia_nan == 1)
             stored = f"{D.m.ndiag // 2 + 1} of {D.m.ndiag} offsets stored (symmetric)" if D.m.dia_sym else \
                 f"{D.m.ndiag} offsets"
             pb = D.m.dia_sym == 2
-            kname = "k_spmv_dia7z" if pb else "k_spmv_dia"
-            blocked = ", plane-blocked" if pb else ""
-            label = f"{kname}<{ty}, EpiStore<{ty}>> (DIA, {stored}{blocked}, per-row mask)"
+            kname = "k_spmv_dia7r" if pb else "k_spmv_dia"
+            walk = ", plane-walking" if pb else ""
+            sel = "NaN-encoded absent entries" if (pb and D.m.dia_nan) else "per-row mask"
+            label = f"{kname}<{ty}, EpiStore<{ty}>> (DIA, {stored}{walk}, {sel})"
         elif D.format == ua.FMT_SELL64:
             nbytes, label = csr_b, f"k_spmv_sell<{ty}, EpiStore<{ty}>> (SELL-64)"
         else:
@@ -108,10 +121,115 @@ def kernel_rooflines(ua, ctx, A, n_rows, reps=20):
     return out
 
 
-def cpu_baseline(n_full, gpu_iters, sample_n=128, seed=3):
-    """Oracle (plain C, OpenMP) MG-PCG timed on a bounded sample: the same
-    operator family at sample_n^3; per-iteration time scaled by the row ratio
-    and multiplied by the GPU iteration count at the full size."""
+def generic_csr_kernels(ua, ctx, n, seed, reps=20):
+    """y = A x with the general sparse formats on a matrix with no diagonal structure: the n^3
+    variable-coefficient operator renumbered by a seeded permutation of its 64-row blocks
+    (generator kind 5; at 256^3 the FP64 CSR arrays are 1.47 GB). Row-block CSR and SELL-64 in
+    FP64, and the automatic plan in FP32; algorithmic bytes are those of the CSR arrays."""
+    A = ua.gen_stencil(ua.STENCIL_3D7_BLOCKPERM, n, n, n, seed=seed)
+    N = A.shape[0]
+    out = {}
+    names = {ua.FMT_ROWBLOCK: "k_spmv_rowblock", ua.FMT_SELL64: "k_spmv_sell", ua.FMT_DIA: "k_spmv_dia",
+             ua.FMT_VI: "k_spmv_vi"}
+    for dt, fmt, key in ((np.float64, ua.FMT_ROWBLOCK, "perm_rowblock_f64"), (np.float64, ua.FMT_SELL64, "perm_sell64_f64"),
+                         (np.float32, ua.FMT_AUTO, "perm_auto_f32")):
+        vb = 8 if dt is np.float64 else 4
+        D = ua.DeviceCsr(ctx, A, dt, fmt=fmt)
+        x = ua.DeviceArray.from_numpy(ctx, ua.gen_uniform(5, N).astype(dt))
+        y = ua.DeviceArray(ctx, N, dt)
+        ms = time_kernel(ua, ctx, lambda: D.spmv(x, y), reps)
+        nb = spmv_bytes(N, A.nnz, vb)
+        ty = "double" if dt is np.float64 else "float"
+        out[key] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
+                    "label": f"{names.get(D.format, D.format)}<{ty}, EpiStore<{ty}>> on the block-permuted "
+                             f"{n}^3 operator ({A.nnz} nonzeros, format {D.format})"}
+        del D
+        x.free()
+        y.free()
+    return out
+
+
+class ChebStep(ctypes.Structure):
+    _fields_ = [("pprev", ctypes.c_void_p), ("p_dtype", ctypes.c_int32), ("rprev", ctypes.c_void_p),
+                ("xin", ctypes.c_void_p), ("xout", ctypes.c_void_p), ("x_dtype", ctypes.c_int32),
+                ("rout", ctypes.c_void_p), ("pout", ctypes.c_void_p), ("a_prev", ctypes.c_double),
+                ("beta", ctypes.c_double), ("a_cur", ctypes.c_double), ("last", ctypes.c_int32),
+                ("dot_with", ctypes.c_void_p), ("dot_d", ctypes.c_void_p)]
+
+
+def solve_kernels(ua, ctx, A, interps, n_rows, reps=20):
+    """Average launch time and algorithmic bytes of the fused fine-level launches one CG
+    iteration of the 256^3 solve makes (V(2,2) Chebyshev from a zero guess: pre-smoothing is one
+    last-step launch, then r_R = r - A c, the restriction, the interpolation-add, p0 = r - A c and
+    the last smoothing step with z.r fused), in FP64 (the FP64 form) and FP32 (the mixed form),
+    plus the CG's A p with p.Ap (FP64 in both forms)."""
+    L = ua.lib()
+    out = {}
+    N = n_rows
+    F64, F32 = ua.F64, ua.F32
+    vec = {}
+    for dt in (np.float64, np.float32):
+        vec[dt] = [ua.DeviceArray.from_numpy(ctx, ua.gen_uniform(20 + i, N).astype(dt)) for i in range(4)]
+    r64 = ua.DeviceArray.from_numpy(ctx, ua.gen_uniform(30, N))
+    dd = ua.DeviceArray(ctx, 2)
+    P = ua.DeviceCsr(ctx, interps[-1], np.float32)   # 128^3 -> 256^3 interpolation (auto plan)
+    R = ua.DeviceCsr(ctx, ua.Csr.from_scipy(interps[-1].to_scipy().T.tocsr()), np.float32)
+    nc = interps[-1].shape[1]
+    xc = ua.DeviceArray.from_numpy(ctx, ua.gen_uniform(31, nc).astype(np.float32))
+    mc = ua.DeviceArray.from_numpy(ctx, np.ones(nc, np.float32))
+    for dt, vb, tag, code in ((np.float64, 8, "f64", F64), (np.float32, 4, "f32", F32)):
+        D = ua.DeviceCsr(ctx, A, dt)
+        base = dia_bytes(N, D.m.ndiag, vb, D.m.dia_sym, D.m.dia_nan == 1) if D.format == ua.FMT_DIA else \
+            spmv_bytes(N, A.nnz, vb)
+        v0, v1, v2, v3 = (ctypes.c_void_p(a.ptr) for a in vec[dt])
+        ty = "double" if dt is np.float64 else "float"
+
+        def cheb(pprev, xin, xout, dot):
+            st = ChebStep(pprev=pprev, p_dtype=code, rprev=None, xin=xin, xout=xout, x_dtype=code, rout=None,
+                          pout=None, a_prev=0.3, beta=0.1, a_cur=0.2, last=1,
+                          dot_with=ctypes.c_void_p(r64.ptr) if dot else None,
+                          dot_d=ctypes.c_void_p(dd.ptr) if dot else None)
+            return lambda: L.uhip_cheb_step_apply(ctx.h, D.ref, ctypes.byref(st))
+
+        forms = [
+            ("pre_smooth", f"EpiCheb<{ty}> last step, zero guess", cheb(v0, None, v1, False), base),
+            ("residual", f"EpiResidual<{ty}> r_R = r - A c",
+             lambda: L.uhip_spmv_residual(ctx.h, D.ref, v1, v0, code, v2), base + vb * N),
+            ("post_smooth_dot", f"EpiCheb<{ty}, dot> last step with z.r (r FP64)", cheb(v2, v1, v3, True),
+             base + vb * N + 8 * N),
+        ]
+        if dt is np.float64:
+            forms.append(("cg_Ap_dot", "EpiStoreDot<double> A p with p.Ap",
+                          lambda: L.uhip_spmv_dot(ctx.h, D.ref, v0, v1, ctypes.c_void_p(dd.ptr)), base))
+        for key, label, launch, nbytes in forms:
+            def run(launch=launch):
+                ua.check(launch())
+            ms = time_kernel(ua, ctx, run, reps)
+            out[f"{key}_{tag}"] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                                   "label": f"{label} ({'k_spmv_dia7r' if D.m.dia_sym == 2 else 'format ' + str(D.format)})"}
+        del D
+    xs = vec[np.float32]
+    ms = time_kernel(ua, ctx, lambda: ua.check(L.uhip_spmv_masked(ctx.h, R.ref, ctypes.c_void_p(xs[0].ptr),
+                                                                  ctypes.c_void_p(mc.ptr), ctypes.c_void_p(xc.ptr))),
+                     reps)
+    nb = vi_bytes(R, 4) + 4 * nc
+    out["restrict_f32"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
+                           "label": f"EpiMaskStore<float> R r_R, format {R.format} (VI = 3), {R.m.ndiag} slots"}
+    ms = time_kernel(ua, ctx, lambda: ua.check(L.uhip_spmv_add(ctx.h, P.ref, ctypes.c_void_p(xc.ptr),
+                                                               ctypes.c_void_p(xs[1].ptr))), reps)
+    nb = vi_bytes(P, 4, add=True)
+    out["interpolate_f32"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
+                              "label": f"EpiAddTo<float> c += P c_c, format {P.format} (VI = 3), {P.m.ndiag} slots"}
+    return out
+
+
+def cpu_baseline(n, levels, seed, gpu_iters, x_fp64, x_mixed, reps=1, iters_1core=4):
+    """The oracle (plain C, OpenMP, FP64) MG-PCG on the benchmark's own problem: the same 256^3
+    operator, hierarchy, right-hand side and tolerance. Setup (Galerkin products, power method,
+    dense Cholesky) is timed apart from the solve; the solve runs to convergence `reps` times (the
+    median is the value); its iteration count and solution are compared with the GPU's. The one-core
+    figure times `iters_1core` iterations of the same solve on one thread and scales them to the
+    iteration count."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as orc
     import utopia_amd as ua
@@ -119,38 +237,42 @@ def cpu_baseline(n_full, gpu_iters, sample_n=128, seed=3):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     threads = min(threads, 16)
     orc.set_threads(threads)
-    n = sample_n
     A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=seed)
     b = ua.gen_uniform(seed + 100, A.shape[0])
-    interps = ua.mg_hierarchy(n, n, n, 5)
+    atol = 1e-10 * float(np.linalg.norm(b))
+    interps = ua.mg_hierarchy(n, n, n, levels)
     ua.srand(1)
     t0 = time.perf_counter()
     mg = orc.MG(interps, pre=2, post=2)
     mg.update(A)
     t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, st = mg.cg_solve(b, atol=1e-10 * np.linalg.norm(b), max_it=500)
-    t_solve = time.perf_counter() - t0
+    log(f"cpu baseline: oracle setup {t_setup:.1f} s ({threads} threads)")
+    times, x, st = [], None, None
+    for r in range(reps):
+        t0 = time.perf_counter()
+        x, st = mg.cg_solve(b, atol=atol, max_it=1000)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu baseline: oracle solve {r}: {times[-1]:.1f} s, {st['iterates'] + 1} iterations")
     its = st["iterates"] + 1
-    per_it = t_solve / its
-    scale = (n_full ** 3) / (n ** 3)
-    est = per_it * scale * gpu_iters
-    # the same solve on one core: a fixed number of iterations of the same hierarchy
-    k1 = 8
+    t_solve = float(np.median(times))
     orc.set_threads(1)
     t0 = time.perf_counter()
-    _, st1 = mg.cg_solve(b, atol=0.0, max_it=k1)
+    _, st1 = mg.cg_solve(b, atol=0.0, max_it=iters_1core)
     its1 = st1["iterates"] + 1
     per_it1 = (time.perf_counter() - t0) / its1
     orc.set_threads(threads)
+    nx = float(np.linalg.norm(x))
     return {
-        "value": est, "unit": "s", "cores": threads, "kind": "port",
-        "value_1core": per_it1 * scale * gpu_iters,
-        "sample": (f"oracle FP64 MG-PCG (V(2,2) Chebyshev, 5 levels) on the {n}^3 variable-coefficient "
-                   f"operator: {its} iterations in {t_solve:.2f} s ({per_it * 1e3:.1f} ms/iteration, setup "
-                   f"{t_setup:.1f} s); scaled by the row ratio {scale:.0f} and the {gpu_iters} iterations of the "
-                   f"{n_full}^3 solve; value_1core: {its1} iterations of the same solve on one core "
-                   f"({per_it1 * 1e3:.0f} ms/iteration), scaled the same way"),
+        "value": t_solve, "unit": "s", "cores": threads, "kind": "port",
+        "value_1core": per_it1 * its,
+        "setup_s": t_setup, "solve_s": times, "iterations": its,
+        "gpu_iterations": gpu_iters,
+        "rel_error_fp64_vs_oracle": float(np.linalg.norm(x_fp64 - x) / nx),
+        "rel_error_mixed_vs_oracle": float(np.linalg.norm(x_mixed - x) / nx),
+        "sample": (f"oracle FP64 MG-PCG (V(2,2) Chebyshev, {levels} levels, dense Cholesky coarse solve) on the "
+                   f"same {n}^3 variable-coefficient problem to rtol 1e-10: {its} iterations, solve "
+                   f"{t_solve:.1f} s (median of {reps}), setup {t_setup:.1f} s apart; value_1core: {its1} "
+                   f"iterations of the same solve on one thread ({per_it1:.2f} s each) times {its}"),
     }
 
 
@@ -253,7 +375,6 @@ def bench_mprgp(a, n=1024):
     for i in range(a.warmup + a.steps):
         ua.check(L.uhip_vec_set_f64(ctx.h, n * n, 0.0, ctypes.c_void_p(xd.ptr)))
         ctx.sync()
-        ua.srand(1)  # power-method start vector (rand(), as in the reference)
         t = time.perf_counter()
         q.apply_device(bd, xd)
         ctx.sync()
@@ -371,7 +492,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune", type=int, default=5,
                     help="reps > 0: every SpMV plan of the hierarchy is chosen by timing (uhip_csr_autotune)")
-    ap.add_argument("--cpu-sample-n", type=int, default=128)
+    ap.add_argument("--cpu-reps", type=int, default=1,
+                    help="oracle solves at the full size timed for the CPU baseline (median reported)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV / streaming kernel timings (for counter collection runs)")
     ap.add_argument("--workload", choices=["mgpcg256", "batched", "mprgp1024", "shard"], default="mgpcg256",
@@ -402,6 +524,7 @@ def main(argv=None):
     log(f"generated {n}^3 variable-coefficient operator: N={N} nnz={nnz} in {time.perf_counter() - t0:.1f} s")
     if a.kernels_only:
         kern = kernel_rooflines(ua, ctx, A, N)
+        kern.update(generic_csr_kernels(ua, ctx, n, a.seed))
         for k, v in kern.items():
             log(f"kernel {k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s, {v['bytes']} algorithmic bytes, {v.get('label', '')}")
         print(json.dumps({"kernels": kern}))
@@ -450,21 +573,26 @@ def main(argv=None):
     log(f"mixed vs fp64: rel. error {err_mixed:.2e}")
 
     kern = kernel_rooflines(ua, ctx, A, N)
+    kern.update(solve_kernels(ua, ctx, A, interps, N))
+    kern.update(generic_csr_kernels(ua, ctx, n, a.seed))
     for k, v in kern.items():
-        log(f"kernel {k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s ({100 * v['gbs'] / HBM_PEAK_GBS:.1f}% of 8 TB/s)")
+        log(f"kernel {k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s ({100 * v['gbs'] / HBM_PEAK_GBS:.1f}% of 8 TB/s) "
+            f"{v['label']}")
 
     cpu = None
     if not a.no_cpu_baseline:
         t0 = time.perf_counter()
-        cpu = cpu_baseline(n, results["mixed"]["iterations"], a.cpu_sample_n, a.seed)
-        log(f"cpu baseline ({time.perf_counter() - t0:.1f} s): {cpu['value']:.1f} s estimated, {cpu['sample']}")
+        cpu = cpu_baseline(n, a.levels, a.seed, results["mixed"]["iterations"], x64, xmx, reps=a.cpu_reps)
+        log(f"cpu baseline ({time.perf_counter() - t0:.1f} s): {cpu['value']:.1f} s, {cpu['sample']}")
 
-    # The roofline line is for the kernel that dominates the mixed solve (the value):
-    # the FP32 fine-level operator application (4 launches per CG iteration, about 30 %
-    # of the solve); the FP64 one is in "kernels". HBM bytes per launch come from the
-    # committed counter runs (FETCH_SIZE / WRITE_SIZE), used only when they were taken
-    # on the kernel measured here.
-    rk = "spmv_csr_f32"
+    # The roofline line is for the launch with the largest share of a mixed CG iteration: the
+    # CG's A p with p.Ap (FP64, one launch per iteration, shared with the FP64 form); the FP32
+    # V-cycle launches and the plain products are in "kernels". HBM bytes per launch come from
+    # the committed counter runs (FETCH_SIZE / WRITE_SIZE), used only when they were taken on
+    # the kernel measured here.
+    per_iteration = {"cg_Ap_dot_f64": 1, "residual_f32": 2, "pre_smooth_f32": 1, "post_smooth_dot_f32": 1,
+                     "restrict_f32": 1, "interpolate_f32": 1}
+    rk = max(per_iteration, key=lambda k: kern[k]["ms"] * per_iteration[k])
     traffic = {}
     tf = ROOT / "profiles" / "spmv_varcoef256_traffic.json"
     if tf.exists():
@@ -501,6 +629,7 @@ def main(argv=None):
                   "rel_residual": mixed["rel_residual"], "rel_error_vs_fp64": err_mixed},
         "speedup_mixed_vs_fp64": fp64["s_per_solve"] / mixed["s_per_solve"],
         "roofline": {"bound": "hbm", "kernel": f"{rk}: {sp['label']}",
+                     "launches_per_cg_iteration": per_iteration[rk],
                      "achieved": sp["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS,
                      "traffic": traffic.get(rk), "bytes_per_launch": sp["bytes"], "avg_launch_ms": sp["ms"]},
         "kernels": {k: {"gbs": v["gbs"], "ms": v["ms"], "frac": v["gbs"] / HBM_PEAK_GBS, "label": v["label"],

@@ -61,6 +61,8 @@ int uhip_sync(uhip_ctx ctx);
 int uhip_malloc(uhip_ctx ctx, void **ptr, size_t bytes);
 int uhip_free(uhip_ctx ctx, void *ptr);
 int uhip_memcpy_htod(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
+/* 1 if p is device (or managed) memory, e.g. from hipMalloc or torch, 0 for host memory */
+int uhip_pointer_is_device(const void *p);
 int uhip_memcpy_dtoh(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
 int uhip_memcpy_dtod(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
 int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h);
@@ -201,13 +203,15 @@ typedef struct uhip_csr {
     int32_t dia_sym;    /* DIA of a bitwise-symmetric matrix: dia_val holds only the offsets >= 0
                          * (ndiag/2 + 1 arrays); offset -o of row r is read as offset +o of row r - o.
                          * 2: the same storage of a 7-point operator whose largest offset is a multiple
-                         * of 64, run by the plane-blocked kernel (FP64: the products with dots; its
-                         * other products run the symmetric kernel on the same arrays) */
+                         * of 64, run by the plane-walking 7-point kernel in both precisions */
     int32_t spmm_kernel; /* multi-RHS products (uhip_spmm*): 0 by row length (wavefront per row when
                           * nnz >= 32 rows), 1 one thread per row, 2 one wavefront per row */
     int32_t *vi_base;   /* VI: per-row base column (the row's first column; 0 for an empty row) */
     void *vi_table;     /* VI: the matrix's distinct values (dtype), at most 256 */
     int32_t vi_nvals;   /* VI: number of distinct values */
+    struct uhip_csr *tr; /* A^T with its own plan (uhip_csr_plan_transpose), freed with A's plan */
+    int32_t dia_nan;    /* DIA: 1 when the matrix holds no NaN; the entries a row does not hold are
+                         * then NaN in dia_val (else zero) and kernels may select on the value */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)
@@ -234,6 +238,8 @@ int uhip_csr_plan(uhip_ctx ctx, uhip_csr *A, int32_t block_nnz);
  * of CSR-VI, Kourtis et al. 2008), the distinct values in a table. dia_val then
  * holds the codes row by row: 1, 2 or 4 64-bit words per row (ndiag <= 8, 16,
  * 32), the code of slot d in byte d % 8 of word d / 8.
+ * DIA value arrays hold NaN for the entries a row does not hold when the matrix holds no NaN
+ * (dia_nan = 1), zero otherwise.
  * UHIP_FMT_AUTO takes DIA when it applies with at most 25% padding, else VI when
  * it applies with at most 3 slots per stored entry, else SELL-64 when its padding
  * adds at most 25% to nnz, else the row-block CSR kernel. Each y_i is the sum of exactly the row's CSR entries in column
@@ -251,6 +257,13 @@ int uhip_ctx_set_autotune(uhip_ctx ctx, int reps);
 
 /* y = A x                         MatMult (utopia_petsc_Matrix.cpp:1326-1358) */
 int uhip_spmv(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y);
+/* Build (once) A's transpose and its automatic plan in A->tr, for uhip_spmv_transpose. */
+int uhip_csr_plan_transpose(uhip_ctx ctx, uhip_csr *A);
+/* y = A^T x                       MatMultTranspose (PetscMatrix::transpose_multiply, utopia_petsc_Matrix.cpp:1389),
+ * used by IPTransfer::restrict (solvers/multilevel/transfer/utopia_IPTransfer.hpp:65-74). Row c of A^T
+ * lists A's entries of column c by increasing row, so y_c adds a_ic x_i in the order of MatMultTranspose's
+ * scatter over the rows of A: the same sums bit for bit. Plans the transpose on first use. */
+int uhip_spmv_transpose(uhip_ctx ctx, uhip_csr *A, const void *x, void *y);
 /* y = alpha A x + beta y          MatMult + VecAXPBY (Eval_Binary, utopia_core/eval/utopia_Eval_Binary.hpp:88-94) */
 int uhip_spmv_axpby(uhip_ctx ctx, const uhip_csr *A, double alpha, const void *x, double beta, void *y);
 /* y = alpha A x + beta z          g = A x - rhs of MPRGP
@@ -338,7 +351,7 @@ int uhip_csr_boolean_restrict_or(uhip_ctx ctx, const uhip_csr *R, double tol, co
 /* ------------------------------------------------------------------------
  * Sparse matrix products (setup). Pattern arithmetic is integer-exact.
  * ---------------------------------------------------------------------- */
-/* B = A^T (structure and values; cols sorted)   MatTranspose (utopia_petsc_Matrix.cpp:462),
+/* B = A^T (structure and values, FP64 or FP32; cols sorted)   MatTranspose (utopia_petsc_Matrix.cpp:462),
  * used by IPRTransfer's R = transpose(I) (utopia_IPRTransfer.hpp:28-39).
  * Allocates B->rowptr/col/val (free with uhip_csr_free). */
 int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B);
@@ -480,6 +493,14 @@ int uhip_solver_set_transfer_operators(uhip_solver s, int n, const int32_t *nrow
 int uhip_solver_update_crs(uhip_solver s, int32_t n, const int32_t *rowptr, const int32_t *col, const double *val);
 /* same on caller-owned device arrays (PetscMatrix::wrap semantics: not freed) */
 int uhip_solver_update_crs_device(uhip_solver s, int32_t n, int64_t nnz, int32_t *rowptr, int32_t *col, double *val);
+/* isolver_lsolve_update_with_preconditioner_crs (plug_in_and_out/isolver/utopia_isolver_lsolve.cpp:96-133)
+ * -> PreconditionedSolver::update(op, prec) (solvers/linear/utopia_PreconditionedSolver.hpp:48-61): the CG
+ * iterates on A, its preconditioner (Jacobi, Multigrid, AMG) is built from the second matrix. Host arrays
+ * (copied) or, with on_device, caller-owned device arrays (wrapped, not freed). UHIP_ERR_UNSUPPORTED for
+ * solver types without a preconditioner. */
+int uhip_solver_update_with_preconditioner_crs(uhip_solver s, int32_t n, const int32_t *rowptr, const int32_t *col,
+                                               const double *val, const int32_t *prec_rowptr,
+                                               const int32_t *prec_col, const double *prec_val, int on_device);
 /* QPSolver::set_box_constraints (solvers/nonlinear/constrained/quadratic_programming/utopia_QPSolver.hpp:34-44);
  * NULL bound = none (filled with -/+DBL_MAX, BoxConstraints::fill_empty_bounds). Call after update. */
 int uhip_solver_set_box_constraints(uhip_solver s, const double *lb, const double *ub);
@@ -602,7 +623,11 @@ int64_t uhip_gen_nnz(int kind, int32_t nx, int32_t ny, int32_t nz);
  * kind 2: 3D 7-point variable coefficient: cell kappa = exp(N(0,1)) from seed,
  *         face coefficient 2 ka kb/(ka+kb), boundary faces use the cell value
  * kind 3: 2D 9-point Laplacian (diag 8, all eight neighbours -1) + shift*I
- * kind 4: 2D 5-point with per-edge coefficients exp(N(0, 0.5)) from seed */
+ * kind 4: 2D 5-point with per-edge coefficients exp(N(0, 0.5)) from seed
+ * kind 5: kind 2 renumbered, P A P^T: the full 64-row blocks are permuted by a seeded
+ *         Fisher-Yates shuffle (a tail block of fewer than 64 rows keeps its place), rows keep
+ *         sorted columns. A symmetric operator with no diagonal structure, for the general
+ *         sparse formats. */
 int uhip_gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift,
                      int32_t *rowptr, int32_t *col, double *val);
 /* Trilinear (bilinear / linear for nz==1 / ny==1) prolongation from the coarse

@@ -213,6 +213,8 @@ int or_cholesky(int32_t n, double *A) {
         if (!(d > 0.0)) return j + 1;
         d = sqrt(d);
         A[(int64_t)j * n + j] = d;
+        /* rows below j are independent: each keeps its own sequential sum */
+#pragma omp parallel for schedule(static) if (n - j > 256)
         for (int32_t i = j + 1; i < n; ++i) {
             double s = A[(int64_t)i * n + j];
             for (int32_t k = 0; k < j; ++k) s -= A[(int64_t)i * n + k] * A[(int64_t)j * n + k];

@@ -16,8 +16,13 @@ def test_library_exports_every_declared_symbol():
     declared = set(_abi.declared_symbols())
     out = subprocess.run(["nm", "-D", "--defined-only", str(_abi.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
-    exported = {l.split()[-1] for l in out.splitlines() if " T uhip_" in l}
+    exported = {l.split()[-1] for l in out.splitlines() if " T uhip_" in l or " T isolver_lsolve_" in l}
     assert declared, "no prototypes parsed from include/utopia_hip.h"
+    # the isolver plug-out's entry points (utopia_isolver_lsolve.cpp:42-220) under their reference names
+    assert {n for n in declared if n.startswith("isolver_lsolve_")} == {
+        "isolver_lsolve_" + f for f in ("init", "update_crs", "update_with_preconditioner_crs", "apply",
+                                        "set_max_iterations", "set_atol", "set_stol", "set_rtol",
+                                        "set_verbosity", "destroy")}
     assert declared <= exported, sorted(declared - exported)
     for name in declared:
         assert getattr(lib, name) is not None
@@ -83,6 +88,34 @@ def test_varcoef_is_spd_with_harmonic_faces():
     off = -A[corner].sum() + A[corner, corner]  # sum of the three interior faces
     assert np.isclose(A[corner, corner], off + 3 * kap[corner], rtol=1e-15)
     assert np.linalg.eigvalsh(A).min() > 0
+
+
+def test_block_permuted_operator_is_the_renumbered_varcoef_operator():
+    """Kind 5 = P A P^T with A the kind-2 operator and P a permutation that moves whole 64-row
+    blocks (the tail block of 56 rows stays): recover P from the rows themselves, then compare."""
+    nx, ny, nz, seed = 13, 11, 8, 9  # 1144 rows: 17 full blocks and a tail of 56
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, nx, ny, nz, seed=seed).to_scipy()
+    B = ua.gen_stencil(ua.STENCIL_3D7_BLOCKPERM, nx, ny, nz, seed=seed)
+    n = A.shape[0]
+    assert np.all(np.diff(B.rowptr) > 0)
+    for r in range(n):  # sorted columns within every row
+        c = B.col[B.rowptr[r]:B.rowptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
+    Bs = B.to_scipy()
+    assert abs(Bs - Bs.T).max() == 0
+    # the diagonal identifies each row: kappa draws are distinct with probability 1
+    da, db = A.diagonal(), Bs.diagonal()
+    order_a, order_b = np.argsort(da), np.argsort(db)
+    perm = np.empty(n, np.int64)
+    perm[order_a] = order_b  # perm[old row] = new row
+    assert np.array_equal(np.sort(da), np.sort(db)) and len(np.unique(da)) == n
+    full = (n // 64) * 64
+    assert np.array_equal(perm[full:], np.arange(full, n))
+    blocks = perm[:full].reshape(-1, 64)
+    assert np.all(blocks % 64 == np.arange(64)) and np.all(np.diff(blocks, axis=1) == 1)
+    assert not np.array_equal(blocks[:, 0], np.arange(0, full, 64))  # not the identity
+    P = sp.csr_matrix((np.ones(n), (perm, np.arange(n))), shape=(n, n))
+    assert abs(P @ A @ P.T - Bs).max() == 0
 
 
 def test_edge_coefficient_operator_is_symmetric_and_diagonally_dominant():

@@ -153,3 +153,25 @@ def test_mg_full_cycle_reference_fixture(ctx, golden):
     s3.set_transfer_operators(Is)
     s3.update(A)
     assert rel(xf, s3.apply(rhs)) <= 1e-14
+
+
+def test_block_mg_fixture_matches_oracle(ctx, golden):
+    """PetscLinearSolverTest::test_block_mg (tests/petsc/utopia_PetscLinearSolverTest.cpp:343-383):
+    the reference's data/mg system (410 dofs, block size 2), one coarse level, symmetric
+    Gauss-Seidel smoothing, pre/post 3, fix_semidefinite_operators, 20 cycles. GPU against the
+    oracle with the GPU's colouring, and the reference's assertion approxeq(rhs, A x, 1e-6)."""
+    d = golden / "utopia_data"
+    A = petsc_io.read_mat(d / "mg_A.bin")
+    I = petsc_io.read_mat(d / "mg_I.bin")
+    rhs = petsc_io.read_vec(d / "mg_rhs.bin")
+    s = ua.Multigrid(ctx, smoother=2, pre_smoothing_steps=3, post_smoothing_steps=3, fix_semidefinite_operators=1,
+                     max_it=20, atol=1e-15, rtol=1e-15, stol=1e-15, block_size=2)
+    s.set_transfer_operators([I])
+    s.update(A)
+    x = s.apply(rhs)
+    assert np.abs(rhs - A.to_scipy() @ x).max() <= 1e-6
+    mg = orc.MG([I], smoother=2, pre=3, post=3, masks=True, fix_semidefinite=True)
+    mg.update(A)
+    mg.set_colors(1, *s.colors(1))  # level 0, the coarsest, is the dense solve
+    xo = mg.apply(rhs, max_it=20, atol=1e-15, rtol=1e-15)
+    assert rel(x, xo) <= 1e-10

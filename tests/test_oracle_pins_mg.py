@@ -279,3 +279,24 @@ def test_jacobi_and_point_jacobi_zero_diagonal_rows():
         bad = affine_iterate(-dz[:, None] * (Ad - np.diag(d)), dz * b, x0, nu)
         assert _rel(bad, refp) > 1e-6
     assert orc.jacobi_smooth(A, b, x0, 1)[2] == x0[2]
+
+
+def test_block_mg_fixture_reference_assertion(golden):
+    """PetscLinearSolverTest::test_block_mg (tests/petsc/utopia_PetscLinearSolverTest.cpp:343-383)
+    on the reference's data/mg fixture: Multigrid with symmetric Gauss-Seidel smoothing (PETSc's
+    SOR with omega 1; multicolour order here), pre/post 3, fix_semidefinite_operators, 20
+    cycles; the reference asserts approxeq(rhs, A x, 1e-6)."""
+    d = golden / "utopia_data"
+    A = petsc_io.read_mat(d / "mg_A.bin")
+    I = petsc_io.read_mat(d / "mg_I.bin")
+    rhs = petsc_io.read_vec(d / "mg_rhs.bin")
+    assert A.shape == (410, 410) and I.shape == (410, 126)
+    mg = orc.MG([I], smoother=2, pre=3, post=3, masks=True, fix_semidefinite=True)
+    orc.srand(1)
+    assert mg.update(A) == 0
+    for l in range(2):
+        rp, col, val, shape = mg.level(l)
+        color, nc = orc.jp_color(ua.Csr(rp, col, val, shape))
+        mg.set_colors(l, color, nc)
+    x = mg.apply(rhs, max_it=20, atol=1e-15, rtol=1e-15)
+    assert np.abs(rhs - A.to_scipy() @ x).max() <= 1e-6

@@ -85,7 +85,7 @@ class Csr:
         return Csr(m.indptr.astype(np.int32), m.indices.astype(np.int32), m.data.astype(np.float64), m.shape)
 
 
-STENCIL_2D5, STENCIL_3D7, STENCIL_3D7_VARCOEF, STENCIL_2D9, STENCIL_2D5_EDGECOEF = range(5)
+STENCIL_2D5, STENCIL_3D7, STENCIL_3D7_VARCOEF, STENCIL_2D9, STENCIL_2D5_EDGECOEF, STENCIL_3D7_BLOCKPERM = range(6)
 
 
 def gen_stencil(kind: int, nx: int, ny: int, nz: int = 1, seed: int = 0, shift: float = 0.0) -> Csr:

@@ -14,6 +14,9 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 HEADER = ROOT / "include" / "utopia_hip.h"
+ISOLVER_HEADER = ROOT / "include" / "utopia_hip_isolver.h"
+# (header, prefix of the functions it declares)
+HEADERS = ((HEADER, "uhip_"), (ISOLVER_HEADER, "isolver_lsolve_"))
 LIB_PATH = PKG / "lib" / "libutopia_hip.so"
 
 
@@ -46,6 +49,8 @@ class uhip_csr(ctypes.Structure):
         ("vi_base", ctypes.c_void_p),
         ("vi_table", ctypes.c_void_p),
         ("vi_nvals", ctypes.c_int32),
+        ("tr", ctypes.c_void_p),
+        ("dia_nan", ctypes.c_int32),
     ]
 
 
@@ -78,6 +83,9 @@ _SCALARS = {
     "size_t": ctypes.c_size_t,
     "double": ctypes.c_double,
     "float": ctypes.c_float,
+    "ptrdiff_t": ctypes.c_ssize_t,
+    "isolver_idx_t": ctypes.c_int,
+    "isolver_scalar_t": ctypes.c_double,
     "void": None,
 }
 
@@ -107,12 +115,12 @@ def _expand_macros(text: str) -> str:
     return text
 
 
-def parse_header(path: Path = HEADER):
-    """Return {name: (restype_str, [argtype_str, ...])} for every uhip_* prototype."""
+def parse_header(path: Path = HEADER, prefix: str = "uhip_"):
+    """Return {name: (restype_str, [argtype_str, ...])} for every prototype named prefix*."""
     text = _expand_macros(_strip_comments(path.read_text()))
     text = re.sub(r"#[^\n]*", " ", text)
     protos = {}
-    for m in re.finditer(r"([A-Za-z_][\w\s\*]*?)\b(uhip_\w+)\s*\(([^;{]*?)\)\s*;", text):
+    for m in re.finditer(r"([A-Za-z_][\w\s\*]*?)\b(%s\w+)\s*\(([^;{]*?)\)\s*;" % prefix, text):
         ret, name, args = m.group(1).strip(), m.group(2), m.group(3).strip()
         ret = ret.split()[-1] if "*" not in ret else ret
         argl = [] if args in ("", "void") else [a.strip() for a in args.split(",")]
@@ -159,7 +167,10 @@ def lib():
     except ImportError:
         pass
     L = ctypes.CDLL(str(LIB_PATH))
-    for name, (ret, args) in parse_header().items():
+    protos = {}
+    for path, prefix in HEADERS:
+        protos.update(parse_header(path, prefix))
+    for name, (ret, args) in protos.items():
         fn = getattr(L, name, None)
         if fn is None:
             continue
@@ -170,4 +181,8 @@ def lib():
 
 
 def declared_symbols():
-    return sorted(parse_header().keys())
+    """Every function declared by the ABI headers (include/utopia_hip.h, include/utopia_hip_isolver.h)."""
+    names = set()
+    for path, prefix in HEADERS:
+        names.update(parse_header(path, prefix).keys())
+    return sorted(names)

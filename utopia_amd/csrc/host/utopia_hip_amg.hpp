@@ -16,6 +16,8 @@
 #include "utopia_hip_multigrid.hpp"
 
 namespace utopia {
+// the solvers of the HIP backend (the tensor types, HipVector and HipCrsMatrix, are in utopia)
+namespace hip {
 
 class Agglomerate {
 public:
@@ -265,4 +267,5 @@ private:
     double setup_s_ = 0.0;
 };
 
+}  // namespace hip
 }  // namespace utopia

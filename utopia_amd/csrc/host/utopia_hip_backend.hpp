@@ -59,6 +59,51 @@ private:
 
 using HipContextPtr = std::shared_ptr<HipContext>;
 
+// utopia's operation tags (utopia_core/expressions/utopia_Operators.hpp:53-401) are only declared
+// here: the surface takes them by reference, so a translation unit that includes utopia's core
+// passes utopia's own tags, and one that does not defines them itself.
+class Plus;
+class Min;
+class Max;
+class Abs;
+
+// One GPU driven by one process: the communicator of every HIP tensor
+// (utopia_core/interfaces/utopia_Communicator.hpp:18-300, the SelfCommunicator case).
+class HipCommunicator {
+public:
+    int rank() const { return 0; }
+    int size() const { return 1; }
+    void barrier() const {}
+    template <typename T>
+    T sum(const T &v) const { return v; }
+    template <typename T>
+    T min(const T &v) const { return v; }
+    template <typename T>
+    T max(const T &v) const { return v; }
+    static const HipCommunicator &get() {
+        static const HipCommunicator c;
+        return c;
+    }
+};
+
+// Vector layout (utopia_core/interfaces/utopia_Layout.hpp): local and global sizes agree on one GPU.
+struct HipLayout {
+    int64_t local_size_ = 0, size_ = 0;
+    int64_t local_size() const { return local_size_; }
+    int64_t size() const { return size_; }
+    const HipCommunicator &comm() const { return HipCommunicator::get(); }
+    bool same(const HipLayout &o) const { return local_size_ == o.local_size_ && size_ == o.size_; }
+};
+
+// Range of locally owned indices (utopia_core/utopia_Range.hpp)
+struct HipRange {
+    int64_t begin_ = 0, end_ = 0;
+    int64_t begin() const { return begin_; }
+    int64_t end() const { return end_; }
+    int64_t extent() const { return end_ - begin_; }
+    bool inside(int64_t i) const { return begin_ <= i && i < end_; }
+};
+
 template <typename T>
 struct HipDtype;
 template <>
@@ -145,6 +190,16 @@ public:
     }
 
     int64_t size() const { return n_; }
+    // layout / range / comm of PetscVector (utopia_petsc_Vector.hpp:150-152,289): one GPU owns every entry
+    int64_t local_size() const { return n_; }
+    HipLayout layout() const { return HipLayout{n_, n_}; }
+    HipRange range() const { return HipRange{0, n_}; }
+    const HipCommunicator &comm() const { return HipCommunicator::get(); }
+    void zeros(const HipLayout &l) { zeros(l.local_size()); }
+    void values(const HipLayout &l, T v) { values(l.local_size(), v); }
+    void construct(const HipVectorT &o) { *this = o; }
+    void assign(const HipVectorT &o) { *this = o; }
+    void clear() { release(); }
     bool empty() const { return d_ == nullptr; }
     T *data() { return d_; }
     const T *data() const { return d_; }
@@ -173,6 +228,31 @@ public:
     double min() const;
     double max() const;
     bool has_nan_or_inf() const;
+    // Reducible / Transformable with utopia's tags (utopia_petsc_Vector.hpp:338-397)
+    double reduce(const Plus &) const { return sum(); }
+    double reduce(const Min &) const { return min(); }
+    double reduce(const Max &) const { return max(); }
+    void transform(const Abs &) { abs(); }
+
+    // Host mirror for utopia's local views (Read / Write / ReadAndWrite locks, l_get / l_set / l_add;
+    // utopia_petsc_Vector.hpp:160-250, utopia_core/utopia_DeviceView.hpp:67-111): a lock copies the
+    // vector to the host, the element accessors work on that copy, and a write unlock copies it back.
+    // For element-wise set-up code written against those views; the solvers' hot loops run in kernels.
+    void read_lock() const { mirror_down(); }
+    void read_unlock() const { mirror_.clear(); }
+    template <class Mode>
+    void write_lock(Mode) { mirror_.assign((size_t)n_, T(0)); }
+    template <class Mode>
+    void write_unlock(Mode) { mirror_up(); }
+    template <class Mode>
+    void read_and_write_lock(Mode) { mirror_down(); }
+    template <class Mode>
+    void read_and_write_unlock(Mode) { mirror_up(); }
+    T l_get(int64_t i) const { return mirror_.at((size_t)i); }
+    void l_set(int64_t i, T v) { mirror_.at((size_t)i) = v; }
+    void l_add(int64_t i, T v) { mirror_.at((size_t)i) += v; }
+    T get(int64_t i) const { return l_get(i); }
+    void set(int64_t i, T v) { l_set(i, v); }
 
     void upload(const T *host) { hip_check(uhip_memcpy_htod(ctx(), d_, host, sizeof(T) * n_), "upload"); }
     void download(T *host) const { hip_check(uhip_memcpy_dtoh(ctx(), host, d_, sizeof(T) * n_), "download"); }
@@ -183,6 +263,14 @@ public:
     }
 
 private:
+    void mirror_down() const {
+        mirror_.resize((size_t)n_);
+        if (n_ > 0) download(mirror_.data());
+    }
+    void mirror_up() {
+        if (n_ > 0 && mirror_.size() == (size_t)n_) upload(mirror_.data());
+        mirror_.clear();
+    }
     void allocate(int64_t n) {
         if (!ctx_) throw HipError(UHIP_ERR_INVALID, "HipVector: no context");
         void *p = nullptr;
@@ -211,6 +299,7 @@ private:
     T *d_ = nullptr;
     int64_t n_ = 0;
     bool own_ = true;
+    mutable std::vector<T> mirror_;  // host copy while a lock is held
 };
 
 using HipVector = HipVectorT<double>;
@@ -405,6 +494,8 @@ public:
         m_.nblocks = m_.nslices = m_.nparts = m_.ndiag = m_.dia_sym = m_.spmm_kernel = m_.vi_nvals = 0;
         m_.vi_base = nullptr;
         m_.vi_table = nullptr;
+        m_.tr = nullptr;
+        m_.dia_nan = 0;
         m_.format = UHIP_FMT_ROWBLOCK;
         plan(block_nnz);
     }
@@ -414,6 +505,55 @@ public:
     int64_t rows() const { return m_.nrows; }
     int64_t cols() const { return m_.ncols; }
     int64_t nnz() const { return m_.nnz; }
+    // sizes of PetscMatrix (utopia_petsc_Matrix.hpp): one GPU owns every row
+    int64_t local_rows() const { return m_.nrows; }
+    int64_t local_cols() const { return m_.ncols; }
+    HipRange row_range() const { return HipRange{0, m_.nrows}; }
+    const HipCommunicator &comm() const { return HipCommunicator::get(); }
+
+    // PetscMatrix::read(f(i, j, a_ij)) (utopia_petsc_Matrix.hpp:482) over a host copy of the CSR
+    template <class F>
+    void read(F f) const {
+        std::vector<int32_t> rp, col;
+        std::vector<T> val;
+        to_host(rp, col, val);
+        for (int32_t i = 0; i < m_.nrows; ++i)
+            for (int32_t e = rp[(size_t)i]; e < rp[(size_t)i + 1]; ++e) f((int64_t)i, (int64_t)col[(size_t)e], val[(size_t)e]);
+    }
+    // PetscMatrix::transform_values(f) (utopia_petsc_Matrix.hpp:476): a_ij = f(a_ij) on a host copy,
+    // then the plan is rebuilt
+    template <class F>
+    void transform_values(F f) {
+        std::vector<int32_t> rp, col;
+        std::vector<T> val;
+        to_host(rp, col, val);
+        for (auto &v : val) v = f(v);
+        if (!val.empty()) hip_check(uhip_memcpy_htod(c(), m_.val, val.data(), sizeof(T) * val.size()), "transform_values");
+        replan();
+    }
+    // ||A||_inf = max_i sum_j |a_ij| (MatNorm NORM_INFINITY)
+    double norm_infty() const {
+        double best = 0;
+        std::vector<int32_t> rp, col;
+        std::vector<T> val;
+        to_host(rp, col, val);
+        for (int32_t i = 0; i < m_.nrows; ++i) {
+            double s = 0;
+            for (int32_t e = rp[(size_t)i]; e < rp[(size_t)i + 1]; ++e) s += std::fabs((double)val[(size_t)e]);
+            best = s > best ? s : best;
+        }
+        return best;
+    }
+    void to_host(std::vector<int32_t> &rp, std::vector<int32_t> &col, std::vector<T> &val) const {
+        rp.assign((size_t)m_.nrows + 1, 0);
+        col.assign((size_t)m_.nnz, 0);
+        val.assign((size_t)m_.nnz, T(0));
+        hip_check(uhip_memcpy_dtoh(c(), rp.data(), m_.rowptr, sizeof(int32_t) * rp.size()), "d2h rowptr");
+        if (m_.nnz) {
+            hip_check(uhip_memcpy_dtoh(c(), col.data(), m_.col, sizeof(int32_t) * col.size()), "d2h col");
+            hip_check(uhip_memcpy_dtoh(c(), val.data(), m_.val, sizeof(T) * val.size()), "d2h val");
+        }
+    }
     bool empty() const { return m_.rowptr == nullptr; }
     const uhip_csr &raw() const { return m_; }
     uhip_csr &raw() { return m_; }
@@ -431,6 +571,13 @@ public:
         hip_check(uhip_spmv(c(), &m_, x.data(), y.data()), "spmv");
     }
     void apply(const Vector &x, Vector &y) const { multiply(x, y); }
+
+    // y = A^T x  (PetscMatrix::transpose_multiply -> MatMultTranspose, utopia_petsc_Matrix.cpp:1389);
+    // the transpose and its plan are built on the first call and kept with A's plan
+    void transpose_multiply(const Vector &x, Vector &y) {
+        if (y.size() != cols() || y.empty()) y = Vector(ctx_, cols());
+        hip_check(uhip_spmv_transpose(c(), &m_, x.data(), y.data()), "transpose_multiply");
+    }
 
     // r = b - A x  (Eval<ResidualExpr>)
     void residual(const Vector &x, const Vector &b, Vector &r) const {

@@ -21,6 +21,8 @@
 #include "utopia_hip_solvers.hpp"
 
 namespace utopia {
+// the solvers of the HIP backend (the tensor types, HipVector and HipCrsMatrix, are in utopia)
+namespace hip {
 
 class DeviceInts {
 public:
@@ -562,4 +564,5 @@ inline std::shared_ptr<const HipCrsMatrixT<float>> BatchedMultigridCG::cast<floa
     return std::make_shared<HipCrsMatrix32>(m->cast_f32());
 }
 
+}  // namespace hip
 }  // namespace utopia

@@ -9,6 +9,8 @@
 #include "utopia_hip_qp.hpp"
 
 namespace utopia {
+// the solvers of the HIP backend (the tensor types, HipVector and HipCrsMatrix, are in utopia)
+namespace hip {
 
 class MonotoneMultigrid final : public QPSolverBase {
 public:
@@ -152,4 +154,5 @@ private:
     double as_tol_ = 0.0;
 };
 
+}  // namespace hip
 }  // namespace utopia

@@ -9,6 +9,8 @@
 #include "utopia_hip_solvers.hpp"
 
 namespace utopia {
+// the solvers of the HIP backend (the tensor types, HipVector and HipCrsMatrix, are in utopia)
+namespace hip {
 
 // Factorization<M,V> on the coarsest level: dense FP64 Cholesky, then the
 // explicit inverse so that every coarse solve is a single GEMV.
@@ -834,4 +836,5 @@ private:
     HipVector z64_;
 };
 
+}  // namespace hip
 }  // namespace utopia

@@ -12,6 +12,8 @@
 #include "utopia_hip_solvers.hpp"
 
 namespace utopia {
+// the solvers of the HIP backend (the tensor types, HipVector and HipCrsMatrix, are in utopia)
+namespace hip {
 
 struct BoxConstraints {
     std::shared_ptr<HipVector> lower, upper;
@@ -404,4 +406,5 @@ private:
     HipVector x_old_, x_half_, p_, u_, Ap_, d_;
 };
 
+}  // namespace hip
 }  // namespace utopia

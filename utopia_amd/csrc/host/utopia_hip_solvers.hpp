@@ -21,6 +21,8 @@
 #include "utopia_hip_backend.hpp"
 
 namespace utopia {
+// the solvers of the HIP backend (the tensor types, HipVector and HipCrsMatrix, are in utopia)
+namespace hip {
 
 // ConvergenceReason codes (solvers/utopia_ConvergenceReason.hpp:84-100)
 struct ConvergenceReason {
@@ -187,6 +189,13 @@ public:
     void update(const std::shared_ptr<const Matrix> &op) override {
         IterativeSolver<Matrix, Vector>::update(op);
         if (precond_) precond_->update(op);
+        init_memory(op->rows(), op->context());
+    }
+    // PreconditionedSolver::update(op, prec) (utopia_PreconditionedSolver.hpp:48-61): the
+    // iteration runs on op, the preconditioner is built from prec
+    void update(const std::shared_ptr<const Matrix> &op, const std::shared_ptr<const Matrix> &prec) {
+        IterativeSolver<Matrix, Vector>::update(op);
+        if (precond_) precond_->update(prec);
         init_memory(op->rows(), op->context());
     }
 
@@ -449,4 +458,5 @@ bool chebyshev_smooth_fused(const HipCrsMatrixT<T> &A, const void *b, int b_dt, 
     return dot_with != nullptr;
 }
 
+}  // namespace hip
 }  // namespace utopia

@@ -8,6 +8,7 @@
 #include "host/utopia_hip_batched.hpp"
 
 using namespace utopia;
+using namespace utopia::hip;
 
 struct uhip_bsolver_s {
     HipContextPtr ctx;

@@ -41,6 +41,7 @@ int64_t uhip_gen_nnz(int kind, int32_t nx, int32_t ny, int32_t nz) {
             return 5 * X * Y - 2 * X - 2 * Y;
         case 1:
         case 2:
+        case 5:
             return 7 * X * Y * Z - 2 * (X * Y + Y * Z + X * Z);
         case 3:
             return 9 * X * Y - 6 * X - 6 * Y + 4;
@@ -51,6 +52,42 @@ int64_t uhip_gen_nnz(int kind, int32_t nx, int32_t ny, int32_t nz) {
 
 static int gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift, int32_t *rowptr,
                        int32_t *col, double *val);
+
+// Kind 5: the kind-2 operator P A P^T, P the seeded permutation of its full 64-row blocks (a
+// tail block of fewer than 64 rows keeps its place). Rows keep their sorted column order.
+static int gen_block_permuted(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift, int32_t *rowptr,
+                              int32_t *col, double *val) {
+    const int64_t N = (int64_t)nx * ny * nz, nnz = uhip_gen_nnz(2, nx, ny, nz);
+    std::vector<int32_t> rp((size_t)N + 1), c((size_t)nnz);
+    std::vector<double> v((size_t)nnz);
+    if (const int st = gen_stencil(2, nx, ny, nz, seed, shift, rp.data(), c.data(), v.data())) return st;
+    const int64_t nb = N / 64, full = nb * 64;
+    std::vector<int64_t> to(nb), from(nb);  // to[old block] = new block; from = inverse
+    for (int64_t i = 0; i < nb; ++i) to[i] = i;
+    SplitMix64 rng(seed ^ 0x5851F42D4C957F2Dull);
+    for (int64_t i = nb - 1; i > 0; --i) {  // Fisher-Yates
+        const int64_t j = (int64_t)(rng.next() % (uint64_t)(i + 1));
+        const int64_t t = to[i];
+        to[i] = to[j];
+        to[j] = t;
+    }
+    for (int64_t i = 0; i < nb; ++i) from[to[i]] = i;
+    int64_t e = 0;
+    rowptr[0] = 0;
+    for (int64_t q = 0; q < N; ++q) {
+        const int64_t r = q < full ? from[q >> 6] * 64 + (q & 63) : q;
+        const int64_t s0 = e;
+        for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+            const int64_t cc = c[k] < full ? to[c[k] >> 6] * 64 + (c[k] & 63) : c[k];
+            int64_t t = e++;
+            for (; t > s0 && col[t - 1] > cc; --t) col[t] = col[t - 1], val[t] = val[t - 1];
+            col[t] = (int32_t)cc;
+            val[t] = v[k];
+        }
+        rowptr[q + 1] = (int32_t)e;
+    }
+    return UHIP_OK;
+}
 
 int uhip_gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double shift, int32_t *rowptr,
                      int32_t *col, double *val) {
@@ -104,6 +141,7 @@ static int gen_stencil(int kind, int32_t nx, int32_t ny, int32_t nz, uint64_t se
             }
         return UHIP_OK;
     }
+    if (kind == 5) return gen_block_permuted(nx, ny, nz, seed, shift, rowptr, col, val);
     if (kind != 1 && kind != 2) return UHIP_ERR_INVALID;
     const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny, N = sz * nz;
     std::vector<double> kap;

@@ -145,6 +145,16 @@ int uhip_ctx_destroy(uhip_ctx ctx) {
 
 int64_t uhip_ctx_workspace_generation(uhip_ctx ctx) { return ctx ? ctx->workspace_gen : -1; }
 
+int uhip_pointer_is_device(const void *p) {
+    if (!p) return 0;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // unregistered host memory: not an error
+        return 0;
+    }
+    return (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged) ? 1 : 0;
+}
+
 int uhip_ctx_set_gate(uhip_ctx ctx, const int32_t *gate_d) {
     UHIP_REQUIRE(ctx, "uhip_ctx_set_gate: null context");
     ctx->gate = gate_d;

@@ -13,6 +13,7 @@
 #include "host/utopia_hip_qp.hpp"
 
 using namespace utopia;
+using namespace utopia::hip;
 
 struct uhip_solver_s {
     int type = 0;
@@ -20,6 +21,7 @@ struct uhip_solver_s {
     std::map<std::string, double> params;
     std::vector<std::shared_ptr<HipCrsMatrix>> transfers;
     std::shared_ptr<HipCrsMatrix> A;
+    std::shared_ptr<HipCrsMatrix> prec;  // update_with_preconditioner: the preconditioner's matrix
     std::shared_ptr<LinearSolver<HipCrsMatrix, HipVector>> solver;
     std::shared_ptr<ConjugateGradient<HipCrsMatrix, HipVector>> cg;
     std::shared_ptr<Multigrid> mg;
@@ -242,7 +244,13 @@ int do_update(uhip_solver_s *s) {
     const auto t0 = std::chrono::steady_clock::now();
     build(s);
     set_bounds(s);
-    s->solver->update(s->A);
+    if (s->prec) {
+        // PreconditionedSolver::update(op, prec): only the CG types carry a preconditioner
+        if (!s->cg) throw HipError(UHIP_ERR_UNSUPPORTED, "update_with_preconditioner: not a preconditioned solver");
+        s->cg->update(s->A, s->prec);
+    } else {
+        s->solver->update(s->A);
+    }
     s->ctx->sync();
     s->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return UHIP_OK;
@@ -324,6 +332,7 @@ int uhip_solver_update_crs(uhip_solver s, int32_t n, const int32_t *rowptr, cons
     try {
         s->A = std::make_shared<HipCrsMatrix>();
         s->A->from_host(s->ctx, n, n, rowptr, col, val, (int32_t)param(s, "block_nnz", 0));
+        s->prec.reset();
         return do_update(s);
     } catch (const std::exception &e) {
         return fail(e);
@@ -335,6 +344,32 @@ int uhip_solver_update_crs_device(uhip_solver s, int32_t n, int64_t nnz, int32_t
     try {
         s->A = std::make_shared<HipCrsMatrix>();
         s->A->wrap(s->ctx, n, n, nnz, rowptr, col, val, (int32_t)param(s, "block_nnz", 0));
+        s->prec.reset();
+        return do_update(s);
+    } catch (const std::exception &e) {
+        return fail(e);
+    }
+}
+
+int uhip_solver_update_with_preconditioner_crs(uhip_solver s, int32_t n, const int32_t *rowptr, const int32_t *col,
+                                               const double *val, const int32_t *prec_rowptr,
+                                               const int32_t *prec_col, const double *prec_val, int on_device) {
+    if (!s || n < 0 || !rowptr || !col || !val || !prec_rowptr || !prec_col || !prec_val) return UHIP_ERR_INVALID;
+    try {
+        const int32_t bn = (int32_t)param(s, "block_nnz", 0);
+        auto A = std::make_shared<HipCrsMatrix>(), P = std::make_shared<HipCrsMatrix>();
+        if (on_device) {
+            int32_t nz[2] = {0, 0};
+            hip_check(uhip_memcpy_dtoh(s->ctx->get(), &nz[0], rowptr + n, sizeof(int32_t)), "nnz");
+            hip_check(uhip_memcpy_dtoh(s->ctx->get(), &nz[1], prec_rowptr + n, sizeof(int32_t)), "prec nnz");
+            A->wrap(s->ctx, n, n, nz[0], (int32_t *)rowptr, (int32_t *)col, (double *)val, bn);
+            P->wrap(s->ctx, n, n, nz[1], (int32_t *)prec_rowptr, (int32_t *)prec_col, (double *)prec_val, bn);
+        } else {
+            A->from_host(s->ctx, n, n, rowptr, col, val, bn);
+            P->from_host(s->ctx, n, n, prec_rowptr, prec_col, prec_val, bn);
+        }
+        s->A = A;
+        s->prec = P;
         return do_update(s);
     } catch (const std::exception &e) {
         return fail(e);
@@ -531,10 +566,12 @@ int uhip_solver_get_mask(uhip_solver s, int l, double *mask) {
 
 // Override the Chebyshev bounds of level l after update (tests pinning the cycle).
 int uhip_solver_get_colors(uhip_solver s, int l, int32_t *color, int32_t *ncolors) {
-    if (!s || !s->mg || !color || !ncolors || l < 1 || l >= s->mg->n_levels()) return UHIP_ERR_INVALID;
+    if (!s || !s->mg || !color || !ncolors || l < 1 || l >= s->mg->n_levels())
+        return uhip::set_error(UHIP_ERR_INVALID, "uhip_solver_get_colors: no smoothed level %d (levels 1..n_levels-1)", l);
     try {
         const auto &lv = s->mg->engine().levels[(size_t)l];
-        if (!lv.groups) return UHIP_ERR_INVALID;
+        if (!lv.groups)
+            return uhip::set_error(UHIP_ERR_INVALID, "uhip_solver_get_colors: level %d has no colouring (GaussSeidel smoother only)", l);
         *ncolors = lv.groups->ncolors();
         hip_check(uhip_memcpy_dtoh(s->ctx->get(), color, lv.groups->color(), sizeof(int32_t) * (size_t)lv.A->rows()),
                   "colors");

@@ -227,9 +227,10 @@ int uhip_csr_free(uhip_ctx ctx, uhip_csr *A) {
     return uhip_csr_plan_destroy(ctx, A);
 }
 
-int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {
-    UHIP_REQUIRE(ctx && A && B, "uhip_csr_transpose: null argument");
-    UHIP_REQUIRE(A->dtype == UHIP_F64, "uhip_csr_transpose: FP64 matrices only (setup precision)");
+}  // extern "C"
+
+template <typename T>
+static int transpose_typed(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {
     hipStream_t s = ctx->stream;
     const int64_t nnz = A->nnz;
     const int32_t n = A->nrows, m = A->ncols;
@@ -237,10 +238,10 @@ int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {
     B->nrows = m;
     B->ncols = n;
     B->nnz = nnz;
-    B->dtype = UHIP_F64;
+    B->dtype = A->dtype;
     UHIP_CHECK_HIP(hipMalloc(&B->rowptr, sizeof(int32_t) * ((size_t)m + 1)));
     UHIP_CHECK_HIP(hipMalloc(&B->col, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
-    UHIP_CHECK_HIP(hipMalloc(&B->val, sizeof(double) * std::max<int64_t>(nnz, 1)));
+    UHIP_CHECK_HIP(hipMalloc(&B->val, sizeof(T) * std::max<int64_t>(nnz, 1)));
     int32_t *cnt = nullptr;
     UHIP_CHECK_HIP(hipMalloc(&cnt, sizeof(int32_t) * ((size_t)m + 1)));
     UHIP_CHECK_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * ((size_t)m + 1), s));
@@ -257,20 +258,20 @@ int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {
     tmp = nullptr;
     if (nnz > 0) {
         uint64_t *k0 = nullptr, *k1 = nullptr;
-        double *v0 = nullptr;
+        T *v0 = nullptr;
         UHIP_CHECK_HIP(hipMalloc(&k0, sizeof(uint64_t) * nnz));
         UHIP_CHECK_HIP(hipMalloc(&k1, sizeof(uint64_t) * nnz));
-        UHIP_CHECK_HIP(hipMalloc(&v0, sizeof(double) * nnz));
-        hipLaunchKernelGGL(k_transpose_keys<double>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
-                           A->rowptr, A->col, (const double *)A->val, k0, v0);
+        UHIP_CHECK_HIP(hipMalloc(&v0, sizeof(T) * nnz));
+        hipLaunchKernelGGL(k_transpose_keys<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                           A->rowptr, A->col, (const T *)A->val, k0, v0);
         UHIP_CHECK_LAUNCH();
         int hi_bits = 1;
         while ((1ll << hi_bits) < (int64_t)m) ++hi_bits;
         const int end_bit = 32 + hi_bits;
-        UHIP_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, (double *)B->val, (int)nnz, 0,
+        UHIP_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, (T *)B->val, (int)nnz, 0,
                                                           end_bit, s));
         UHIP_CHECK_HIP(hipMalloc(&tmp, tb));
-        UHIP_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, (double *)B->val, (int)nnz, 0,
+        UHIP_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, (T *)B->val, (int)nnz, 0,
                                                           end_bit, s));
         hipLaunchKernelGGL(k_keys_low, dim3(grid_for(nnz, 4, 1 << 16)), dim3(kBlock), 0, s, nnz, k1, B->col);
         UHIP_CHECK_LAUNCH();
@@ -283,6 +284,15 @@ int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
     hipFree(cnt);
     return UHIP_OK;
+}
+
+extern "C" {
+
+int uhip_csr_transpose(uhip_ctx ctx, const uhip_csr *A, uhip_csr *B) {
+    UHIP_REQUIRE(ctx && A && B, "uhip_csr_transpose: null argument");
+    if (A->dtype == UHIP_F64) return transpose_typed<double>(ctx, A, B);
+    if (A->dtype == UHIP_F32) return transpose_typed<float>(ctx, A, B);
+    return set_error(UHIP_ERR_INVALID, "uhip_csr_transpose: dtype %d", A->dtype);
 }
 
 int uhip_csr_spgemm(uhip_ctx ctx, const uhip_csr *A, const uhip_csr *B, uhip_csr *C) {

@@ -207,12 +207,15 @@ __global__ __launch_bounds__(kBlock) void k_dia_scan(int32_t n, const int32_t *_
     if (threadIdx.x == 0 && bad) atomicOr(gflags, bad);
 }
 
-// Row r writes all ndiag slots of its diagonals (value or zero) and its mask.
+// Row r writes all ndiag slots of its diagonals (its value, or NaN where the row does not hold
+// the diagonal) and its mask; has_nan[0] |= 1 when a stored value is NaN (the plan then zeroes
+// the absent slots instead).
 template <typename T, typename TM>
 __global__ __launch_bounds__(kBlock) void k_dia_fill(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
                                                      const int32_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
-                                                     T *__restrict__ dval, TM *__restrict__ mask) {
+                                                     T *__restrict__ dval, TM *__restrict__ mask,
+                                                     int32_t *__restrict__ has_nan) {
     __shared__ int32_t so[32];
     if (threadIdx.x < ndiag) so[threadIdx.x] = off[threadIdx.x];
     __syncthreads();
@@ -221,17 +224,32 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int32_t n, int32_t ndiag, c
     int32_t e = rowptr[r];
     const int32_t e1 = rowptr[r + 1];
     uint32_t m = 0;
+    bool nan_seen = false;
     const int64_t ld = dia_ld(n);
     for (int d = 0; d < ndiag; ++d) {
-        T v = T(0);
+        T v = (T)__builtin_nan("");
         if (e < e1 && col[e] - r == so[d]) {
             v = val[e];
+            nan_seen |= (v != v);
             m |= 1u << d;
             ++e;
         }
         dval[(int64_t)d * ld + r] = v;
     }
     mask[r] = (TM)m;
+    if (nan_seen) atomicOr(has_nan, 1);
+}
+
+// absent slots to zero (a matrix that holds NaN values keeps the mask as the only record)
+template <typename T, typename TM>
+__global__ __launch_bounds__(kBlock) void k_dia_zero_absent(int32_t n, int32_t ndiag, T *__restrict__ dval,
+                                                            const TM *__restrict__ mask) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t ld = dia_ld(n);
+    const uint32_t m = (uint32_t)mask[r];
+    for (int d = 0; d < ndiag; ++d)
+        if (!((m >> d) & 1u)) dval[(int64_t)d * ld + r] = T(0);
 }
 
 __device__ __forceinline__ bool same_bits(double a, double b) { return __double_as_longlong(a) == __double_as_longlong(b); }
@@ -295,9 +313,10 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     UHIP_CHECK_HIP(hipMalloc(&A->dia_val, vb * (size_t)nd * (size_t)dia_ld(n)));
     UHIP_CHECK_HIP(hipMalloc(&A->dia_mask, mb * (size_t)n));
     const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+    UHIP_CHECK_HIP(hipMemsetAsync(flags, 0, sizeof(int32_t), s));
 #define UHIP_DIA_FILL(T, TM)                                                                                      \
     hipLaunchKernelGGL((k_dia_fill<T, TM>), g, b, 0, s, n, nd, table, A->rowptr, A->col, (const T *)A->val,      \
-                       (T *)A->dia_val, (TM *)A->dia_mask)
+                       (T *)A->dia_val, (TM *)A->dia_mask, flags)
     if (A->dtype == UHIP_F64) {
         if (mb == 1) UHIP_DIA_FILL(double, uint8_t);
         else if (mb == 2) UHIP_DIA_FILL(double, uint16_t);
@@ -309,6 +328,24 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     }
 #undef UHIP_DIA_FILL
     UHIP_CHECK_LAUNCH();
+    int32_t has_nan = 0;
+    UHIP_CHECK_HIP(hipMemcpyAsync(&has_nan, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    if (has_nan) {
+#define UHIP_DIA_ZERO(T, TM) \
+    hipLaunchKernelGGL((k_dia_zero_absent<T, TM>), g, b, 0, s, n, nd, (T *)A->dia_val, (const TM *)A->dia_mask)
+        if (A->dtype == UHIP_F64) {
+            if (mb == 1) UHIP_DIA_ZERO(double, uint8_t);
+            else if (mb == 2) UHIP_DIA_ZERO(double, uint16_t);
+            else UHIP_DIA_ZERO(double, uint32_t);
+        } else {
+            if (mb == 1) UHIP_DIA_ZERO(float, uint8_t);
+            else if (mb == 2) UHIP_DIA_ZERO(float, uint16_t);
+            else UHIP_DIA_ZERO(float, uint32_t);
+        }
+#undef UHIP_DIA_ZERO
+        UHIP_CHECK_LAUNCH();
+    }
     // symmetric offsets -o..0..+o: keep only 0..+o when the values are symmetric bit for bit
     bool sym_offs = (nd & 1) && offs[(size_t)nd / 2] == 0;
     for (int32_t d = 0; d < nd && sym_offs; ++d) sym_offs = offs[(size_t)d] == -offs[(size_t)(nd - 1 - d)];
@@ -346,8 +383,7 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     }
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
     // symmetric 7-point operators with the largest offset a multiple of 64 take the
-    // plane-blocked storage (the same arrays; in FP64 only the dot-carrying products run
-    // the plane-blocked kernel, see launch_rowblock)
+    // plane-walking 7-point kernel (the same arrays)
     const int planes = dia7_planes(A->dtype);
     if (sym && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] && offs[6] % 64 == 0 &&
         (int64_t)n + (int64_t)planes * offs[6] < INT32_MAX)
@@ -355,6 +391,7 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->dia_sym = sym;
+    A->dia_nan = has_nan ? 0 : 1;
     A->format = UHIP_FMT_DIA;
     A->nparts = sym == 2 ? dia7z_workgroups(n, offs[6], planes)
                          : (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
@@ -810,6 +847,13 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
 
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     UHIP_REQUIRE(A, "uhip_csr_plan_destroy: null matrix");
+    if (A->tr) {
+        uhip_csr *t = A->tr;
+        A->tr = nullptr;
+        const int st = uhip_csr_free(ctx, t);
+        delete t;
+        if (st != UHIP_OK) return st;
+    }
     if (A->blocks || A->slice_off || A->sell_col || A->sell_val || A->dia_off || A->dia_val || A->dia_mask ||
         A->vi_base || A->vi_table) {
         if (ctx) UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -827,6 +871,7 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     A->dia_mask = nullptr;
     A->ndiag = 0;
     A->dia_sym = 0;
+    A->dia_nan = 0;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));
@@ -847,6 +892,28 @@ int uhip_spmv(uhip_ctx ctx, const uhip_csr *A, const void *x, void *y) {
     if (A->dtype == UHIP_F64) return spmv_typed<double, double>(ctx, A, x, y);
     if (A->dtype == UHIP_F32) return spmv_typed<float, float>(ctx, A, x, y);
     return set_error(UHIP_ERR_INVALID, "uhip_spmv: dtype %d", A->dtype);
+}
+
+int uhip_csr_plan_transpose(uhip_ctx ctx, uhip_csr *A) {
+    UHIP_REQUIRE(ctx && A, "uhip_csr_plan_transpose: null argument");
+    if (A->tr) return UHIP_OK;
+    auto *t = new uhip_csr();
+    memset(t, 0, sizeof(*t));
+    int st = uhip_csr_transpose(ctx, A, t);
+    if (st == UHIP_OK) st = uhip_csr_plan(ctx, t, 0);
+    if (st != UHIP_OK) {
+        uhip_csr_free(ctx, t);
+        delete t;
+        return st;
+    }
+    A->tr = t;
+    return UHIP_OK;
+}
+
+int uhip_spmv_transpose(uhip_ctx ctx, uhip_csr *A, const void *x, void *y) {
+    UHIP_REQUIRE(ctx && A && x && y, "uhip_spmv_transpose: null argument");
+    UHIP_TRY(uhip_csr_plan_transpose(ctx, A));
+    return uhip_spmv(ctx, A->tr, x, y);
 }
 
 int uhip_spmv_axpby(uhip_ctx ctx, const uhip_csr *A, double alpha, const void *x, double beta, void *y) {

@@ -256,28 +256,66 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia(int32_t nrows, int32_t ndia
     }
 }
 
-// Symmetric 7-point DIA (offsets -S2, -S1, -1, 0, +1, +S1, +S2, S2 a multiple of 64;
-// stored: main, +1, +S1, +S2), blocked over the largest offset: a wave owns 64
-// consecutive rows of P consecutive planes (plane stride S2), lane l
-// row slab * P * S2 + z * S2 + seg * 64 + l (P = Dia7Planes). The -S2 mirror and x at +-S2 come from
-// the neighbouring plane in registers, the -1 mirror from a wave shuffle of the +1
-// values; the +S1 diagonal is read through the cache, so its -S1 mirror (the same
-// array S1 rows back, read by the wave S1/64 segments earlier) hits L2, and the other
-// diagonals stream. The row sums keep the column order (bit-identical to the CSR sum).
-// planes per wave: 4 for FP32; 2 for FP64, whose 4-plane form holds 110+ VGPRs
+// planes per wave of the 7-point kernel (k_spmv_dia7r): FP64 walks 8 planes with the loads of
+// the next 2 in flight; FP32 issues the loads of its 4 planes up front
 template <typename TA>
 struct Dia7Planes {
-    static constexpr int P = sizeof(TA) == 8 ? 2 : 4;
+    static constexpr int P = sizeof(TA) == 8 ? 8 : 4;
+    static constexpr int LA = sizeof(TA) == 8 ? 2 : 4;
 };
 inline int dia7_planes(int dtype) { return dtype == UHIP_F64 ? Dia7Planes<double>::P : Dia7Planes<float>::P; }
 
-template <typename TAcc, class Epi, typename TA, typename TX>
-__global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int32_t *__restrict__ off,
+inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2, int planes) {
+    const int64_t slabs = ((int64_t)nrows + (int64_t)planes * S2 - 1) / ((int64_t)planes * S2);
+    const int64_t waves = slabs * (S2 / 64);
+    return (int32_t)((waves + kDiaWG / 64 - 1) / (kDiaWG / 64));
+}
+
+// Epilogues that read the product's own input at row r (the x of A x) take it from the
+// kernel's registers in the rolling 7-point kernel: they define load_nox (the other inputs)
+// and apply_x (the row sum, those inputs and x_r).
+template <class Epi, class = void>
+struct EpiTakesX {
+    static constexpr bool value = false;
+};
+template <class Epi>
+struct EpiTakesX<Epi, decltype((void)&Epi::template apply_x<double>, void())> {
+    static constexpr bool value = true;
+};
+
+// ±1 diagonal with its -1 mirror: the stored +1 value of rows r - 1 and r as one 2-vector load
+template <typename T>
+struct PairOf;
+template <>
+struct PairOf<double> {
+    typedef double t __attribute__((ext_vector_type(2), aligned(8)));
+};
+template <>
+struct PairOf<float> {
+    typedef float t __attribute__((ext_vector_type(2), aligned(4)));
+};
+
+// Symmetric 7-point DIA (offsets -S2, -S1, -1, 0, +1, +S1, +S2, S2 a multiple of 64; stored:
+// main, +1, +S1, +S2), streamed plane by plane: a wave owns 64 consecutive rows (a segment) of
+// each of P consecutive planes, lane l row slab * P * S2 + z * S2 + seg * 64 + l, and walks the
+// planes with the loads of the next LA planes in flight (LA = P: all issued up front). Per row:
+//  - the -S2 entry is the +S2 value of the row one plane back (registers), x at +-S2 the
+//    neighbouring planes' x (registers; x at +S2 is loaded with the plane and becomes x_r of
+//    the next one);
+//  - the -1 entry is the +1 value of row r - 1, loaded with the row's own +1 value as one
+//    2-vector load (no cross-lane exchange, no extra load at the segment edge);
+//  - the -S1 entry is the +S1 value of row r - S1, read through the cache (the wave S1/64
+//    segments back read it);
+//  - entries the row does not hold are NaN in the value arrays of a NaN-free matrix (NANSEL, the
+//    plan's dia_nan) and are dropped by a select on the value, or dropped by the row's mask bits.
+// Every row sum adds the row's CSR entries in column order: bit-identical to the CSR sum.
+template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool NANSEL>
+__global__ __launch_bounds__(kDiaWG) void k_spmv_dia7r(int32_t nrows, const int32_t *__restrict__ off,
                                                        const TA *__restrict__ val, const uint8_t *__restrict__ mask,
                                                        const TX *__restrict__ x, Epi epi,
                                                        double *__restrict__ partials, int64_t stride) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
-    constexpr int P = Dia7Planes<TA>::P;
+    constexpr bool TX_IN = EpiTakesX<Epi>::value;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
     const int64_t ld = dia_ld(nrows);
@@ -291,55 +329,73 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
     for (int d = 0; d < ND; ++d) part[d] = 0.0;
     if (slab0 < nrows) {
         const TA *v0 = val, *v1 = val + ld, *vS1 = val + 2 * ld, *vS2 = val + 3 * ld;
-        const int32_t last = (int32_t)ld - 1;
-        int32_t r[P];
-        uint32_t m[P];
-        TA a0[P], a1[P], aS1[P], aS2[P], mS1[P], edge[P];
-        typename Epi::In ein[P];
-        TX xc[P], xm1[P], xp1[P], xmS1[P], xpS1[P];
-        // x is read at clamped rows whether or not the entry exists (the mask only
-        // selects in the sum), so no load waits for the mask
-        const int32_t xlast = nrows - 1;
+        const TA nan = (TA)__builtin_nan("");
+        const int32_t last = (int32_t)ld - 1, xlast = nrows - 1;
         auto cl = [xlast](int32_t v) { return min(max(v, 0), xlast); };
+        const int32_t rb = (int32_t)slab0 + seg * 64 + lane;
+        struct Plane {
+            TA a0, a1, m1, aS1, aS2, mS1;
+            uint32_t m;
+            typename Epi::In ein;
+            TX xm1, xp1, xmS1, xpS1, xn;
+        };
+        // x and the mirrors are read at clamped rows whether or not the entry exists (the select
+        // drops it), so no load waits for another; mirrors before row 0 are NaN
+        auto load = [&](int z) {
+            Plane q;
+            const int32_t r = rb + z * S2;
+            const int32_t rc = min(r, last);
+            if constexpr (!NANSEL) q.m = r < nrows ? (uint32_t)mask[min(r, xlast)] : 0u;
+            else q.m = 0u;
+            q.a0 = ld_stream(v0 + rc);
+            const typename PairOf<TA>::t w =
+                __builtin_nontemporal_load((const typename PairOf<TA>::t *)(v1 + max(rc - 1, 0)));
+            q.m1 = rc >= 1 ? w.x : nan;
+            q.a1 = rc >= 1 ? w.y : w.x;
+            q.aS1 = vS1[rc];
+            const TA ms = vS1[max(rc - S1, 0)];
+            q.mS1 = rc >= S1 ? ms : nan;
+            q.aS2 = ld_stream(vS2 + rc);
+            if constexpr (TX_IN) q.ein = epi.load_nox(min(r, xlast));
+            else q.ein = epi.load(min(r, xlast));
+            q.xm1 = x[cl(r - 1)];
+            q.xp1 = x[cl(r + 1)];
+            q.xmS1 = x[cl(r - S1)];
+            q.xpS1 = x[cl(r + S1)];
+            q.xn = x[cl(r + S2)];
+            return q;
+        };
+        const TA ps = ld_stream(vS2 + max(min(rb, last) - S2, 0));
+        TA prev_aS2 = rb >= S2 ? ps : nan;
+        TX prev_xc = x[cl(rb - S2)];
+        TX cur_xc = x[cl(rb)];
+        Plane q[LA + 1];
 #pragma unroll
+        for (int i = 0; i < LA; ++i) q[i] = load(i);
+        // with every plane's loads issued up front (LA == P) the walk is unrolled
+#pragma unroll(LA >= P ? P : 1)
         for (int z = 0; z < P; ++z) {
-            r[z] = (int32_t)(slab0 + (int64_t)z * S2) + seg * 64 + lane;
-            const int32_t rc = min(r[z], last);
-            m[z] = (uint32_t)mask[min(r[z], xlast)];
-            a0[z] = ld_stream(v0 + rc);
-            a1[z] = ld_stream(v1 + rc);
-            aS1[z] = vS1[rc];
-            aS2[z] = ld_stream(vS2 + rc);
-            mS1[z] = vS1[max(rc - S1, 0)];
-            // the -1 mirror of lane 0 (the others shuffle it); loaded by lane 0 alone, since a
-            // whole-wave load would fetch again the lines the +1 stream just read
-            edge[z] = lane == 0 ? v1[max(rc - 1, 0)] : TA(0);
-            ein[z] = epi.load(min(r[z], xlast));
-            xc[z] = x[cl(r[z])];
-            xm1[z] = x[cl(r[z] - 1)];
-            xp1[z] = x[cl(r[z] + 1)];
-            xmS1[z] = x[cl(r[z] - S1)];
-            xpS1[z] = x[cl(r[z] + S1)];
-        }
-#pragma unroll
-        for (int z = 0; z < P; ++z) m[z] = r[z] < nrows ? m[z] : 0u;
-        const TA mS2_0 = ld_stream(vS2 + max(min(r[0], last) - S2, 0));
-        const TX xmS2_0 = x[cl(r[0] - S2)];
-        const TX xpS2_P = x[cl(r[P - 1] + S2)];
-#pragma unroll
-        for (int z = 0; z < P; ++z) {
-            const TA up = __shfl_up(a1[z], 1, 64);
-            const TA m1 = lane == 0 ? edge[z] : up;
-            const TA vals[7] = {z == 0 ? mS2_0 : aS2[z > 0 ? z - 1 : 0], mS1[z], m1, a0[z], a1[z], aS1[z], aS2[z]};
-            const TX xs[7] = {z == 0 ? xmS2_0 : xc[z > 0 ? z - 1 : 0], xmS1[z], xm1[z], xc[z], xp1[z], xpS1[z],
-                              z == P - 1 ? xpS2_P : xc[z < P - 1 ? z + 1 : 0]};
+            if (z + LA < P) q[LA] = load(z + LA);
+            const Plane &c = q[0];
+            const TA vals[7] = {prev_aS2, c.mS1, c.m1, c.a0, c.a1, c.aS1, c.aS2};
+            const TX xs[7] = {prev_xc, c.xmS1, c.xm1, cur_xc, c.xp1, c.xpS1, c.xn};
             TAcc acc = TAcc(0);
 #pragma unroll
             for (int d = 0; d < 7; ++d) {
-                const TAcc p = (TAcc)vals[d] * (TAcc)xs[d];
-                acc = ((m[z] >> d) & 1u) ? acc + p : acc;
+                const TAcc pr = (TAcc)vals[d] * (TAcc)xs[d];
+                if constexpr (NANSEL) acc = (vals[d] == vals[d]) ? acc + pr : acc;
+                else acc = ((c.m >> d) & 1u) ? acc + pr : acc;
             }
-            if (r[z] < nrows) epi.apply(r[z], acc, ein[z], part);
+            const int32_t r = rb + z * S2;
+            if (r < nrows) {
+                if constexpr (TX_IN) epi.apply_x(r, acc, c.ein, cur_xc, part);
+                else epi.apply(r, acc, c.ein, part);
+            }
+            prev_aS2 = c.aS2;
+            prev_xc = cur_xc;
+            cur_xc = c.xn;
+#pragma unroll
+            for (int i = 0; i < LA; ++i) q[i] = q[i + 1];
         }
     }
     if constexpr (Epi::kDots > 0) {
@@ -349,12 +405,6 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7z(int32_t nrows, const int3
             if (threadIdx.x == 0) partials[d * stride + b] = t;
         }
     }
-}
-
-inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2, int planes) {
-    const int64_t slabs = ((int64_t)nrows + (int64_t)planes * S2 - 1) / ((int64_t)planes * S2);
-    const int64_t waves = slabs * (S2 / 64);
-    return (int32_t)((waves + kDiaWG / 64 - 1) / (kDiaWG / 64));
 }
 
 // Rows per lane of the VI kernel: for at most 8 slots 4 (FP32) or 2 (FP64), else 1
@@ -496,17 +546,6 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
     }
 }
 
-template <typename TAcc, typename TY>
-struct EpiStore;
-template <class Epi>
-struct IsEpiStore {
-    static constexpr bool value = false;
-};
-template <typename TAcc, typename TY>
-struct IsEpiStore<EpiStore<TAcc, TY>> {
-    static constexpr bool value = true;
-};
-
 // Dispatch over the plan's format and LDS budget (block_nnz = 4/5 * CAPK * WG).
 // Matrix values have the accumulation type TAcc.
 // With parts, the per-workgroup partial sums of a dot-carrying epilogue go to parts (cap
@@ -557,18 +596,16 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
 #define UHIP_DIA_LAUNCH(TM, SYM)                                                                                  \
     hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, TM, SYM>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag, \
                        A->dia_off, v, (const TM *)A->dia_mask, x, epi, parts, pstride)
-        // FP64 plane-blocked storage: every product with dots and the plain product run the
-        // plane-blocked kernel (its grid is nparts, the partial sums the finish reads); the
-        // others (smoothing steps, residuals) run the symmetric kernel over the same arrays,
-        // which is faster for them in FP64 and writes no partial sums.
-        const bool blocked = A->dia_sym == 2 && (sizeof(TA) == 4 || Epi::kDots > 0 || IsEpiStore<Epi>::value);
-        if (blocked) {
-            hipLaunchKernelGGL((k_spmv_dia7z<TAcc, Epi, TA, TX>), g, blk, 0, stream_of(ctx), A->nrows, A->dia_off, v,
-                               (const uint8_t *)A->dia_mask, x, epi, parts, pstride);
-        } else if (A->dia_sym == 2) {
-            const dim3 gs((A->nrows + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype));
-            hipLaunchKernelGGL((k_spmv_dia<TAcc, Epi, TA, TX, uint8_t, true>), gs, blk, 0, stream_of(ctx), A->nrows,
-                               A->ndiag, A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, parts, pstride);
+        // symmetric 7-point storage: the plane-walking kernel (its grid is nparts, the partial
+        // sums the finish reads), selecting on NaN-encoded absent entries when the plan has them
+        if (A->dia_sym == 2) {
+            constexpr int P = Dia7Planes<TA>::P, LA = Dia7Planes<TA>::LA;
+            if (A->dia_nan)
+                hipLaunchKernelGGL((k_spmv_dia7r<TAcc, Epi, TA, TX, P, LA, true>), g, blk, 0, stream_of(ctx), A->nrows,
+                                   A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, parts, pstride);
+            else
+                hipLaunchKernelGGL((k_spmv_dia7r<TAcc, Epi, TA, TX, P, LA, false>), g, blk, 0, stream_of(ctx), A->nrows,
+                                   A->dia_off, v, (const uint8_t *)A->dia_mask, x, epi, parts, pstride);
         } else if (A->ndiag <= 8) {
             if (A->dia_sym) UHIP_DIA_LAUNCH(uint8_t, true);
             else UHIP_DIA_LAUNCH(uint8_t, false);
@@ -684,12 +721,12 @@ struct EpiResidual {
     };
     const TB *b;
     TY *y;
-    __device__ In load(int r) const { return In{(TAcc)b[r]}; }
+    __device__ In load(int r) const { return In{(TAcc)ld_stream(b + r)}; }
     __device__ void apply(int r, TAcc s, const In &in, double *) const { st_stream(y + r, (TY)(in.b - s)); }
     __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
 };
 
-// y = s, part += x_r * s  (p.Ap of CG)
+// y = s, part += x_r * s  (p.Ap of CG); x is the product's input
 template <typename TAcc, typename TX, typename TY>
 struct EpiStoreDot {
     static constexpr int kDots = 1;
@@ -704,6 +741,11 @@ struct EpiStoreDot {
         part[0] += in.x * (double)s;
     }
     __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
+    __device__ In load_nox(int) const { return In{0.0}; }
+    template <typename TXV>
+    __device__ void apply_x(int r, TAcc s, const In &, TXV xr, double *part) const {
+        apply(r, s, In{(double)xr}, part);
+    }
 };
 
 // y = s, part[0] += x_r s, part[1] += z_r x_r   (A x with x.Ax and z.x in one pass)
@@ -716,13 +758,18 @@ struct EpiStoreDot2 {
     const TX *x;
     const double *z;
     TY *y;
-    __device__ In load(int r) const { return In{(double)x[r], z[r]}; }
+    __device__ In load(int r) const { return In{(double)x[r], ld_stream(z + r)}; }
     __device__ void apply(int r, TAcc s, const In &in, double *part) const {
         st_stream(y + r, (TY)s);
         part[0] += in.x * (double)s;
         part[1] += in.z * in.x;
     }
     __device__ void operator()(int r, TAcc s, double *part) const { apply(r, s, load(r), part); }
+    __device__ In load_nox(int r) const { return In{0.0, ld_stream(z + r)}; }
+    template <typename TXV>
+    __device__ void apply_x(int r, TAcc s, const In &in, TXV xr, double *part) const {
+        apply(r, s, In{(double)xr, in.z}, part);
+    }
 };
 
 // y = mask .* s  (restriction followed by MultiLevelMask::apply); mask may be null
@@ -770,7 +817,15 @@ struct EpiJacobi {
     const TAcc *xin;
     TXO *out;
     __device__ In load(int r) const {
-        return In{(TAcc)b[r], d[r], (ADD && xin) ? xin[r] : TAcc(0)};
+        return In{(TAcc)ld_stream(b + r), ld_stream(d + r), (ADD && xin) ? xin[r] : TAcc(0)};
+    }
+    // xin is the product's input: the rolling 7-point kernel hands x_r over
+    __device__ In load_nox(int r) const { return In{(TAcc)ld_stream(b + r), ld_stream(d + r), TAcc(0)}; }
+    template <typename TXV>
+    __device__ void apply_x(int r, TAcc s, const In &in0, TXV xr, double *part) const {
+        In in = in0;
+        in.x = ADD ? (TAcc)xr : TAcc(0);
+        apply(r, s, in, part);
     }
     __device__ void apply(int r, TAcc s, const In &in, double *) const {
         const TAcc res = in.b - s;
@@ -806,10 +861,26 @@ struct EpiCheb {
     __device__ In load(int r) const {
         In in;
         in.pp = (TAcc)pprev[r];
-        in.rp = rprev ? (TAcc)rprev[r] : -in.pp;
-        in.x = xin ? (TAcc)xin[r] : TAcc(0);
-        in.dw = DOT ? dot_with[r] : 0.0;
+        in.rp = rprev ? (TAcc)ld_stream(rprev + r) : -in.pp;
+        in.x = xin ? (TAcc)ld_stream(xin + r) : TAcc(0);
+        in.dw = DOT ? ld_stream(dot_with + r) : 0.0;
         return in;
+    }
+    // pprev is the product's input: the rolling 7-point kernel hands p_{k-1} at row r over
+    __device__ In load_nox(int r) const {
+        In in;
+        in.pp = TAcc(0);
+        in.rp = rprev ? (TAcc)ld_stream(rprev + r) : TAcc(0);
+        in.x = xin ? (TAcc)ld_stream(xin + r) : TAcc(0);
+        in.dw = DOT ? ld_stream(dot_with + r) : 0.0;
+        return in;
+    }
+    template <typename TXV>
+    __device__ void apply_x(int r, TAcc s, const In &in0, TXV xr, double *part) const {
+        In in = in0;
+        in.pp = (TAcc)xr;
+        if (!rprev) in.rp = -in.pp;
+        apply(r, s, in, part);
     }
     __device__ void apply(int r, TAcc s, const In &in, double *part) const {
         const TAcc rk = in.rp + a_prev * s;
