@@ -66,6 +66,12 @@ int uhip_pointer_is_device(const void *p);
 int uhip_memcpy_dtoh(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
 int uhip_memcpy_dtod(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
 int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h);
+/* Asynchronous read-back of n device scalars: fetch enqueues the copy on the context stream
+ * (the host goes on enqueuing work), wait blocks until that copy is done and returns the values.
+ * One fetch in flight per context: a second fetch replaces the first. No reference analogue
+ * (it lets the CG enqueue its preconditioner before it reads alpha and |r|). */
+int uhip_fetch_scalars(uhip_ctx ctx, const double *src_d, int n);
+int uhip_fetch_wait(uhip_ctx ctx, int n, double *dst_h);
 int uhip_write_scalars(uhip_ctx ctx, double *dst_d, int n, const double *src_h);
 int uhip_event_timer_start(uhip_ctx ctx, void **ev_pair);
 int uhip_event_timer_stop(uhip_ctx ctx, void *ev_pair, float *ms);
@@ -160,12 +166,13 @@ int uhip_cg_pupdate_f64(uhip_ctx ctx, int64_t n, double *p, const double *z, dou
  * identical arithmetic, x_{k+1} = x_k + alpha_k p_k is formed before p_k is replaced):
  * cg_residual:  alpha = s[i_num]/s[i_den]; r_new = r - alpha Ap (r_new32 = (float)r_new if given);
  *               s[i_alpha] = alpha; s[i_rnorm] = |r_new|_2  (non-finite alpha: nothing written but s[i_alpha])
- * cg_xpupdate:  x += s[i_alpha] p; if z: p = z + (s[i_num]/s[i_den]) p, then s[i_den] = s[i_num] */
+ * cg_xpupdate:  x += s[i_alpha] p; if z: p = z + (s[i_num]/s[i_den]) p, then s[i_den] = s[i_num]
+ *               (non-finite s[i_alpha]: x and p are left as they are) */
 int uhip_cg_residual_f64(uhip_ctx ctx, int64_t n, const double *r, const double *Ap, double *r_new, float *r_new32,
                          double *scalars, int i_num, int i_den, int i_alpha, int i_rnorm);
 /* x += s[i_alpha] p; p = z + (s[i_num]/s[i_den]) p with z FP64 or FP32 (an FP32 preconditioner
  * output converts exactly); the scalar slots are left as they are (a caller alternating the
- * two r.z slots needs no copy) */
+ * two r.z slots needs no copy); non-finite s[i_alpha]: x and p are left as they are */
 int uhip_cg_xpupdate2(uhip_ctx ctx, int64_t n, double *x, double *p, const void *z, int z_dtype, double *scalars,
                       int i_alpha, int i_num, int i_den);
 int uhip_cg_xpupdate_f64(uhip_ctx ctx, int64_t n, double *x, double *p, const double *z, double *scalars, int i_alpha,
@@ -607,7 +614,7 @@ int uhip_bsolver_solve(uhip_bsolver s, const double *B_d, double *X_d, int64_t *
 int uhip_bsolver_get_conv(uhip_bsolver s, int32_t *conv_it, int32_t *flags);
 /* dL/dA on the pattern of the finest operator from lambda and x (device, [n x k]) */
 int uhip_bsolver_pattern_grad(uhip_bsolver s, const double *lambda_d, const double *x_d, double *grad_d);
-/* setup_seconds solve_seconds iterations graph_replays graph_captures n_levels eig_max_<l>_<s> */
+/* setup_seconds solve_seconds iterations graph_replays graph_captures graph_iterations n_levels eig_max_<l>_<s> */
 int uhip_bsolver_get_info(uhip_bsolver s, const char *key, double *value);
 int uhip_bsolver_set_level_eig(uhip_bsolver s, int l, int32_t sys, double eig_max);
 

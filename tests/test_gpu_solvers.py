@@ -277,9 +277,7 @@ def test_mprgp_256_active_set_matches_oracle(ctx):
     q = ua.MPRGP(ctx, atol=1e-9, rtol=0, max_it=100000)
     q.update(A)
     q.set_box_constraints(lb, None)
-    ua.srand(1)  # the power-method start vector comes from rand(), as in the reference
-    x = q.apply(b)
-    ua.srand(1)
+    x = q.apply(b)  # MPRGP's power method starts from ones (utopia_MPRGP.hpp:356)
     xo, so = orc.mprgp_solve(A, b, lb=lb, atol=1e-9, rtol=0, max_it=100000)
     print(f"mprgp 256^2: gpu it={q.status['iterates']} oracle it={so['iterates']}")
     act = _kkt(A, b, x, lb, 1e-8)

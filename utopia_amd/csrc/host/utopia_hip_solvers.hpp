@@ -241,26 +241,37 @@ private:
         bool stop = false;
         float *r32 = precond_->rhs_f32_buffer(b.size());
         int i_rz = S_RZ, i_rznew = S_RZNEW;  // the two r.z slots swap roles every iteration
+        // alpha and |r_new| come back asynchronously: while the host waits for them the stream
+        // already holds the preconditioner and the x/p update of the same iteration, so the GPU
+        // does not idle on the read-back. Those are enqueued before the convergence test; when it
+        // stops the loop, x += alpha p was what the iteration had left to do (the update kernel
+        // leaves x alone when alpha is not finite) and the preconditioner's output is unused. When
+        // the residual is expected to reach atol in this iteration (its last reduction factor
+        // applied once more) the test comes first instead, which saves that last preconditioner.
+        double r_prev = -1.0, r_prev2 = -1.0;
         while (!stop) {
             hip_check(uhip_spmv_dot(c, &A.raw(), p_.data(), Ap_.data(), s + S_PAP), "Ap, dot(p,Ap)");
             // alpha, r_new = r - alpha Ap, |r_new|; x += alpha p is applied with the p update
             hip_check(uhip_cg_residual_f64(c, b.size(), r_.data(), Ap_.data(), r_new_.data(), r32, s, i_rz, S_PAP,
                                            S_ALPHA, S_RNORM),
                       "cg residual");
+            A.context()->fetch_scalars(s + S_ALPHA, 2);
+            const bool near = r_prev > 0 && r_prev2 > 0 && r_prev * (r_prev / r_prev2) <= 2.0 * this->atol_;
             double h[2];
-            A.context()->read_scalars(s + S_ALPHA, 2, h);
-            const double alpha = h[0];
-            if (std::isinf(alpha) || std::isnan(alpha)) {  // stops before touching x
-                stop = this->check_convergence(it, r_norm, 1, 1);
-                break;
-            }
-            r_norm = h[1];
-            if (r_norm <= this->atol_) {
-                hip_check(uhip_cg_xpupdate_f64(c, b.size(), x.data(), p_.data(), nullptr, s, S_ALPHA, 0, 0),
-                          "x update");
-                this->print_iter(it, r_norm);
-                stop = this->check_convergence(it, r_norm, 1, 1);
-                break;
+            if (near) {
+                A.context()->fetch_wait(2, h);
+                if (std::isinf(h[0]) || std::isnan(h[0])) {  // stops before touching x
+                    stop = this->check_convergence(it, r_norm, 1, 1);
+                    break;
+                }
+                if (h[1] <= this->atol_) {
+                    r_norm = h[1];
+                    hip_check(uhip_cg_xpupdate_f64(c, b.size(), x.data(), p_.data(), nullptr, s, S_ALPHA, 0, 0),
+                              "x update");
+                    this->print_iter(it, r_norm);
+                    stop = this->check_convergence(it, r_norm, 1, 1);
+                    break;
+                }
             }
             // z_new.set(0); precond->apply(r_new, z_new); dot(z_new, r_new)
             const float *z32 = r32 ? precond_->apply_zero_initial_guess_dot_z32(r_new_, r32, s + i_rznew) : nullptr;
@@ -272,6 +283,22 @@ private:
             hip_check(uhip_cg_xpupdate2(c, b.size(), x.data(), p_.data(), z32 ? (const void *)z32 : z_new_.data(),
                                         z32 ? UHIP_F32 : UHIP_F64, s, S_ALPHA, i_rznew, i_rz),
                       "x, p update");
+            if (!near) {
+                A.context()->fetch_wait(2, h);
+                if (std::isinf(h[0]) || std::isnan(h[0])) {  // the update above left x alone
+                    stop = this->check_convergence(it, r_norm, 1, 1);
+                    break;
+                }
+                if (h[1] <= this->atol_) {  // x += alpha p is in the stream
+                    r_norm = h[1];
+                    this->print_iter(it, r_norm);
+                    stop = this->check_convergence(it, r_norm, 1, 1);
+                    break;
+                }
+            }
+            r_norm = h[1];
+            r_prev2 = r_prev;
+            r_prev = r_norm;
             std::swap(r_, r_new_);
             std::swap(z_, z_new_);
             std::swap(i_rz, i_rznew);

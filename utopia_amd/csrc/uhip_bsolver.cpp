@@ -2,6 +2,8 @@
 // batch, in the create / setup / solve / destroy shape of the isolver_lsolve_*
 // plug-out (plug_in_and_out/isolver/utopia_isolver_lsolve.cpp:42-220). The
 // batched solver itself has no reference analogue.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -137,6 +139,7 @@ int uhip_bsolver_get_info(uhip_bsolver s, const char *key, double *value) {
     else if (k == "iterations") *value = (double)b.iterations();
     else if (k == "graph_replays") *value = (double)b.graph_replays();
     else if (k == "graph_captures") *value = (double)b.graph_captures();
+    else if (k == "graph_iterations") *value = std::max(1.0, std::floor(b.param("graph_iterations", 16)));
     else if (k == "n_levels") *value = (double)b.levels();
     else if (k.rfind("eig_max_", 0) == 0) {
         int l = -1, sys = -1;

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpupdate(int64_t n, double *__res
                                                         const TZ *__restrict__ z, const double *__restrict__ s,
                                                         int i_alpha, int i_num, int i_den) {
     const double alpha = s[i_alpha];
+    if (!isfinite(alpha)) return;  // the CG stops before touching x
     const double beta = z ? s[i_num] / s[i_den] : 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double pi = p[i];

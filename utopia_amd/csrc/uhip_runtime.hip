@@ -115,6 +115,10 @@ int uhip_ctx_create(int device, void *stream, uhip_ctx *out) {
         st = set_error(UHIP_ERR_ALLOC, "hipMalloc scalars");
     if (st == UHIP_OK && hipHostMalloc(&c->host_scalars, sizeof(double) * UHIP_NUM_SCALARS) != hipSuccess)
         st = set_error(UHIP_ERR_ALLOC, "hipHostMalloc scalars");
+    if (st == UHIP_OK && hipHostMalloc(&c->host_fetch, sizeof(double) * UHIP_NUM_SCALARS) != hipSuccess)
+        st = set_error(UHIP_ERR_ALLOC, "hipHostMalloc fetch scalars");
+    if (st == UHIP_OK && hipEventCreateWithFlags(&c->fetch_done, hipEventDisableTiming) != hipSuccess)
+        st = set_error(UHIP_ERR_HIP, "hipEventCreate");
     if (st == UHIP_OK && hipMalloc(&c->counters, sizeof(unsigned int) * 64) != hipSuccess)
         st = set_error(UHIP_ERR_ALLOC, "hipMalloc counters");
     if (st == UHIP_OK && hipMemsetAsync(c->dev_scalars, 0, sizeof(double) * UHIP_NUM_SCALARS, c->stream) != hipSuccess)
@@ -137,6 +141,8 @@ int uhip_ctx_destroy(uhip_ctx ctx) {
     if (ctx->partials) hipFree(ctx->partials);
     if (ctx->dev_scalars) hipFree(ctx->dev_scalars);
     if (ctx->host_scalars) hipHostFree(ctx->host_scalars);
+    if (ctx->host_fetch) hipHostFree(ctx->host_fetch);
+    if (ctx->fetch_done) hipEventDestroy(ctx->fetch_done);
     if (ctx->counters) hipFree(ctx->counters);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -219,6 +225,22 @@ int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h) {
     UHIP_CHECK_HIP(hipMemcpyAsync(ctx->host_scalars, src_d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     memcpy(dst_h, ctx->host_scalars, sizeof(double) * n);
+    return UHIP_OK;
+}
+
+int uhip_fetch_scalars(uhip_ctx ctx, const double *src_d, int n) {
+    UHIP_REQUIRE(ctx && src_d, "uhip_fetch_scalars: null argument");
+    UHIP_REQUIRE(n > 0 && n <= UHIP_NUM_SCALARS, "uhip_fetch_scalars: n=%d out of range", n);
+    UHIP_CHECK_HIP(hipMemcpyAsync(ctx->host_fetch, src_d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    UHIP_CHECK_HIP(hipEventRecord(ctx->fetch_done, ctx->stream));
+    return UHIP_OK;
+}
+
+int uhip_fetch_wait(uhip_ctx ctx, int n, double *dst_h) {
+    UHIP_REQUIRE(ctx && dst_h, "uhip_fetch_wait: null argument");
+    UHIP_REQUIRE(n > 0 && n <= UHIP_NUM_SCALARS, "uhip_fetch_wait: n=%d out of range", n);
+    UHIP_CHECK_HIP(hipEventSynchronize(ctx->fetch_done));
+    memcpy(dst_h, ctx->host_fetch, sizeof(double) * n);
     return UHIP_OK;
 }
 

@@ -720,14 +720,22 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
             return A->dtype == UHIP_F64 ? spmv_typed<double, double>(ctx, A, x, y) : spmv_typed<float, float>(ctx, A, x, y);
         };
         if ((st = run()) != UHIP_OK) break;
-        hipEventRecord(e0, ctx->stream);
-        for (int r = 0; r < reps && st == UHIP_OK; ++r) st = run();
-        hipEventRecord(e1, ctx->stream);
-        hipEventSynchronize(e1);
+        // Each launch timed on its own, median of reps: in a solve every product waits for the
+        // previous kernel, so a launch's own duration is what counts. Back-to-back launches of
+        // small kernels all time at the dispatch rate, which ranks candidates at random.
+        std::vector<float> t;
+        for (int r = 0; r < reps && st == UHIP_OK; ++r) {
+            hipEventRecord(e0, ctx->stream);
+            st = run();
+            hipEventRecord(e1, ctx->stream);
+            hipEventSynchronize(e1);
+            float one = 0.f;
+            hipEventElapsedTime(&one, e0, e1);
+            t.push_back(one);
+        }
         if (st != UHIP_OK) break;
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
-        ms /= (float)reps;
+        std::sort(t.begin(), t.end());
+        const float ms = t[t.size() / 2];
         if (ms < best) {
             best = ms;
             win = c;
