@@ -496,6 +496,9 @@ def main(argv=None):
                     help="reps > 0: every SpMV plan of the hierarchy is chosen by timing (uhip_csr_autotune)")
     ap.add_argument("--cpu-reps", type=int, default=1,
                     help="oracle solves at the full size timed for the CPU baseline (median reported)")
+    ap.add_argument("--solve-only", choices=["fp64", "mixed"],
+                    help="time the solves of one form and stop (no kernel table, no CPU baseline): for "
+                         "kernel-trace runs that attribute a solve's time to its kernels")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the SpMV / streaming kernel timings (for counter collection runs)")
     ap.add_argument("--workload", choices=["mgpcg256", "batched", "mprgp1024", "shard"], default="mgpcg256",
@@ -526,6 +529,7 @@ def main(argv=None):
     log(f"generated {n}^3 variable-coefficient operator: N={N} nnz={nnz} in {time.perf_counter() - t0:.1f} s")
     if a.kernels_only:
         kern = kernel_rooflines(ua, ctx, A, N)
+        kern.update(solve_kernels(ua, ctx, A, interps, N))
         kern.update(generic_csr_kernels(ua, ctx, n, a.seed))
         for k, v in kern.items():
             log(f"kernel {k}: {v['ms']:.3f} ms, {v['gbs']:.0f} GB/s, {v['bytes']} algorithmic bytes, {v.get('label', '')}")
@@ -536,7 +540,10 @@ def main(argv=None):
     xd = ua.DeviceArray(ctx, N)
 
     results = {}
-    for form, kind in (("fp64", "multigrid"), ("mixed", "multigrid_mixed")):
+    forms = (("fp64", "multigrid"), ("mixed", "multigrid_mixed"))
+    if a.solve_only:
+        forms = tuple(f for f in forms if f[0] == a.solve_only)
+    for form, kind in forms:
         ua.srand(1)
         cg = ua.ConjugateGradient(ctx, kind, atol=atol, rtol=0, stol=0, max_it=1000, pre_smoothing_steps=2,
                                   post_smoothing_steps=2, autotune=a.autotune)
@@ -569,6 +576,11 @@ def main(argv=None):
                          "rel_residual": res / float(np.linalg.norm(b)), "x": x}
         del cg
 
+    if a.solve_only:
+        r = results[a.solve_only]
+        r.pop("x")
+        print(json.dumps({"solve_only": a.solve_only, **r}))
+        return 0
     x64 = results["fp64"].pop("x")
     xmx = results["mixed"].pop("x")
     err_mixed = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
