@@ -204,7 +204,8 @@ typedef struct uhip_csr {
     int32_t nparts;     /* workgroups of one SpMV launch (partial sums of fused dots) */
     int32_t ndiag;      /* DIA: distinct column offsets (col - row), <= 32 */
     int32_t *dia_off;   /* DIA: ndiag offsets, ascending (device) */
-    void *dia_val;      /* DIA: ndiag x ld values (ld = nrows rounded up to 64), diagonal-major, zero where absent */
+    void *dia_val;      /* DIA: ndiag x ld values (ld = nrows rounded up to 64), diagonal-major, zero (or NaN,
+                         * see dia_nan) where absent */
     void *dia_mask;     /* DIA: per-row bit d set iff the row stores offset d (uint8 / uint16 / uint32
                          * for ndiag <= 8 / 16 / 32) */
     int32_t dia_sym;    /* DIA of a bitwise-symmetric matrix: dia_val holds only the offsets >= 0

@@ -167,7 +167,7 @@ def test_spmv_dia_bit_exact(ctx, name, dt):
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.format == ua.FMT_DIA
     if name in DIA_SYMMETRIC:
-        # 2: plane-blocked storage (in FP64 the plain product runs the symmetric kernel on it)
+        # 2: the plane-walking 7-point layout, which every product of either precision runs
         assert D.m.dia_sym == DIA_SYMMETRIC[name]
     x = ua.gen_uniform(17, n)
     if dt is np.float32:
@@ -222,19 +222,21 @@ def test_dia_rejects_unsorted_and_rectangular(ctx):
 
 
 @pytest.mark.parametrize("fmt,dims", [(ua.FMT_ROWBLOCK, (20, 18, 13)), (ua.FMT_SELL64, (20, 18, 13)),
-                                      (ua.FMT_DIA, (20, 18, 13)), (ua.FMT_DIA, (16, 16, 13))])
+                                      (ua.FMT_DIA, (20, 18, 13)), (ua.FMT_DIA, (16, 16, 13)),
+                                      (ua.FMT_DIA, (256, 256, 21))])
 def test_fused_epilogues_both_formats_with_dot(ctx, fmt, dims):
+    """(256, 256, 21): the benchmark's plane size (S2 = 65,536) with 2 full 8-plane slabs and a
+    partial one in FP64, 5 full 4-plane slabs and a partial one in FP32."""
     A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, *dims, seed=9)
     n = A.shape[0]
     x, b = ua.gen_uniform(1, n), ua.gen_uniform(2, n)
     Ax = orc.spmv(A, x)
     D = ua.DeviceCsr(ctx, A, fmt=fmt)
     D32 = ua.DeviceCsr(ctx, A, dtype=np.float32, fmt=fmt)
-    if dims == (16, 16, 13):
-        # plane-blocked 7-point kernel, with a partial last slab
-        # plane-blocked storage in both precisions: FP64 runs the dot products with the
-        # plane-blocked kernel and the other epilogues with the symmetric one
-        assert D32.m.dia_sym == 2 and D.m.dia_sym == 2
+    if fmt == ua.FMT_DIA and dims[0] * dims[1] % 64 == 0:
+        # the plane-walking 7-point kernel runs every epilogue in both precisions, here with
+        # a partial last slab
+        assert D32.m.dia_sym == 2 and D.m.dia_sym == 2 and D.m.dia_nan == 1
     x32, b32 = x.astype(np.float32), b.astype(np.float32)
     x32d, b32d = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray.from_numpy(ctx, b32)
     out32 = ua.DeviceArray(ctx, n, np.float32)
@@ -453,6 +455,33 @@ def test_dense_cholesky_and_inverse(ctx, n):
     xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, n)
     ua.check(L().uhip_dense_gemv(ctx.h, n, P(Inv), P(xd), ua.F64, P(yd), ua.F64))
     assert np.abs(yd.numpy() - Ainv @ x).max() <= 1e-13 * np.abs(Ainv).sum(1).max() * np.abs(x).max()
+
+
+def test_dense_cholesky_and_inverse_at_the_coarse_size(ctx):
+    """n = 4096, the coarsest operator of the 256^3 hierarchy: here the 16^3 variable-coefficient
+    operator made dense (condition ~1e3). Backward error of the factor, and the inverse against
+    the identity within the conditioning bound; the GEMV against numpy."""
+    import scipy.sparse.linalg as spla
+    S = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 16, 16, seed=3).to_scipy()
+    n = S.shape[0]
+    A = S.toarray()
+    lmax = spla.eigsh(S, k=1, which="LA", return_eigenvectors=False)[0]
+    lmin = spla.eigsh(S.tocsc(), k=1, sigma=0, which="LM", return_eigenvectors=False)[0]
+    Ad = ua.DeviceArray.from_numpy(ctx, A.ravel())
+    info = ctypes.c_int()
+    ua.check(L().uhip_dense_cholesky(ctx.h, n, P(Ad), n, ctypes.byref(info)))
+    assert info.value == 0
+    Lg = np.tril(Ad.numpy().reshape(n, n))
+    assert np.linalg.norm(Lg @ Lg.T - A) <= 1e-14 * np.linalg.norm(A)
+    Inv = ua.DeviceArray(ctx, n * n)
+    ua.check(L().uhip_dense_cholesky_inverse(ctx.h, n, P(Ad), n, P(Inv)))
+    Ainv = Inv.numpy().reshape(n, n)
+    assert np.abs(Ainv - Ainv.T).max() <= 1e-14 * np.abs(Ainv).max() * (lmax / lmin)
+    assert np.abs(Ainv @ A - np.eye(n)).max() <= 1e-15 * n * (lmax / lmin)
+    x = ua.gen_uniform(9, n)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, n)
+    ua.check(L().uhip_dense_gemv(ctx.h, n, P(Inv), P(xd), ua.F64, P(yd), ua.F64))
+    assert np.abs(yd.numpy() - Ainv @ x).max() <= 1e-14 * np.abs(Ainv).sum(1).max() * np.abs(x).max()
 
 
 def test_dense_cholesky_reports_indefinite(ctx):
