@@ -158,11 +158,11 @@ class ChebStep(ctypes.Structure):
 
 
 def solve_kernels(ua, ctx, A, interps, n_rows, reps=20):
-    """Average launch time and algorithmic bytes of the fused fine-level launches one CG
-    iteration of the 256^3 solve makes (V(2,2) Chebyshev from a zero guess: pre-smoothing is one
-    last-step launch, then r_R = r - A c, the restriction, the interpolation-add, p0 = r - A c and
-    the last smoothing step with z.r fused), in FP64 (the FP64 form) and FP32 (the mixed form),
-    plus the CG's A p with p.Ap (FP64 in both forms)."""
+    """Average launch time and algorithmic bytes of the fine-level launches of one CG iteration
+    of the 256^3 solve (V(2,2) Chebyshev from a zero guess), in FP64 (the FP64 form) and FP32 (the
+    mixed form): the fused smoothing pairs the solve runs (the pre-smoothing step with r_R = r - A c;
+    p0 = r - A c with the last smoothing step and z.r), the single-product launches they replace,
+    the restriction and the interpolation-add, and the CG's A p with p.Ap (FP64 in both forms)."""
     L = ua.lib()
     out = {}
     N = n_rows
@@ -201,12 +201,23 @@ def solve_kernels(ua, ctx, A, interps, n_rows, reps=20):
         if dt is np.float64:
             forms.append(("cg_Ap_dot", "EpiStoreDot<double> A p with p.Ap",
                           lambda: L.uhip_spmv_dot(ctx.h, D.ref, v0, v1, ctypes.c_void_p(dd.ptr)), base))
+        if L.uhip_cheb_pair_supported(D.ref):
+            # the fused smoothing pairs the V-cycle runs on this level: values once, b in, c and
+            # r_R out; x, b in, z out and r (FP64) for z.r
+            forms.append(("pre_pair", f"k_dia7_pair<{ty}> last step from a zero guess, then r_R = b - A c",
+                          lambda: L.uhip_cheb_pre_residual(ctx.h, D.ref, v0, 0.3, 0.1, 0.2, v1, v2),
+                          base + vb * N))
+            forms.append(("post_pair", f"k_dia7_pair<{ty}, dot> p0 = b - A c, then the last step with z.r",
+                          lambda: L.uhip_cheb_residual_post(ctx.h, D.ref, v0, v1, 0.3, 0.1, 0.2, v3,
+                                                            ctypes.c_void_p(r64.ptr), ctypes.c_void_p(dd.ptr)),
+                          base + vb * N + 8 * N))
         for key, label, launch, nbytes in forms:
             def run(launch=launch):
                 ua.check(launch())
             ms = time_kernel(ua, ctx, run, reps)
+            kname = "" if "pair" in key else f" ({'k_spmv_dia7r' if D.m.dia_sym == 2 else 'format ' + str(D.format)})"
             out[f"{key}_{tag}"] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
-                                   "label": f"{label} ({'k_spmv_dia7r' if D.m.dia_sym == 2 else 'format ' + str(D.format)})"}
+                                   "label": f"{label}{kname}"}
         del D
     xs = vec[np.float32]
     ms = time_kernel(ua, ctx, lambda: ua.check(L.uhip_spmv_masked(ctx.h, R.ref, ctypes.c_void_p(xs[0].ptr),
@@ -599,13 +610,16 @@ def main(argv=None):
         cpu = cpu_baseline(n, a.levels, a.seed, results["mixed"]["iterations"], x64, xmx, reps=a.cpu_reps)
         log(f"cpu baseline ({time.perf_counter() - t0:.1f} s): {cpu['value']:.1f} s, {cpu['sample']}")
 
-    # The roofline line is for the launch with the largest share of a mixed CG iteration: the
-    # CG's A p with p.Ap (FP64, one launch per iteration, shared with the FP64 form); the FP32
-    # V-cycle launches and the plain products are in "kernels". HBM bytes per launch come from
+    # The roofline line is for the launch with the largest share of a mixed CG iteration (the
+    # CG's A p with p.Ap, the fused smoothing pairs of the fine level, the transfers); the other
+    # launch forms and the plain products are in "kernels". HBM bytes per launch come from
     # the committed counter runs (FETCH_SIZE / WRITE_SIZE), used only when they were taken on
     # the kernel measured here.
-    per_iteration = {"cg_Ap_dot_f64": 1, "residual_f32": 2, "pre_smooth_f32": 1, "post_smooth_dot_f32": 1,
-                     "restrict_f32": 1, "interpolate_f32": 1}
+    per_iteration = {"cg_Ap_dot_f64": 1, "pre_pair_f32": 1, "post_pair_f32": 1, "restrict_f32": 1,
+                     "interpolate_f32": 1}
+    if "pre_pair_f32" not in kern:  # planes that do not split into whole chunks: one launch per product
+        per_iteration = {"cg_Ap_dot_f64": 1, "residual_f32": 2, "pre_smooth_f32": 1, "post_smooth_dot_f32": 1,
+                         "restrict_f32": 1, "interpolate_f32": 1}
     rk = max(per_iteration, key=lambda k: kern[k]["ms"] * per_iteration[k])
     traffic = {}
     tf = ROOT / "profiles" / "spmv_varcoef256_traffic.json"

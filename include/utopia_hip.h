@@ -220,6 +220,7 @@ typedef struct uhip_csr {
     struct uhip_csr *tr; /* A^T with its own plan (uhip_csr_plan_transpose), freed with A's plan */
     int32_t dia_nan;    /* DIA: 1 when the matrix holds no NaN; the entries a row does not hold are
                          * then NaN in dia_val (else zero) and kernels may select on the value */
+    int32_t dia_s1, dia_s2; /* dia_sym 2: the offsets S1 and S2 (host copies of dia_off[5], dia_off[6]) */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)
@@ -335,6 +336,28 @@ typedef struct uhip_cheb_step {
     double *dot_d;
 } uhip_cheb_step;
 int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *step);
+/* Two Chebyshev products in one pass over the matrix (temporal blocking of a smoothing pair, in
+ * A's precision for every vector; dot_with FP64). A must be a plane-walking 7-point operator
+ * (dia_sym 2, dia_nan 1) whose planes split into whole chunks of 512 rows, with S1 <= 256 and
+ * nrows a multiple of S2; uhip_cheb_pair_supported says whether it is, and the other entry
+ * points return UHIP_ERR_UNSUPPORTED when it is not (the caller then runs the two launches).
+ *  uhip_cheb_pre_residual: the last step of a two-step smoothing from a zero guess, c = a_prev b +
+ *      a_cur (-(-b + a_prev A b) + beta b), written to c, then r_R = b - A c written to rR: the
+ *      values of uhip_cheb_step_apply (pprev = b, no rprev, no xin, last) followed by
+ *      uhip_spmv_residual, bit for bit.
+ *  uhip_cheb_residual_post: pa = b - A x (not stored), then the last step with pprev = pa,
+ *      xin = x: out = x + a_prev pa + a_cur (-(-pa + a_prev A pa) + beta pa), and dot_d[0] =
+ *      out . dot_with when given: the values of uhip_spmv_residual followed by
+ *      uhip_cheb_step_apply, bit for bit except the dot, summed in another order.
+ * The reference applies the smoother's products one at a time (ChebyshevSmoother::smooth,
+ * solvers/smoothers/utopia_ChebyshevSmoother.hpp); these fuse the two that meet where the
+ * pre-smoothing ends and the post-smoothing starts (Multigrid::standard_cycle,
+ * utopia_Multigrid.hpp:280-300, 346-360). */
+int uhip_cheb_pair_supported(const uhip_csr *A);
+int uhip_cheb_pre_residual(uhip_ctx ctx, const uhip_csr *A, const void *b, double a_prev, double beta,
+                           double a_cur, void *c, void *rR);
+int uhip_cheb_residual_post(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *b, double a_prev,
+                            double beta, double a_cur, void *out, const double *dot_with, double *dot_d);
 
 /* Diagonal:  d_i = A_ii (0 if absent)   MatGetDiagonal (utopia_petsc_Matrix.cpp:744-756) */
 int uhip_csr_diag(uhip_ctx ctx, const uhip_csr *A, void *d);
@@ -485,7 +508,9 @@ int uhip_solver_destroy(uhip_solver s);
 /* read(Input&)-style parameters: atol rtol stol max_it verbose reset_initial_guess refine
  * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
  * cycle_type (1 MULTIPLICATIVE_CYCLE, 3 FULL_CYCLE; utopia_MultiLevelBase.hpp:12-14) v_cycle_repetition
- * use_line_search (utopia_Multigrid.hpp:324-341)
+ * use_line_search (utopia_Multigrid.hpp:324-341) fused_smoothing (default 1: on plane-walking 7-point
+ * levels the pre-smoothing's last step and the residual, and the post-smoothing's residual and step,
+ * run as one pass each, uhip_cheb_pre_residual / uhip_cheb_residual_post; 0: one launch per product)
  * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour, 3 PointJacobi) scale_max_eig scale_min_eig power_method_tol
  * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each coarse_space_it
  * active_set_tol; AlgebraicMultigrid: n_levels bmax weight block_size (2-4: BlockAgglomerate,

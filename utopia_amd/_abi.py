@@ -51,6 +51,8 @@ class uhip_csr(ctypes.Structure):
         ("vi_nvals", ctypes.c_int32),
         ("tr", ctypes.c_void_p),
         ("dia_nan", ctypes.c_int32),
+        ("dia_s1", ctypes.c_int32),
+        ("dia_s2", ctypes.c_int32),
     ]
 
 

@@ -283,6 +283,7 @@ struct MGLevel {
     std::vector<double> alpha, beta;            // Chebyshev coefficients for pre / post
     std::vector<double> alpha_post, beta_post;
     double eig_max = 0, eig_min = 0;
+    bool pair = false;  // Chebyshev smoothing pairs fused with their residuals (uhip_cheb_pre_residual)
 };
 
 enum MGSmoother { MG_CHEBYSHEV = 0, MG_JACOBI = 1, MG_GAUSS_SEIDEL = 2, MG_POINT_JACOBI = 3 };
@@ -297,12 +298,14 @@ public:
     const Factorization *coarse = nullptr;
     int pre = 3, post = 3, mg_type = 1, smoother = MG_CHEBYSHEV;
     bool masks = true, gs_symmetric = true, line_search = false;
+    bool fuse_pairs = true;  // levels whose operator allows it run each smoothing pair in one pass
 
     void allocate_work() {
         for (size_t l = 0; l < levels.size(); ++l) {
             auto &L = levels[l];
             const int64_t n = L.A->rows();
             auto ctx = L.A->context();
+            L.pair = fuse_pairs && smoother == MG_CHEBYSHEV && l > 0 && uhip_cheb_pair_supported(&L.A->raw());
             for (HipVectorT<T> *v : {&L.r, &L.c, &L.rR, &L.rk, &L.pa, &L.pb}) *v = HipVectorT<T>(ctx, n, T(0));
             if (line_search && l > 0) L.cI = HipVectorT<T>(ctx, n, T(0));
             if (smoother == MG_CHEBYSHEV && l > 0) {
@@ -332,9 +335,16 @@ public:
         for (int k = 0; k < mg_type; ++k) {
             const bool first = (k == 0);
             const bool final_pass = (k == mg_type - 1);
-            smooth(L, r, r_dt, /*zero_guess=*/first, L.c.data(), dt, pre, L.alpha, L.beta);
-            // r_R = r - A c
-            hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), r, r_dt, L.rR.data()), "mg residual");
+            if (L.pair && first && pre == 2 && r_dt == dt) {
+                // the two-step smoothing from c = 0 and r_R = r - A c in one pass
+                hip_check(uhip_cheb_pre_residual(c, &L.A->raw(), r, L.alpha[0], L.beta[1], L.alpha[1], L.c.data(),
+                                                 L.rR.data()),
+                          "mg pre-smoothing and residual");
+            } else {
+                smooth(L, r, r_dt, /*zero_guess=*/first, L.c.data(), dt, pre, L.alpha, L.beta);
+                // r_R = r - A c
+                hip_check(uhip_spmv_residual(c, &L.A->raw(), L.c.data(), r, r_dt, L.rR.data()), "mg residual");
+            }
             // r_{l-1} = mask .* (R r_R)
             hip_check(uhip_spmv_masked(c, &L.R->raw(), L.rR.data(), masks && !Lc.mask.empty() ? Lc.mask.data() : nullptr,
                                        Lc.r.data()),
@@ -345,8 +355,17 @@ public:
             else hip_check(uhip_spmv_add(c, &L.P->raw(), Lc.c.data(), L.c.data()), "mg interpolate");
             void *o = final_pass ? out : (void *)L.c.data();
             const int odt = final_pass ? out_dt : dt;
-            dot_done = smooth(L, r, r_dt, false, o, odt, post, L.alpha_post, L.beta_post,
-                              final_pass ? dot_with : nullptr, dot_d);
+            if (L.pair && post == 2 && r_dt == dt && odt == dt && o != (void *)L.c.data() && o != r) {
+                // p0 = r - A c and the last smoothing step on it in one pass
+                const double *dw = final_pass ? dot_with : nullptr;
+                hip_check(uhip_cheb_residual_post(c, &L.A->raw(), L.c.data(), r, L.alpha_post[0], L.beta_post[1],
+                                                  L.alpha_post[1], o, dw, dot_d),
+                          "mg residual and post-smoothing");
+                dot_done = dw != nullptr;
+            } else {
+                dot_done = smooth(L, r, r_dt, false, o, odt, post, L.alpha_post, L.beta_post,
+                                  final_pass ? dot_with : nullptr, dot_d);
+            }
         }
         (void)n;
         return dot_done;
@@ -540,6 +559,16 @@ public:
         }
     }
     bool use_line_search() const { return line_search_; }
+    // the pairs of products where the pre-smoothing ends and the post-smoothing starts run in one
+    // pass each on operators that allow it (uhip_cheb_pair_supported); off: one launch per product
+    void fused_smoothing(bool v) {
+        fused_ = v;
+        if (eng_.fuse_pairs != v && !eng_.levels.empty()) {
+            eng_.fuse_pairs = v;
+            eng_.allocate_work();
+        }
+    }
+    bool fused_smoothing() const { return fused_; }
     void must_generate_masks(bool v) { masks_ = v; }
     bool must_generate_masks() const { return masks_; }
     void fix_semidefinite_operators(bool v) { fix_semi_ = v; }
@@ -626,6 +655,7 @@ public:
         eng_.smoother = cheb ? MG_CHEBYSHEV : (gs ? MG_GAUSS_SEIDEL : (pj ? MG_POINT_JACOBI : MG_JACOBI));
         eng_.gs_symmetric = gs ? gs->symmetric() : true;
         eng_.line_search = line_search_;
+        eng_.fuse_pairs = fused_;
         for (int l = 0; l < L; ++l) {
             auto &lv = eng_.levels[(size_t)l];
             lv.A = A_[(size_t)l];
@@ -720,7 +750,7 @@ private:
     std::vector<HipVector> mask_;
     int n_levels_ = 0, pre_ = 3, post_ = 3, mg_type_ = 1, cycle_type_ = MG_MULTIPLICATIVE_CYCLE;
     int64_t v_cycle_repetition_ = 10;
-    bool masks_ = true, fix_semi_ = false, galerkin_ = true, line_search_ = false;
+    bool masks_ = true, fix_semi_ = false, galerkin_ = true, line_search_ = false, fused_ = true;
     VCycle<double> eng_;
     HipVector r_top_, c_top_;
     double setup_s_ = 0;
@@ -748,6 +778,7 @@ public:
         eng_.masks = src.masks;
         eng_.smoother = src.smoother;
         eng_.line_search = src.line_search;
+        eng_.fuse_pairs = src.fuse_pairs;
         eng_.coarse = src.coarse;
         if (eng_.smoother == MG_GAUSS_SEIDEL)
             throw HipError(UHIP_ERR_UNSUPPORTED, "mixed precision: Chebyshev, Jacobi or PointJacobi smoothing");

@@ -392,6 +392,8 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     A->ndiag = nd;
     A->dia_sym = sym;
     A->dia_nan = has_nan ? 0 : 1;
+    A->dia_s1 = sym == 2 ? offs[5] : 0;
+    A->dia_s2 = sym == 2 ? offs[6] : 0;
     A->format = UHIP_FMT_DIA;
     A->nparts = sym == 2 ? dia7z_workgroups(n, offs[6], planes)
                          : (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
@@ -880,6 +882,7 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     A->ndiag = 0;
     A->dia_sym = 0;
     A->dia_nan = 0;
+    A->dia_s1 = A->dia_s2 = 0;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));
