@@ -21,13 +21,11 @@
 // the plane) meet only dropped products.
 #include "uhip_spmv.hpp"
 
-#ifndef UHIP_PAIR_PLANES
-#define UHIP_PAIR_PLANES 32
-#endif
 
 namespace uhip {
 
 constexpr int kPairMaxS1 = 256;  // y halo rows on either side (one halo row per thread: 2 S1 <= NT)
+constexpr int kPairPlanes = 32;  // planes per workgroup (measured: 8 and 16 are slower at 256^3)
 
 // threads per workgroup and chunk rows per thread: FP32 1024 x 2, FP64 512 x 2 (three planes of a
 // row's values live in registers)
@@ -85,8 +83,108 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
             xcur[j] = x1[r];
         }
     }
-    // a chunk row's seven values and x1 at plane f - 2 (hv, hx: second product now), f - 1
-    // (nv, nx) and f (fv, fx: being loaded); dw at plane f - 2 (hd, loaded a step earlier)
+    double part = 0.0;
+    struct Ld {
+        T a0, m1, a1, aS1, mS1, aS2, xm1, xp1, xmS1, xpS1, xn, b;
+    };
+    // the first product's loads of plane f
+    auto issue = [&](int f, Ld(&L)[NS]) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (!on[j]) continue;
+            const int32_t r = f * S2 + off[j];
+            L[j].a0 = ld_stream(v0 + r);
+            const typename PairOf<T>::t w =
+                __builtin_nontemporal_load((const typename PairOf<T>::t *)(v1 + max(r - 1, 0)));
+            L[j].m1 = r >= 1 ? w.x : nan;
+            L[j].a1 = r >= 1 ? w.y : w.x;
+            L[j].aS1 = vS1[r];
+            const T ms = vS1[max(r - S1, 0)];
+            L[j].mS1 = r >= S1 ? ms : nan;
+            L[j].aS2 = ld_stream(vS2 + r);
+            L[j].xm1 = x1[cl(r - 1)];
+            L[j].xp1 = x1[cl(r + 1)];
+            L[j].xmS1 = x1[cl(r - S1)];
+            L[j].xpS1 = x1[cl(r + S1)];
+            L[j].xn = x1[cl(r + S2)];
+            if constexpr (MODE == 1) L[j].b = ld_stream(b + r);
+        }
+    };
+    // the first product of plane f into its ring slot; the chunk rows' values and x1 to fv, fx
+    auto first = [&](int f, T *slot, const Ld(&L)[NS], T(&fv)[CORE][7], T(&fx)[CORE]) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (!on[j]) continue;
+            const T vals[7] = {prev_a2[j], L[j].mS1, L[j].m1, L[j].a0, L[j].a1, L[j].aS1, L[j].aS2};
+            const T xs[7] = {xprev[j], L[j].xmS1, L[j].xm1, xcur[j], L[j].xp1, L[j].xpS1, L[j].xn};
+            T acc = T(0);
+#pragma unroll
+            for (int d = 0; d < 7; ++d) {
+                const T pr = vals[d] * xs[d];
+                acc = (vals[d] == vals[d]) ? acc + pr : acc;
+            }
+            T v;
+            if constexpr (MODE == 0) {
+                const T pp = xcur[j];
+                const T rk = -pp + a_prev * acc;
+                const T pk = -rk + beta * pp;
+                T xv = T(0);
+                xv = xv + a_prev * pp;
+                xv = xv + a_cur * pk;
+                v = xv;
+            } else {
+                v = L[j].b - acc;
+            }
+            slot[off[j] - base] = v;
+            if (j < CORE) {
+                if constexpr (MODE == 0)
+                    if (f >= zs && f < ze) st_stream(out1 + f * S2 + off[j], v);
+#pragma unroll
+                for (int d = 0; d < 7; ++d) fv[j][d] = vals[d];
+                fx[j] = xcur[j];
+            }
+            prev_a2[j] = L[j].aS2;
+            xprev[j] = xcur[j];
+            xcur[j] = L[j].xn;
+        }
+    };
+    // the second product of plane p from the ring slots of planes p - 1, p, p + 1
+    auto second = [&](int p, const T *cm, const T *c0, const T *cp, const T(&hv)[CORE][7], const T(&hx)[CORE],
+                      const double(&hd)[CORE]) {
+#pragma unroll
+        for (int kk = 0; kk < CORE; ++kk) {
+            const int32_t r = p * S2 + off[kk];
+            const int i = off[kk] - base;
+            const T xs[7] = {cm[i], c0[i - S1], c0[i - 1], c0[i], c0[i + 1], c0[i + S1], cp[i]};
+            T acc = T(0);
+#pragma unroll
+            for (int d = 0; d < 7; ++d) {
+                const T pr = hv[kk][d] * xs[d];
+                acc = (hv[kk][d] == hv[kk][d]) ? acc + pr : acc;
+            }
+            if constexpr (MODE == 0) {
+                st_stream(out2 + r, (T)(hx[kk] - acc));
+            } else {
+                const T pp = c0[i];
+                const T rk = -pp + a_prev * acc;
+                const T pk = -rk + beta * pp;
+                T xv = hx[kk];
+                xv = xv + a_prev * pp;
+                xv = xv + a_cur * pk;
+                st_stream(out2 + r, xv);
+                if constexpr (DOT) part += (double)xv * hd[kk];
+            }
+        }
+    };
+    auto dw_of = [&](int p, double(&d)[CORE]) {
+#pragma unroll
+        for (int kk = 0; kk < CORE; ++kk) {
+            d[kk] = 0.0;
+            if constexpr (DOT)
+                if (p >= zs && p < ze) d[kk] = ld_stream(dw + p * S2 + off[kk]);
+        }
+    };
+    // the first product runs two planes ahead of the second; 4 ring slots, 1 barrier per plane
     T hv[CORE][7], hx[CORE], nv[CORE][7], nx[CORE];
     double hd[CORE];
 #pragma unroll
@@ -96,74 +194,14 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
 #pragma unroll
         for (int d = 0; d < 7; ++d) hv[k][d] = nv[k][d] = nan;
     }
-    double part = 0.0;
-
-    struct Ld {
-        T a0, m1, a1, aS1, mS1, aS2, xm1, xp1, xmS1, xpS1, xn, b;
-    };
     const int kend = ze + 1 - q0;  // the step whose second product is plane ze - 1
     for (int k = 0; k <= kend; ++k) {
         const int f = q0 + k, p = f - 2;
-        const bool do1 = f <= q1, do2 = p >= zs && p < ze;
-        // loads of the first product at plane f, and of dw at plane p + 1 for the next step
         Ld L[NS];
+        if (f <= q1) issue(f, L);
         double fd[CORE];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            if (do1 && on[j]) {
-                const int32_t r = f * S2 + off[j];
-                L[j].a0 = ld_stream(v0 + r);
-                const typename PairOf<T>::t w =
-                    __builtin_nontemporal_load((const typename PairOf<T>::t *)(v1 + max(r - 1, 0)));
-                L[j].m1 = r >= 1 ? w.x : nan;
-                L[j].a1 = r >= 1 ? w.y : w.x;
-                L[j].aS1 = vS1[r];
-                const T ms = vS1[max(r - S1, 0)];
-                L[j].mS1 = r >= S1 ? ms : nan;
-                L[j].aS2 = ld_stream(vS2 + r);
-                L[j].xm1 = x1[cl(r - 1)];
-                L[j].xp1 = x1[cl(r + 1)];
-                L[j].xmS1 = x1[cl(r - S1)];
-                L[j].xpS1 = x1[cl(r + S1)];
-                L[j].xn = x1[cl(r + S2)];
-                if constexpr (MODE == 1) L[j].b = ld_stream(b + r);
-            }
-        }
-#pragma unroll
-        for (int kk = 0; kk < CORE; ++kk) {
-            fd[kk] = 0.0;
-            if constexpr (DOT)
-                if (p + 1 >= zs && p + 1 < ze) fd[kk] = ld_stream(dw + (p + 1) * S2 + off[kk]);
-        }
-        // second product of plane p from the ring slots of planes p - 1, p, p + 1
-        if (do2) {
-            const T *cm = ring[(p + 3) & 3], *c0 = ring[p & 3], *cp = ring[(p + 1) & 3];
-#pragma unroll
-            for (int kk = 0; kk < CORE; ++kk) {
-                const int32_t r = p * S2 + off[kk];
-                const int i = off[kk] - base;
-                const T xs[7] = {cm[i], c0[i - S1], c0[i - 1], c0[i], c0[i + 1], c0[i + S1], cp[i]};
-                T acc = T(0);
-#pragma unroll
-                for (int d = 0; d < 7; ++d) {
-                    const T pr = hv[kk][d] * xs[d];
-                    acc = (hv[kk][d] == hv[kk][d]) ? acc + pr : acc;
-                }
-                if constexpr (MODE == 0) {
-                    st_stream(out2 + r, (T)(hx[kk] - acc));
-                } else {
-                    const T pp = c0[i];
-                    const T rk = -pp + a_prev * acc;
-                    const T pk = -rk + beta * pp;
-                    T xv = hx[kk];
-                    xv = xv + a_prev * pp;
-                    xv = xv + a_cur * pk;
-                    st_stream(out2 + r, xv);
-                    if constexpr (DOT) part += (double)xv * hd[kk];
-                }
-            }
-        }
-        // first product of plane f into its ring slot
+        dw_of(p + 1, fd);
+        if (p >= zs && p < ze) second(p, ring[(p + 3) & 3], ring[p & 3], ring[(p + 1) & 3], hv, hx, hd);
         T fv[CORE][7], fx[CORE];
 #pragma unroll
         for (int kk = 0; kk < CORE; ++kk) {
@@ -171,44 +209,7 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
 #pragma unroll
             for (int d = 0; d < 7; ++d) fv[kk][d] = nan;
         }
-        if (do1) {
-            T *slot = ring[f & 3];
-#pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                if (!on[j]) continue;
-                const T vals[7] = {prev_a2[j], L[j].mS1, L[j].m1, L[j].a0, L[j].a1, L[j].aS1, L[j].aS2};
-                const T xs[7] = {xprev[j], L[j].xmS1, L[j].xm1, xcur[j], L[j].xp1, L[j].xpS1, L[j].xn};
-                T acc = T(0);
-#pragma unroll
-                for (int d = 0; d < 7; ++d) {
-                    const T pr = vals[d] * xs[d];
-                    acc = (vals[d] == vals[d]) ? acc + pr : acc;
-                }
-                T v;
-                if constexpr (MODE == 0) {
-                    const T pp = xcur[j];
-                    const T rk = -pp + a_prev * acc;
-                    const T pk = -rk + beta * pp;
-                    T xv = T(0);
-                    xv = xv + a_prev * pp;
-                    xv = xv + a_cur * pk;
-                    v = xv;
-                } else {
-                    v = L[j].b - acc;
-                }
-                slot[off[j] - base] = v;
-                if (j < CORE) {
-                    if constexpr (MODE == 0)
-                        if (f >= zs && f < ze) st_stream(out1 + f * S2 + off[j], v);
-#pragma unroll
-                    for (int d = 0; d < 7; ++d) fv[j][d] = vals[d];
-                    fx[j] = xcur[j];
-                }
-                prev_a2[j] = L[j].aS2;
-                xprev[j] = xcur[j];
-                xcur[j] = L[j].xn;
-            }
-        }
+        if (f <= q1) first(f, ring[f & 3], L, fv, fx);
 #pragma unroll
         for (int kk = 0; kk < CORE; ++kk) {
             hx[kk] = nx[kk];
@@ -245,7 +246,7 @@ static int pair_launch(uhip_ctx ctx, const uhip_csr *A, const T *x1, const T *b,
     constexpr int NT = PairShape<T>::NT, CORE = PairShape<T>::CORE;
     const int32_t S1 = A->dia_s1, S2 = A->dia_s2, nz = A->nrows / S2;
     const int32_t chunks = S2 / (CORE * NT);
-    const int32_t P = UHIP_PAIR_PLANES;
+    const int32_t P = kPairPlanes;
     const int64_t nblk = (int64_t)chunks * ((nz + P - 1) / P);
     if (nblk <= 0) return UHIP_OK;
     if (nblk > INT32_MAX) return set_error(UHIP_ERR_INVALID, "dia7 pair: grid too large");
