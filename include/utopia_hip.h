@@ -66,12 +66,13 @@ int uhip_pointer_is_device(const void *p);
 int uhip_memcpy_dtoh(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
 int uhip_memcpy_dtod(uhip_ctx ctx, void *dst, const void *src, size_t bytes);
 int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h);
-/* Asynchronous read-back of n device scalars: fetch enqueues the copy on the context stream
- * (the host goes on enqueuing work), wait blocks until that copy is done and returns the values.
- * One fetch in flight per context: a second fetch replaces the first. No reference analogue
- * (it lets the CG enqueue its preconditioner before it reads alpha and |r|). */
-int uhip_fetch_scalars(uhip_ctx ctx, const double *src_d, int n);
-int uhip_fetch_wait(uhip_ctx ctx, int n, double *dst_h);
+/* Asynchronous read-back of n device scalars into one of two slots (0, 1): fetch enqueues the copy
+ * on the context stream (the host goes on enqueuing work), wait blocks until that slot's copy is
+ * done and returns the values. One fetch in flight per slot: a second fetch into a slot replaces
+ * the first. No reference analogue (it lets the CG enqueue its preconditioner before it reads
+ * alpha and |r|, and MPRGP its next product before it reads the state of the new iterate). */
+int uhip_fetch_scalars(uhip_ctx ctx, int slot, const double *src_d, int n);
+int uhip_fetch_wait(uhip_ctx ctx, int slot, int n, double *dst_h);
 int uhip_write_scalars(uhip_ctx ctx, double *dst_d, int n, const double *src_h);
 int uhip_event_timer_start(uhip_ctx ctx, void **ev_pair);
 int uhip_event_timer_stop(uhip_ctx ctx, void *ev_pair, float *ms);

@@ -52,10 +52,12 @@ public:
     void read_scalars(const double *src, int n, double *dst) const {
         hip_check(uhip_read_scalars(ctx_, src, n, dst), "uhip_read_scalars");
     }
-    void fetch_scalars(const double *src, int n) const {
-        hip_check(uhip_fetch_scalars(ctx_, src, n), "uhip_fetch_scalars");
+    void fetch_scalars(int slot, const double *src, int n) const {
+        hip_check(uhip_fetch_scalars(ctx_, slot, src, n), "uhip_fetch_scalars");
     }
-    void fetch_wait(int n, double *dst) const { hip_check(uhip_fetch_wait(ctx_, n, dst), "uhip_fetch_wait"); }
+    void fetch_wait(int slot, int n, double *dst) const {
+        hip_check(uhip_fetch_wait(ctx_, slot, n, dst), "uhip_fetch_wait");
+    }
 
 private:
     uhip_ctx ctx_ = nullptr;

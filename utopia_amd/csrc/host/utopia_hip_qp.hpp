@@ -77,14 +77,23 @@ public:
         double st[3];
         A.context()->read_scalars(s + S_BB, 2, st);
         double beta_beta = st[0], fi_fi = st[1];
+        // A p with p.Ap and g.p, and alpha_f, of the current p: a CG step's first launches. They
+        // read x, g and p only, so they are enqueued ahead of the read-back of the state that
+        // decides whether the next iteration is a CG step (most are); a proportioning step then
+        // leaves their results unused.
+        bool ahead = false;
+        auto cg_products = [&]() {
+            static_assert(S_GP == S_PAP + 1 && S_AF == S_PAP + 2, "p.Ap, g.p and alpha_f in consecutive slots");
+            hip_check(uhip_spmv_dot2_f64(c, &A.raw(), p_.data(), Ap_.data(), g_.data(), s + S_PAP), "Ap, g.p");
+            hip_check(uhip_mprgp_alpha_f_f64(c, n, x.data(), p_.data(), lb, ub, s + S_AF), "alpha_f");
+            A.context()->fetch_scalars(1, s + S_PAP, 3);
+        };
         while (!converged) {
             bool state_done = false;
             if (beta_beta <= (gamma * gamma * fi_fi)) {
-                static_assert(S_GP == S_PAP + 1, "p.Ap and g.p in consecutive slots");
-                hip_check(uhip_spmv_dot2_f64(c, &A.raw(), p_.data(), Ap_.data(), g_.data(), s + S_PAP), "Ap, g.p");
-                hip_check(uhip_mprgp_alpha_f_f64(c, n, x.data(), p_.data(), lb, ub, s + S_AF), "alpha_f");
+                if (!ahead) cg_products();
                 double h[3];
-                A.context()->read_scalars(s + S_PAP, 3, h);
+                A.context()->fetch_wait(1, 3, h);
                 const double pAp = h[0], gp_dot = h[1], alpha_f = h[2];
                 if (pAp <= 0.0) return true;  // negative curvature
                 const double alpha_cg = gp_dot / pAp;
@@ -121,7 +130,10 @@ public:
                 hip_check(uhip_mprgp_state_f64(c, n, x.data(), g_.data(), lb, ub, fi_.data(), beta_.data(),
                                                gp_.data(), s + S_BB),
                           "state");
-            A.context()->read_scalars(s + S_BB, 3, st);
+            A.context()->fetch_scalars(0, s + S_BB, 3);
+            cg_products();
+            ahead = true;
+            A.context()->fetch_wait(0, 3, st);
             beta_beta = st[0];
             fi_fi = st[1];
             gnorm = std::sqrt(st[2]);

@@ -255,11 +255,11 @@ private:
             hip_check(uhip_cg_residual_f64(c, b.size(), r_.data(), Ap_.data(), r_new_.data(), r32, s, i_rz, S_PAP,
                                            S_ALPHA, S_RNORM),
                       "cg residual");
-            A.context()->fetch_scalars(s + S_ALPHA, 2);
+            A.context()->fetch_scalars(0, s + S_ALPHA, 2);
             const bool near = r_prev > 0 && r_prev2 > 0 && r_prev * (r_prev / r_prev2) <= 2.0 * this->atol_;
             double h[2];
             if (near) {
-                A.context()->fetch_wait(2, h);
+                A.context()->fetch_wait(0, 2, h);
                 if (std::isinf(h[0]) || std::isnan(h[0])) {  // stops before touching x
                     stop = this->check_convergence(it, r_norm, 1, 1);
                     break;
@@ -284,7 +284,7 @@ private:
                                         z32 ? UHIP_F32 : UHIP_F64, s, S_ALPHA, i_rznew, i_rz),
                       "x, p update");
             if (!near) {
-                A.context()->fetch_wait(2, h);
+                A.context()->fetch_wait(0, 2, h);
                 if (std::isinf(h[0]) || std::isnan(h[0])) {  // the update above left x alone
                     stop = this->check_convergence(it, r_norm, 1, 1);
                     break;

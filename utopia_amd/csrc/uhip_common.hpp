@@ -57,8 +57,8 @@ struct uhip_context_s {
     const int32_t *gate = nullptr;  // device counter: batched kernels skip their work while it reads 0
     double *dev_scalars = nullptr;  // UHIP_NUM_SCALARS device scalar slots
     double *host_scalars = nullptr; // pinned mirror used for read-back
-    double *host_fetch = nullptr;   // pinned target of the asynchronous read-back
-    hipEvent_t fetch_done = nullptr; // recorded after that read-back
+    double *host_fetch = nullptr;        // pinned targets of the asynchronous read-backs (2 slots)
+    hipEvent_t fetch_done[2] = {nullptr, nullptr}; // recorded after each slot's read-back
     unsigned int *counters = nullptr;
     int autotune_reps = 0;          // > 0: automatic SpMV plans are chosen by timing
 };

@@ -115,10 +115,11 @@ int uhip_ctx_create(int device, void *stream, uhip_ctx *out) {
         st = set_error(UHIP_ERR_ALLOC, "hipMalloc scalars");
     if (st == UHIP_OK && hipHostMalloc(&c->host_scalars, sizeof(double) * UHIP_NUM_SCALARS) != hipSuccess)
         st = set_error(UHIP_ERR_ALLOC, "hipHostMalloc scalars");
-    if (st == UHIP_OK && hipHostMalloc(&c->host_fetch, sizeof(double) * UHIP_NUM_SCALARS) != hipSuccess)
+    if (st == UHIP_OK && hipHostMalloc(&c->host_fetch, sizeof(double) * 2 * UHIP_NUM_SCALARS) != hipSuccess)
         st = set_error(UHIP_ERR_ALLOC, "hipHostMalloc fetch scalars");
-    if (st == UHIP_OK && hipEventCreateWithFlags(&c->fetch_done, hipEventDisableTiming) != hipSuccess)
-        st = set_error(UHIP_ERR_HIP, "hipEventCreate");
+    for (int k = 0; k < 2; ++k)
+        if (st == UHIP_OK && hipEventCreateWithFlags(&c->fetch_done[k], hipEventDisableTiming) != hipSuccess)
+            st = set_error(UHIP_ERR_HIP, "hipEventCreate");
     if (st == UHIP_OK && hipMalloc(&c->counters, sizeof(unsigned int) * 64) != hipSuccess)
         st = set_error(UHIP_ERR_ALLOC, "hipMalloc counters");
     if (st == UHIP_OK && hipMemsetAsync(c->dev_scalars, 0, sizeof(double) * UHIP_NUM_SCALARS, c->stream) != hipSuccess)
@@ -142,7 +143,8 @@ int uhip_ctx_destroy(uhip_ctx ctx) {
     if (ctx->dev_scalars) hipFree(ctx->dev_scalars);
     if (ctx->host_scalars) hipHostFree(ctx->host_scalars);
     if (ctx->host_fetch) hipHostFree(ctx->host_fetch);
-    if (ctx->fetch_done) hipEventDestroy(ctx->fetch_done);
+    for (int k = 0; k < 2; ++k)
+        if (ctx->fetch_done[k]) hipEventDestroy(ctx->fetch_done[k]);
     if (ctx->counters) hipFree(ctx->counters);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -228,19 +230,22 @@ int uhip_read_scalars(uhip_ctx ctx, const double *src_d, int n, double *dst_h) {
     return UHIP_OK;
 }
 
-int uhip_fetch_scalars(uhip_ctx ctx, const double *src_d, int n) {
+int uhip_fetch_scalars(uhip_ctx ctx, int slot, const double *src_d, int n) {
     UHIP_REQUIRE(ctx && src_d, "uhip_fetch_scalars: null argument");
+    UHIP_REQUIRE(slot == 0 || slot == 1, "uhip_fetch_scalars: slot %d (0 or 1)", slot);
     UHIP_REQUIRE(n > 0 && n <= UHIP_NUM_SCALARS, "uhip_fetch_scalars: n=%d out of range", n);
-    UHIP_CHECK_HIP(hipMemcpyAsync(ctx->host_fetch, src_d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-    UHIP_CHECK_HIP(hipEventRecord(ctx->fetch_done, ctx->stream));
+    UHIP_CHECK_HIP(hipMemcpyAsync(ctx->host_fetch + slot * UHIP_NUM_SCALARS, src_d, sizeof(double) * n,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    UHIP_CHECK_HIP(hipEventRecord(ctx->fetch_done[slot], ctx->stream));
     return UHIP_OK;
 }
 
-int uhip_fetch_wait(uhip_ctx ctx, int n, double *dst_h) {
+int uhip_fetch_wait(uhip_ctx ctx, int slot, int n, double *dst_h) {
     UHIP_REQUIRE(ctx && dst_h, "uhip_fetch_wait: null argument");
+    UHIP_REQUIRE(slot == 0 || slot == 1, "uhip_fetch_wait: slot %d (0 or 1)", slot);
     UHIP_REQUIRE(n > 0 && n <= UHIP_NUM_SCALARS, "uhip_fetch_wait: n=%d out of range", n);
-    UHIP_CHECK_HIP(hipEventSynchronize(ctx->fetch_done));
-    memcpy(dst_h, ctx->host_fetch, sizeof(double) * n);
+    UHIP_CHECK_HIP(hipEventSynchronize(ctx->fetch_done[slot]));
+    memcpy(dst_h, ctx->host_fetch + slot * UHIP_NUM_SCALARS, sizeof(double) * n);
     return UHIP_OK;
 }
 
