@@ -205,6 +205,17 @@ def test_dia_skips_absent_entries_exactly(ctx):
     ref = orc.spmv(A, x)
     assert np.isfinite(y[8]) and np.array_equal(np.isfinite(y), np.isfinite(ref))
     assert np.array_equal(y[np.isfinite(ref)], ref[np.isfinite(ref)])
+    # full layout of a non-symmetric matrix, absent entries NaN-encoded and selected on the value
+    N = _perturbed(A, 10, 3)
+    D = ua.DeviceCsr(ctx, N, fmt=ua.FMT_DIA)
+    assert D.m.dia_sym == 0 and D.m.dia_nan == 1
+    for col in (7, 8):
+        x = ua.gen_uniform(5, 64)
+        x[col] = np.inf
+        D.spmv(ua.DeviceArray.from_numpy(ctx, x), yd)
+        y, ref = yd.numpy(), orc.spmv(N, x)
+        assert np.array_equal(np.isfinite(y), np.isfinite(ref))
+        assert np.array_equal(y[np.isfinite(ref)], ref[np.isfinite(ref)])
 
 
 def test_dia_rejects_unsorted_and_rectangular(ctx):
