@@ -33,6 +33,10 @@ template <typename T>
 struct PairShape {
     static constexpr int NT = sizeof(T) == 4 ? 1024 : 512;
     static constexpr int CORE = 2;
+    // FP64 issues the next plane's loads before the barrier ending a step (they fly through it
+    // and the next second product: measured faster); FP32 issues them at the start of the step
+    // (measured faster there)
+    static constexpr bool EARLY = sizeof(T) == 8;
 };
 
 // MODE 0 (pre-smoothing pair): c = a_prev b + a_cur (-(-b + a_prev A b) + beta b), the last
@@ -195,10 +199,12 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
         for (int d = 0; d < 7; ++d) hv[k][d] = nv[k][d] = nan;
     }
     const int kend = ze + 1 - q0;  // the step whose second product is plane ze - 1
+    constexpr bool EARLY = PairShape<T>::EARLY;
+    Ld L[NS];
+    if (EARLY) issue(q0, L);
     for (int k = 0; k <= kend; ++k) {
         const int f = q0 + k, p = f - 2;
-        Ld L[NS];
-        if (f <= q1) issue(f, L);
+        if (!EARLY && f <= q1) issue(f, L);
         double fd[CORE];
         dw_of(p + 1, fd);
         if (p >= zs && p < ze) second(p, ring[(p + 3) & 3], ring[p & 3], ring[(p + 1) & 3], hv, hx, hd);
@@ -210,6 +216,7 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
             for (int d = 0; d < 7; ++d) fv[kk][d] = nan;
         }
         if (f <= q1) first(f, ring[f & 3], L, fv, fx);
+        if (EARLY && f + 1 <= q1) issue(f + 1, L);
 #pragma unroll
         for (int kk = 0; kk < CORE; ++kk) {
             hx[kk] = nx[kk];
