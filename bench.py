@@ -50,13 +50,20 @@ def dia_bytes(n, ndiag, val_bytes, sym=0, nan_select=False):
 
 
 def vi_bytes(D, val_bytes, add=False):
-    # value-indexed transfer: per row the base column (4 B), slot mask (1/2/4 B) and 8-bit codes
+    # value-indexed transfer: per row the base column (4 B) and either a signature id (1 B; the
+    # signature table is read per workgroup from cache) or its slot mask (1/2/4 B) and 8-bit codes
     # (8 B per 8 slots), the value table, x read once (ncols), y written (and read by y += A x)
     m = D.m
     nw = (m.ndiag + 7) // 8
     mb = 1 if m.ndiag <= 8 else (2 if m.ndiag <= 16 else 4)
     rows, cols = m.nrows, m.ncols
-    return (4 + mb + 8 * nw) * rows + val_bytes * m.vi_nvals + val_bytes * cols + val_bytes * rows * (2 if add else 1)
+    per_row = 4 + 1 if m.vi_nsig > 0 else 4 + mb + 8 * nw
+    tables = val_bytes * m.vi_nvals + (8 * (nw + 1) * m.vi_nsig if m.vi_nsig > 0 else 0)
+    return per_row * rows + tables + val_bytes * cols + val_bytes * rows * (2 if add else 1)
+
+
+def vi_sig_note(D):
+    return f", {D.m.vi_nsig} row signatures" if D.m.vi_nsig > 0 else ""
 
 
 def time_kernel(ua, ctx, launch, reps):
@@ -225,12 +232,14 @@ def solve_kernels(ua, ctx, A, interps, n_rows, reps=20):
                      reps)
     nb = vi_bytes(R, 4) + 4 * nc
     out["restrict_f32"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
-                           "label": f"EpiMaskStore<float> R r_R, format {R.format} (VI = 3), {R.m.ndiag} slots"}
+                           "label": f"EpiMaskStore<float> R r_R, format {R.format} (VI = 3), {R.m.ndiag} slots"
+                                    f"{vi_sig_note(R)}"}
     ms = time_kernel(ua, ctx, lambda: ua.check(L.uhip_spmv_add(ctx.h, P.ref, ctypes.c_void_p(xc.ptr),
                                                                ctypes.c_void_p(xs[1].ptr))), reps)
     nb = vi_bytes(P, 4, add=True)
     out["interpolate_f32"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
-                              "label": f"EpiAddTo<float> c += P c_c, format {P.format} (VI = 3), {P.m.ndiag} slots"}
+                              "label": f"EpiAddTo<float> c += P c_c, format {P.format} (VI = 3), {P.m.ndiag} slots"
+                                       f"{vi_sig_note(P)}"}
     return out
 
 

@@ -222,6 +222,9 @@ typedef struct uhip_csr {
     int32_t dia_nan;    /* DIA: 1 when the matrix holds no NaN; the entries a row does not hold are
                          * then NaN in dia_val (else zero) and kernels may select on the value */
     int32_t dia_s1, dia_s2; /* dia_sym 2: the offsets S1 and S2 (host copies of dia_off[5], dia_off[6]) */
+    int32_t vi_nsig;    /* VI: > 0 when the rows' (slot mask, codes) pairs take at most 256 distinct
+                         * values: dia_mask then holds one uint8 signature id per row and dia_val
+                         * the vi_nsig signatures (mask word, then the code words) */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)

@@ -621,6 +621,7 @@ def test_spmv_vi_bit_exact(ctx, name, dt):
     m, n = A.shape
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_VI)
     assert D.format == ua.FMT_VI and 1 <= D.m.vi_nvals <= 256 and 1 <= D.m.ndiag <= 32
+    assert 1 <= D.m.vi_nsig <= 256  # interior, face, edge and corner rows: signature ids per row
     x = ua.gen_uniform(17, n).astype(dt)
     ref = orc.spmv_f32(A, x) if dt is np.float32 else orc.spmv(A, x)
     xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, m, dt)
@@ -660,7 +661,7 @@ def test_vi_square_epilogues(ctx, name):
     A = make()
     n = A.shape[0]
     D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_VI)
-    assert D.format == ua.FMT_VI and D.m.ndiag == slots and D.m.vi_nvals == 2
+    assert D.format == ua.FMT_VI and D.m.ndiag == slots and D.m.vi_nvals == 2 and D.m.vi_nsig > 0
     x, b = ua.gen_uniform(21, n), ua.gen_uniform(22, n)
     Ax = orc.spmv(A, x)
     xd, bd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b), ua.DeviceArray(ctx, n)
@@ -672,6 +673,28 @@ def test_vi_square_epilogues(ctx, name):
     ua.check(L().uhip_spmv_dot(ctx.h, D.ref, P(xd), P(yd), ctypes.c_void_p(s)))
     assert np.array_equal(yd.numpy(), Ax)
     assert abs(ctx.read_scalars(s, 1)[0] - x @ Ax) <= 1e-13 * (np.abs(x) @ np.abs(Ax))
+
+
+@pytest.mark.parametrize("slots", [8, 16, 32])
+def test_vi_many_signatures_keep_row_codes(ctx, slots):
+    """Rows with random slot subsets and values (thousands of distinct signatures) keep their
+    per-row masks and codes (one, two and four code words); the product stays bit-exact."""
+    rng = np.random.default_rng(slots)
+    n = 6000
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        k = np.flatnonzero(rng.random(slots) < 0.5)
+        k = np.union1d(k, [0])
+        rows += [i] * len(k)
+        cols += list(i + k)
+        vals += list(rng.integers(1, 9, len(k)) * 0.25)
+    A = ua.Csr.from_scipy(sp.csr_matrix((vals, (rows, cols)), shape=(n, n + slots)))
+    D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_VI)
+    assert D.format == ua.FMT_VI and D.m.vi_nsig == 0 and D.m.ndiag <= slots
+    x = ua.gen_uniform(23, n + slots)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, n)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv(A, x))
 
 
 def test_vi_rejects_many_values_and_nan(ctx):

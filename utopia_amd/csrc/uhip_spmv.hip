@@ -533,6 +533,144 @@ __global__ void k_vi_table(int32_t nvals, const unsigned long long *__restrict__
     else table[i] = (T)__uint_as_float((unsigned)sorted_bits[i]);
 }
 
+// Row signatures of a VI plan: a row's (slot mask, code words) pair. Transfer operators have few
+// distinct ones (interior rows, faces, edges, corners), so a plan keeps one uint8 signature id
+// per row and a table of at most 256 signatures instead of every row's mask and codes.
+__device__ __forceinline__ uint64_t sig_mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    return h ^ (h >> 29);
+}
+
+template <typename TM>
+__global__ __launch_bounds__(kBlock) void k_vi_sig_hash(int32_t n, const uint64_t *__restrict__ codes,
+                                                        const TM *__restrict__ mask, uint64_t *__restrict__ key) {
+    constexpr int NW = (int)sizeof(TM);
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    uint64_t h = sig_mix(0x243F6A8885A308D3ull, (uint64_t)mask[r]);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) h = sig_mix(h, codes[(int64_t)r * NW + w]);
+    key[r] = h;
+}
+
+// signature id of each row (its key's place among the distinct keys) and, per signature, the
+// first row that has it
+__global__ __launch_bounds__(kBlock) void k_vi_sig_assign(int32_t n, const uint64_t *__restrict__ key,
+                                                          const uint64_t *__restrict__ uniq,
+                                                          const int32_t *__restrict__ nuniq, uint8_t *__restrict__ sig,
+                                                          int32_t *__restrict__ rep) {
+    __shared__ uint64_t u[256];
+    const int nu = *nuniq;
+    for (int i = threadIdx.x; i < nu; i += kBlock) u[i] = uniq[i];
+    __syncthreads();
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t k = key[r];
+    int lo = 0, hi = nu - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (u[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    sig[r] = (uint8_t)lo;
+    atomicMin(&rep[lo], r);
+}
+
+template <typename TM>
+__global__ void k_vi_sig_table(int32_t nsig, const int32_t *__restrict__ rep, const uint64_t *__restrict__ codes,
+                               const TM *__restrict__ mask, uint64_t *__restrict__ tab) {
+    constexpr int NW = (int)sizeof(TM);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsig) return;
+    const int32_t r = rep[i];
+    tab[(int64_t)i * (NW + 1)] = (uint64_t)mask[r];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) tab[(int64_t)i * (NW + 1) + 1 + w] = codes[(int64_t)r * NW + w];
+}
+
+// every row's mask and codes equal its signature's (a hash collision would break this)
+template <typename TM>
+__global__ __launch_bounds__(kBlock) void k_vi_sig_check(int32_t n, const uint8_t *__restrict__ sig,
+                                                         const uint64_t *__restrict__ tab,
+                                                         const uint64_t *__restrict__ codes,
+                                                         const TM *__restrict__ mask, int32_t *__restrict__ bad) {
+    constexpr int NW = (int)sizeof(TM);
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t *t = tab + (int64_t)sig[r] * (NW + 1);
+    bool same = t[0] == (uint64_t)mask[r];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) same = same && t[1 + w] == codes[(int64_t)r * NW + w];
+    if (!same) atomicOr(bad, 1);
+}
+
+// Replace a VI plan's per-row masks and codes by signature ids when at most 256 distinct
+// signatures occur. Leaves the plan as it is otherwise.
+template <typename TM>
+static int vi_signatures(uhip_ctx ctx, uhip_csr *A) {
+    hipStream_t s = ctx->stream;
+    const int32_t n = A->nrows;
+    constexpr int NW = (int)sizeof(TM);
+    const uint64_t *codes = (const uint64_t *)A->dia_val;
+    const TM *mask = (const TM *)A->dia_mask;
+    uint64_t *key = nullptr, *sorted = nullptr, *uniq = nullptr, *tab = nullptr;
+    int32_t *nsel = nullptr, *rep = nullptr;
+    uint8_t *sig = nullptr;
+    void *tmp = nullptr;
+    size_t tb = 0, tb2 = 0;
+    auto done = [&](int ret) {
+        hipFree(key);
+        hipFree(sorted);
+        hipFree(uniq);
+        hipFree(nsel);
+        hipFree(rep);
+        hipFree(tmp);
+        if (ret != UHIP_OK) {
+            hipFree(sig);
+            hipFree(tab);
+        }
+        return ret;
+    };
+    const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+    if (hipMalloc(&key, sizeof(uint64_t) * n) != hipSuccess || hipMalloc(&sorted, sizeof(uint64_t) * n) != hipSuccess ||
+        hipMalloc(&uniq, sizeof(uint64_t) * n) != hipSuccess || hipMalloc(&nsel, sizeof(int32_t) * 2) != hipSuccess)
+        return done(set_error(UHIP_ERR_ALLOC, "VI signatures: hipMalloc"));
+    hipLaunchKernelGGL((k_vi_sig_hash<TM>), g, b, 0, s, n, codes, mask, key);
+    hipcub::DeviceRadixSort::SortKeys(nullptr, tb, key, sorted, n, 0, 64, s);
+    hipcub::DeviceSelect::Unique(nullptr, tb2, sorted, uniq, nsel, n, s);
+    if (hipMalloc(&tmp, std::max(tb, tb2)) != hipSuccess) return done(set_error(UHIP_ERR_ALLOC, "VI signatures: temp"));
+    tb = tb2 = std::max(tb, tb2);
+    if (hipcub::DeviceRadixSort::SortKeys(tmp, tb, key, sorted, n, 0, 64, s) != hipSuccess ||
+        hipcub::DeviceSelect::Unique(tmp, tb2, sorted, uniq, nsel, n, s) != hipSuccess)
+        return done(set_error(UHIP_ERR_HIP, "VI signatures: sort / unique"));
+    int32_t nu = 0;
+    UHIP_CHECK_HIP(hipMemcpyAsync(&nu, nsel, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    if (nu < 1 || nu > 256) return done(UHIP_OK);  // too many: keep the per-row codes
+    std::vector<int32_t> big(256, INT32_MAX);
+    if (hipMalloc(&rep, sizeof(int32_t) * 256) != hipSuccess || hipMalloc(&sig, (size_t)n) != hipSuccess ||
+        hipMalloc(&tab, sizeof(uint64_t) * (NW + 1) * (size_t)nu) != hipSuccess)
+        return done(set_error(UHIP_ERR_ALLOC, "VI signatures: hipMalloc"));
+    UHIP_CHECK_HIP(hipMemcpyAsync(rep, big.data(), sizeof(int32_t) * 256, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_vi_sig_assign, g, b, 0, s, n, key, uniq, nsel, sig, rep);
+    hipLaunchKernelGGL((k_vi_sig_table<TM>), dim3(1), dim3(256), 0, s, nu, rep, codes, mask, tab);
+    UHIP_CHECK_HIP(hipMemsetAsync(nsel + 1, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL((k_vi_sig_check<TM>), g, b, 0, s, n, sig, tab, codes, mask, nsel + 1);
+    if (hipGetLastError() != hipSuccess) return done(set_error(UHIP_ERR_HIP, "VI signatures: launch"));
+    int32_t bad = 0;
+    UHIP_CHECK_HIP(hipMemcpyAsync(&bad, nsel + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    UHIP_CHECK_HIP(hipStreamSynchronize(s));
+    if (bad) return done(UHIP_OK);  // a hash collision: keep the per-row codes
+    UHIP_CHECK_HIP(hipFree(A->dia_val));
+    UHIP_CHECK_HIP(hipFree(A->dia_mask));
+    A->dia_val = tab;
+    A->dia_mask = sig;
+    A->vi_nsig = nu;
+    return done(UHIP_OK);
+}
+
 // Plan VI arrays; UHIP_ERR_UNSUPPORTED (nothing kept) when a row is unsorted, the
 // rows reach more than 32 distinct offsets from their first column, the values
 // more than 256 distinct bit patterns (or any NaN), or ndiag * n > max_ratio * nnz.
@@ -605,7 +743,11 @@ static int plan_vi(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->vi_nvals = nv;
+    A->vi_nsig = 0;
     A->format = UHIP_FMT_VI;
+    if (mb == 1) UHIP_TRY(vi_signatures<uint8_t>(ctx, A));
+    else if (mb == 2) UHIP_TRY(vi_signatures<uint16_t>(ctx, A));
+    else UHIP_TRY(vi_signatures<uint32_t>(ctx, A));
     const int32_t rows_per_wg = kDiaWG * vi_rows_per_lane(A->dtype, nd);
     A->nparts = (n + rows_per_wg - 1) / rows_per_wg;
     return UHIP_OK;
@@ -883,6 +1025,7 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     A->dia_sym = 0;
     A->dia_nan = 0;
     A->dia_s1 = A->dia_s2 = 0;
+    A->vi_nsig = 0;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));

@@ -430,20 +430,34 @@ inline int vi_rows_per_lane(int dtype, int32_t ndiag) {
 // is staged in LDS. A row's codes arrive in NW wide loads, a slot that no row of
 // the wavefront holds costs nothing (wave-uniform branch), and the masked,
 // column-ordered sum makes each row sum the sequential CSR sum bit for bit.
-template <typename TAcc, class Epi, typename TA, typename TX, typename TM, int R>
+// SIG (vi_nsig > 0): mask holds one uint8 signature id per row and codes the signatures (mask
+// word, then the NW code words), staged in LDS.
+template <typename TAcc, class Epi, typename TA, typename TX, typename TM, int R, bool SIG = false>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag, const int32_t *__restrict__ off,
                                                     const uint64_t *__restrict__ codes, const TA *__restrict__ table,
                                                     int32_t nvals, const int32_t *__restrict__ vbase,
                                                     const TM *__restrict__ mask, const TX *__restrict__ x, Epi epi,
-                                                    double *__restrict__ partials, int64_t stride) {
+                                                    double *__restrict__ partials, int64_t stride, int32_t nsig) {
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
     constexpr int NW = (int)sizeof(TM);  // 64-bit code words per row: 8 slots each
     constexpr bool kFlat = R == 1 && NW >= 2;
     __shared__ TA tab[256];
     __shared__ int32_t so[32];
+    __shared__ uint64_t stab[SIG ? 256 * (NW + 1) : 1];
+    const uint8_t *sid = (const uint8_t *)mask;
     for (int i = threadIdx.x; i < nvals; i += kDiaWG) tab[i] = table[i];
+    if constexpr (SIG)
+        for (int i = threadIdx.x; i < nsig * (NW + 1); i += kDiaWG) stab[i] = codes[i];
     if (kFlat && threadIdx.x < 32) so[threadIdx.x] = (int32_t)threadIdx.x < ndiag ? off[threadIdx.x] : 0;
     __syncthreads();
+    auto row_mask = [&](int32_t rc) -> uint32_t {
+        if constexpr (SIG) return (uint32_t)stab[(int)sid[rc] * (NW + 1)];
+        else return (uint32_t)mask[rc];
+    };
+    auto row_code = [&](int32_t rc, int w) -> uint64_t {
+        if constexpr (SIG) return stab[(int)sid[rc] * (NW + 1) + 1 + w];
+        else return ld_stream(codes + (int64_t)rc * NW + w);
+    };
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x & 63;
     if constexpr (kFlat) {
@@ -456,12 +470,12 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         for (int d = 0; d < ND; ++d) part[d] = 0.0;
         const int32_t row = b * kDiaWG + (int32_t)threadIdx.x;
         const int32_t rc = min(row, nrows - 1);
-        const uint32_t m = row < nrows ? (uint32_t)mask[rc] : 0u;
+        const uint32_t m = row < nrows ? row_mask(rc) : 0u;
         const int32_t rb = vbase[rc];
         const typename Epi::In ein = epi.load(rc);
         uint64_t c[NW];
 #pragma unroll
-        for (int w = 0; w < NW; ++w) c[w] = ld_stream(codes + (int64_t)rc * NW + w);
+        for (int w = 0; w < NW; ++w) c[w] = row_code(rc, w);
         TX xv[8 * NW];
 #pragma unroll
         for (int d = 0; d < 8 * NW; ++d) xv[d] = x[((m >> d) & 1u) ? rb + so[d] : rb];
@@ -496,7 +510,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         for (int i = 0; i < R; ++i) {
             row[i] = base + 64 * i + lane;
             const int32_t rc = min(row[i], nrows - 1);
-            m[i] = (uint32_t)mask[rc];
+            m[i] = row_mask(rc);
             rb[i] = vbase[rc];
             acc[i] = TAcc(0);
         }
@@ -516,7 +530,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
             if (8 * w >= ndiag) break;
             uint64_t c[R];
 #pragma unroll
-            for (int i = 0; i < R; ++i) c[i] = ld_stream(codes + (int64_t)min(row[i], nrows - 1) * NW + w);
+            for (int i = 0; i < R; ++i) c[i] = row_code(min(row[i], nrows - 1), w);
             TX xv[8][R];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -580,14 +594,20 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         double *parts = parts_in ? parts_in : ctx->partials;
         const int64_t pstride = parts_in ? cap : ctx->max_partials;
         const dim3 g(A->nparts), blk(kDiaWG);
+#define UHIP_VI_LAUNCH2(TM, RR, SIG)                                                                            \
+    hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM, RR, SIG>), g, blk, 0, stream_of(ctx), A->nrows,          \
+                       A->ndiag, A->dia_off, (const uint64_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals, \
+                       A->vi_base, (const TM *)A->dia_mask, x, epi, parts, pstride, A->vi_nsig)
 #define UHIP_VI_LAUNCH(TM, RR)                                                                                  \
-    hipLaunchKernelGGL((k_spmv_vi<TAcc, Epi, TA, TX, TM, RR>), g, blk, 0, stream_of(ctx), A->nrows, A->ndiag,      \
-                       A->dia_off, (const uint64_t *)A->dia_val, (const TA *)A->vi_table, A->vi_nvals,         \
-                       A->vi_base, (const TM *)A->dia_mask, x, epi, parts, pstride)
+    do {                                                                                                        \
+        if (A->vi_nsig > 0) UHIP_VI_LAUNCH2(TM, RR, true);                                                      \
+        else UHIP_VI_LAUNCH2(TM, RR, false);                                                                    \
+    } while (0)
         if (A->ndiag <= 8) UHIP_VI_LAUNCH(uint8_t, vi_rows<TA>(8));
         else if (A->ndiag <= 16) UHIP_VI_LAUNCH(uint16_t, 1);
         else UHIP_VI_LAUNCH(uint32_t, 1);
 #undef UHIP_VI_LAUNCH
+#undef UHIP_VI_LAUNCH2
         UHIP_CHECK_LAUNCH();
         return UHIP_OK;
     }
