@@ -221,10 +221,14 @@ typedef struct uhip_csr {
     struct uhip_csr *tr; /* A^T with its own plan (uhip_csr_plan_transpose), freed with A's plan */
     int32_t dia_nan;    /* DIA: 1 when the matrix holds no NaN; the entries a row does not hold are
                          * then NaN in dia_val (else zero) and kernels may select on the value */
-    int32_t dia_s1, dia_s2; /* dia_sym 2: the offsets S1 and S2 (host copies of dia_off[5], dia_off[6]) */
+    int32_t dia_s1, dia_s2; /* dia_sym 2 or dia_w27: the offsets S1 and S2 (host copies of dia_off[5],
+                             * dia_off[6], or dia_off[16], dia_off[22] of a 27-point plan) */
     int32_t vi_nsig;    /* VI: > 0 when the rows' (slot mask, codes) pairs take at most 256 distinct
                          * values: dia_mask then holds one uint8 signature id per row and dia_val
                          * the vi_nsig signatures (mask word, then the code words) */
+    int32_t dia_w27;    /* DIA: 1 when the offsets are the 27 of dz S2 + dy S1 + dx (dz, dy, dx in
+                         * {-1, 0, 1}, S2 a multiple of 64) and the plane-walking 27-point kernel
+                         * runs (full or symmetric layout) */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)

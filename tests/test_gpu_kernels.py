@@ -14,6 +14,7 @@ import scipy.sparse as sp
 import oracle as orc
 import utopia_amd as ua
 from utopia_amd import petsc_io
+from utopia_amd._abi import uhip_cheb_step
 
 pytestmark = pytest.mark.gpu
 
@@ -179,6 +180,152 @@ def test_spmv_dia_bit_exact(ctx, name, dt):
     yd = ua.DeviceArray(ctx, n, dt)
     D.spmv(xd, yd)
     assert np.array_equal(yd.numpy(), ref)
+
+
+def _galerkin_dims(nx, ny, nz, seed=2):
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, nx, ny, nz, seed=seed)
+    return ua.Csr(*orc.ptap(A, ua.gen_prolongation(nx, ny, nz)))
+
+
+def _symmetrized(A):
+    """(A + A^T) / 2: symmetric bit for bit (a_ij + a_ji and a_ji + a_ij round alike)."""
+    S = A.to_scipy()
+    return ua.Csr.from_scipy(((S + S.T) * 0.5).tocsr())
+
+
+def _with_nan(A, row):
+    val = A.val.copy()
+    val[A.rowptr[row] + 2] = np.nan
+    return ua.Csr(A.rowptr, A.col, val, A.shape)
+
+
+# 27-point operators whose planes are whole multiples of 64 rows (the plane-walking kernel):
+# (coarse grid, expected dia_sym in FP64). Galerkin products are symmetric only to rounding
+# (full layout, NaN-encoded absent entries); symmetrized ones and the constant box stencil
+# take the symmetric layout. Plane counts leave partial 8-plane slabs; 16 x 16 and 32 x 32
+# planes span several 64-row segments (the mirrors one y line and one plane back come from
+# other waves).
+DIA27 = {
+    "galerkin_8x8x11": (lambda: _galerkin_dims(16, 16, 22), 0),
+    "galerkin_16x16x20": (lambda: _galerkin_dims(32, 32, 40), 0),
+    "galerkin_32x32x6": (lambda: _galerkin_dims(64, 64, 12), 0),
+    "galerkin_sym_16x8x10": (lambda: _symmetrized(_galerkin_dims(32, 16, 20)), 1),
+    "box27_16x8x13": (lambda: _box27(16, 8, 13), 1),
+    "galerkin_nan_8x8x11": (lambda: _with_nan(_galerkin_dims(16, 16, 22), 77), 0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DIA27))
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_spmv_dia27_walk_bit_exact(ctx, name, dt):
+    """The plane-walking 27-point kernel (full layout selecting on NaN-encoded absent entries or
+    on the mask, symmetric layout with mirrors): y = A x equal to the CSR sum bit for bit; a NaN
+    entry keeps the mask and gives NaN exactly in its row."""
+    make, sym = DIA27[name]
+    A = make()
+    n = A.shape[0]
+    D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
+    assert D.format == ua.FMT_DIA and D.m.ndiag == 27 and D.m.dia_w27 == 1
+    if dt is np.float64:
+        assert D.m.dia_sym == sym
+        assert D.m.dia_nan == (0 if "nan" in name else 1)
+    x = ua.gen_uniform(17, n)
+    if dt is np.float32:
+        x = x.astype(np.float32)
+        ref = orc.spmv_f32(A, x)
+    else:
+        ref = orc.spmv(A, x)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, n, dt)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), ref, equal_nan=True)
+
+
+@pytest.mark.parametrize("name", ["galerkin_16x16x20", "galerkin_sym_16x8x10", "box27_16x8x13"])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_dia27_walk_epilogues(ctx, name, dt):
+    """Every fused epilogue on the 27-point kernel against the row-block plan of the same matrix
+    (bit-exact to the CSR sums): residual, masked, add, axpby, axpbz, the product with its dot,
+    and a middle and a last Chebyshev step (x taken from the kernel's registers), with z.r."""
+    A = DIA27[name][0]()
+    n = A.shape[0]
+    D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
+    R = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_ROWBLOCK)
+    assert D.m.dia_w27 == 1
+    code = ua.F64 if dt is np.float64 else ua.F32
+    x, b, z, m, pp, rp = (ua.gen_uniform(s, n).astype(dt) for s in (1, 2, 3, 4, 5, 6))
+    m = (m > 0).astype(dt)
+    dw = ua.gen_uniform(7, n)
+    h = ctx.h
+    dev = lambda v: ua.DeviceArray.from_numpy(ctx, v)  # noqa: E731
+    xd, bd, md, ppd, rpd, dwd = (dev(v) for v in (x, b, m, pp, rp, dw))
+
+    def both(run):
+        outs = []
+        for M in (D, R):
+            o = [dev(z), dev(z), dev(z)]
+            dd = ua.DeviceArray(ctx, 1)
+            run(M, o, dd)
+            outs.append(([v.numpy() for v in o], dd.numpy()[0]))
+        return outs
+
+    for run in (lambda M, o, dd: ua.check(L().uhip_spmv_residual(h, M.ref, P(xd), P(bd), code, P(o[0]))),
+                lambda M, o, dd: ua.check(L().uhip_spmv_masked(h, M.ref, P(xd), P(md), P(o[0]))),
+                lambda M, o, dd: ua.check(L().uhip_spmv_add(h, M.ref, P(xd), P(o[0]))),
+                lambda M, o, dd: ua.check(L().uhip_spmv_axpby(h, M.ref, 2.5, P(xd), -0.5, P(o[0]))),
+                lambda M, o, dd: ua.check(L().uhip_spmv_axpbz(h, M.ref, 1.0, P(xd), -1.0, P(bd), P(o[0])))):
+        (o1, _), (o2, _) = both(run)
+        assert np.array_equal(o1[0], o2[0])
+    # A x with x.(A x)
+    s = ctx.scalars
+    outs = []
+    for M in (D, R):
+        o = ua.DeviceArray(ctx, n, dt)
+        ua.check(L().uhip_spmv_dot(h, M.ref, P(xd), P(o), ctypes.c_void_p(s)))
+        outs.append((o.numpy(), ctx.read_scalars(s, 1)[0]))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    Ax = outs[1][0].astype(np.float64)
+    bound = 2 * n * np.finfo(dt).eps * float(np.abs(x.astype(np.float64)) @ np.abs(Ax))
+    assert abs(outs[0][1] - float(x.astype(np.float64) @ Ax)) <= bound
+
+    # Chebyshev: a middle step (x, r, p out) and a last step with z.r
+    def cheb_mid(M, o, dd):
+        st = uhip_cheb_step(pprev=ppd.ptr, p_dtype=code, rprev=rpd.ptr, xin=xd.ptr, xout=o[0].ptr, x_dtype=code,
+                            rout=o[1].ptr, pout=o[2].ptr, a_prev=0.41, beta=0.073, a_cur=0.29, last=0,
+                            dot_with=None, dot_d=None)
+        ua.check(L().uhip_cheb_step_apply(h, M.ref, ctypes.byref(st)))
+
+    def cheb_last(M, o, dd):
+        st = uhip_cheb_step(pprev=ppd.ptr, p_dtype=code, rprev=None, xin=xd.ptr, xout=o[0].ptr, x_dtype=code,
+                            rout=None, pout=None, a_prev=0.41, beta=0.073, a_cur=0.29, last=1, dot_with=dwd.ptr,
+                            dot_d=dd.ptr)
+        ua.check(L().uhip_cheb_step_apply(h, M.ref, ctypes.byref(st)))
+
+    (o1, _), (o2, _) = both(cheb_mid)
+    for a, c in zip(o1, o2):
+        assert np.array_equal(a, c)
+    (o1, d1), (o2, d2) = both(cheb_last)
+    assert np.array_equal(o1[0], o2[0])
+    zr = o1[0].astype(np.float64)
+    bound = 2 * n * np.finfo(np.float64).eps * float(np.abs(zr) @ np.abs(dw))
+    assert abs(d1 - float(zr @ dw)) <= bound and abs(d2 - float(zr @ dw)) <= bound
+
+
+def test_dia27_autotune_keeps_a_valid_plan(ctx):
+    """The autotuner times the 27-point DIA plan with and without the plane walk; whichever it
+    keeps computes the CSR sum bit for bit."""
+    A = _galerkin_dims(32, 32, 40)
+    n = A.shape[0]
+    fmt = ctypes.c_int32()
+    ms = ctypes.c_float()
+    D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_DIA)
+    ua.check(L().uhip_csr_autotune(ctx.h, D.ref, 3, ctypes.byref(fmt), ctypes.byref(ms)))
+    assert fmt.value == D.format
+    if D.format == ua.FMT_DIA:
+        assert D.m.dia_w27 in (0, 1) and (D.m.dia_s2 == 256) == (D.m.dia_w27 == 1)
+    x = ua.gen_uniform(9, n)
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, n)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv(A, x))
 
 
 def test_dia_skips_absent_entries_exactly(ctx):

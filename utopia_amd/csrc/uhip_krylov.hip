@@ -63,8 +63,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double 
         if (blockIdx.x == 0 && threadIdx.x == 0) s[i_alpha] = alpha;
         return;
     }
+    // four grid-strided entries per step, their loads issued together (the grid is capped at
+    // kMaxRedBlocks, so each thread covers many entries)
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t st = (int64_t)gridDim.x * kBlock;
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        double rv[4], av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            rv[u] = __builtin_nontemporal_load(r + i + u * st);
+            av[u] = __builtin_nontemporal_load(Ap + i + u * st);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double v = rv[u] - alpha * av[u];
+            __builtin_nontemporal_store(v, rn + i + u * st);
+            if (rn32) __builtin_nontemporal_store((float)v, rn32 + i + u * st);
+            acc += v * v;
+        }
+    }
+    for (; i < n; i += st) {
         const double v = r[i] - alpha * Ap[i];
         __builtin_nontemporal_store(v, rn + i);
         if (rn32) __builtin_nontemporal_store((float)v, rn32 + i);

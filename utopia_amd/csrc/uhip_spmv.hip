@@ -388,15 +388,27 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     if (sym && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] && offs[6] % 64 == 0 &&
         (int64_t)n + (int64_t)planes * offs[6] < INT32_MAX)
         sym = 2;
+    // the 27 offsets dz S2 + dy S1 + dx in this (sorted) order, S2 a multiple of 64: the
+    // plane-walking 27-point kernel
+    bool w27 = false;
+    if (nd == 27) {
+        const int32_t S1 = offs[16], S2 = offs[22];
+        w27 = S1 > 2 && S2 > 2 * S1 + 2 && S2 % 64 == 0 &&
+              (int64_t)n + (int64_t)(dia27_planes(A->dtype) + 2) * S2 < INT32_MAX;
+        for (int i = 0; i < 27 && w27; ++i)
+            w27 = offs[(size_t)i] == (i / 9 - 1) * S2 + (i / 3 % 3 - 1) * S1 + (i % 3 - 1);
+    }
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->dia_sym = sym;
     A->dia_nan = has_nan ? 0 : 1;
-    A->dia_s1 = sym == 2 ? offs[5] : 0;
-    A->dia_s2 = sym == 2 ? offs[6] : 0;
+    A->dia_w27 = w27 ? 1 : 0;
+    A->dia_s1 = sym == 2 ? offs[5] : (w27 ? offs[16] : 0);
+    A->dia_s2 = sym == 2 ? offs[6] : (w27 ? offs[22] : 0);
     A->format = UHIP_FMT_DIA;
     A->nparts = sym == 2 ? dia7z_workgroups(n, offs[6], planes)
-                         : (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
+                         : (w27 ? dia7z_workgroups(n, offs[22], dia27_planes(A->dtype))
+                                : (n + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype));
     return UHIP_OK;
 }
 
@@ -830,19 +842,27 @@ __global__ __launch_bounds__(kBlock) void k_fill_one(int64_t n, T *__restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) x[i] = T(1);
 }
 
-// Autotuner: time y = A x (x = 1) for every applicable layout - DIA, SELL-64,
-// row-block CSR with 1024 / 2048 / 4096 nonzeros per workgroup - and keep the
-// fastest plan. Each candidate runs reps times after one untimed launch.
+// A 27-point DIA plan run by the generic DIA kernel instead of the plane-walking one.
+static void dia_drop_walk(uhip_csr *A) {
+    if (A->format != UHIP_FMT_DIA || !A->dia_w27) return;
+    A->dia_w27 = 0;
+    A->dia_s1 = A->dia_s2 = 0;
+    A->nparts = (A->nrows + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
+}
+
+// Autotuner: time y = A x (x = 1) for every applicable layout - DIA (27-point plans with and
+// without the plane-walking kernel), VI, SELL-64, row-block CSR with 1024 / 2048 / 4096
+// nonzeros per workgroup - and keep the fastest plan. Each candidate runs reps times after
+// one untimed launch.
 static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out) {
     struct Cand {
         int32_t format, block_nnz;
+        bool walk;
     };
-    const Cand cands[] = {{UHIP_FMT_DIA, 2048},
-                          {UHIP_FMT_VI, 2048},
-                          {UHIP_FMT_SELL64, 2048},
-                          {UHIP_FMT_ROWBLOCK, 1024},
-                          {UHIP_FMT_ROWBLOCK, 2048},
-                          {UHIP_FMT_ROWBLOCK, 4096}};
+    const Cand cands[] = {{UHIP_FMT_DIA, 2048, true},       {UHIP_FMT_DIA, 2048, false},
+                          {UHIP_FMT_VI, 2048, true},        {UHIP_FMT_SELL64, 2048, true},
+                          {UHIP_FMT_ROWBLOCK, 1024, true},  {UHIP_FMT_ROWBLOCK, 2048, true},
+                          {UHIP_FMT_ROWBLOCK, 4096, true}};
     const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
     void *x = nullptr, *y = nullptr;
     UHIP_CHECK_HIP(hipMalloc(&x, vb * (size_t)std::max(A->ncols, 1)));
@@ -854,12 +874,16 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
     UHIP_CHECK_HIP(hipEventCreate(&e0));
     UHIP_CHECK_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    Cand win{UHIP_FMT_ROWBLOCK, 2048};
+    Cand win{UHIP_FMT_ROWBLOCK, 2048, true};
     int st = UHIP_OK;
     for (const Cand &c : cands) {
         st = uhip_csr_plan_format(ctx, A, c.format, c.block_nnz);
         if (st == UHIP_ERR_UNSUPPORTED) continue;  // DIA does not apply
         if (st != UHIP_OK) break;
+        if (!c.walk) {
+            if (!A->dia_w27) continue;  // the same plan as the candidate before
+            dia_drop_walk(A);
+        }
         auto run = [&]() {
             return A->dtype == UHIP_F64 ? spmv_typed<double, double>(ctx, A, x, y) : spmv_typed<float, float>(ctx, A, x, y);
         };
@@ -891,7 +915,9 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
     hipFree(y);
     if (st != UHIP_OK && st != UHIP_ERR_UNSUPPORTED) return st;
     if (best_ms_out) *best_ms_out = best;
-    return uhip_csr_plan_format(ctx, A, win.format, win.block_nnz);
+    UHIP_TRY(uhip_csr_plan_format(ctx, A, win.format, win.block_nnz));
+    if (!win.walk) dia_drop_walk(A);
+    return UHIP_OK;
 }
 
 }  // namespace uhip
@@ -1026,6 +1052,7 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     A->dia_nan = 0;
     A->dia_s1 = A->dia_s2 = 0;
     A->vi_nsig = 0;
+    A->dia_w27 = 0;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));

@@ -414,6 +414,186 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7r(int32_t nrows, const int3
     }
 }
 
+// planes per wave of the 27-point kernel (k_spmv_dia27r) and planes of loads in flight
+// (measured on the 128^3 Galerkin level: 4 and 1 lead 8 / 1, 8 / 2 and 16 / 1 in FP64, tie in FP32)
+template <typename TA>
+struct Dia27Planes {
+    static constexpr int P = 4;
+    static constexpr int LA = 1;
+};
+inline int dia27_planes(int dtype) { return dtype == UHIP_F64 ? Dia27Planes<double>::P : Dia27Planes<float>::P; }
+
+// 27-point DIA (offsets dz S2 + dy S1 + dx for dz, dy, dx in {-1, 0, 1}, S2 a multiple of 64:
+// a Galerkin operator of a 3D grid), streamed plane by plane as in k_spmv_dia7r: a wave owns 64
+// consecutive rows of each of P consecutive planes and walks the planes, the loads of the next
+// LA planes in flight. x at the nine (dy, dx) neighbours of the plane ahead is loaded once per
+// plane and rolls through registers for the row's two other planes.
+// SYM (the plan keeps the offsets >= 0, values symmetric bit for bit): the -o entry of row r is
+// the stored +o value of row r - o:
+//  - -1, and the -S2 +- 1 pair of the next plane, come with a row's own +1, +S2 - 1 and +S2 + 1
+//    values as 2-vector loads (rows r - 1, r or r, r + 1);
+//  - -S2 is the row's +S2 value one plane back (registers);
+//  - -S1 +- 1, -S1 and the six -S2 +- S1 (+- 1) entries are read through the cache (the waves
+//    owning those rows read them as their own);
+// absent entries are dropped by the row's mask bits. The full layout streams all 27 values and
+// selects on the NaN of absent entries (NANSEL) or on the mask. Every row sum adds the row's
+// CSR entries in column order: bit-identical to the CSR sum.
+template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool SYM, bool NANSEL>
+__global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S1, int32_t S2,
+                                                        const TA *__restrict__ val, const uint32_t *__restrict__ mask,
+                                                        const TX *__restrict__ x, Epi epi,
+                                                        double *__restrict__ partials, int64_t stride) {
+    static_assert(!(SYM && NANSEL), "the symmetric layout selects on the mask");
+    constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
+    constexpr bool TX_IN = EpiTakesX<Epi>::value;
+    constexpr int NV = SYM ? 14 : 27;
+    typedef typename PairOf<TA>::t Pair;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int64_t ld = dia_ld(nrows);
+    const int32_t segs = S2 >> 6;
+    const int32_t wv = b * (kDiaWG / 64) + (int32_t)(threadIdx.x >> 6);
+    const int32_t slab = wv / segs, seg = wv - slab * segs;
+    const int64_t slab0 = (int64_t)slab * P * S2;
+    double part[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) part[d] = 0.0;
+    if (slab0 < nrows) {
+        const int32_t last = (int32_t)ld - 1, xlast = nrows - 1;
+        auto cl = [xlast](int32_t v) { return min(max(v, 0), xlast); };
+        auto V = [&](int k) { return val + (int64_t)k * ld; };
+        const int32_t rb = (int32_t)slab0 + seg * 64 + lane;
+        struct Plane {
+            TA v[NV];
+            TA m1, nm1, np1, my[3], mz[6];  // SYM: mirrors
+            uint32_t m;
+            typename Epi::In ein;
+            TX xn[9];
+        };
+        // values, mirrors and x are read at clamped rows whether or not the entry exists (the
+        // select drops it), so no load waits for another
+        auto load = [&](int z) {
+            Plane q;
+            const int32_t r = rb + z * S2;
+            const int32_t rc = min(r, last);
+            if constexpr (!NANSEL) q.m = r < nrows ? mask[min(r, xlast)] : 0u;
+            else q.m = 0u;
+            if constexpr (SYM) {
+                // stored k: offsets 0, 1, S1 - 1, S1, S1 + 1, S2 - S1 - 1, S2 - S1, S2 - S1 + 1,
+                // S2 - 1, S2, S2 + 1, S2 + S1 - 1, S2 + S1, S2 + S1 + 1
+                q.v[0] = ld_stream(V(0) + rc);
+                const Pair w1 = __builtin_nontemporal_load((const Pair *)(V(1) + max(rc - 1, 0)));
+                q.m1 = w1.x;
+                q.v[1] = rc >= 1 ? w1.y : w1.x;
+                q.v[2] = V(2)[rc];
+                q.v[3] = V(3)[rc];
+                q.v[4] = V(4)[rc];
+                q.my[0] = V(4)[max(rc - S1 - 1, 0)];
+                q.my[1] = V(3)[max(rc - S1, 0)];
+                q.my[2] = V(2)[max(rc - S1 + 1, 0)];
+                q.v[5] = V(5)[rc];
+                q.v[6] = V(6)[rc];
+                q.v[7] = V(7)[rc];
+                const Pair w8 = __builtin_nontemporal_load((const Pair *)(V(8) + min(rc, last - 1)));
+                q.v[8] = rc < last ? w8.x : w8.y;
+                q.np1 = w8.y;
+                q.v[9] = ld_stream(V(9) + rc);
+                const Pair w10 = __builtin_nontemporal_load((const Pair *)(V(10) + max(rc - 1, 0)));
+                q.nm1 = w10.x;
+                q.v[10] = rc >= 1 ? w10.y : w10.x;
+                q.v[11] = V(11)[rc];
+                q.v[12] = V(12)[rc];
+                q.v[13] = V(13)[rc];
+                q.mz[0] = V(13)[max(rc - S2 - S1 - 1, 0)];
+                q.mz[1] = V(12)[max(rc - S2 - S1, 0)];
+                q.mz[2] = V(11)[max(rc - S2 - S1 + 1, 0)];
+                q.mz[3] = V(7)[max(rc - S2 + S1 - 1, 0)];
+                q.mz[4] = V(6)[max(rc - S2 + S1, 0)];
+                q.mz[5] = V(5)[max(rc - S2 + S1 + 1, 0)];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 27; ++d) q.v[d] = ld_stream(V(d) + rc);
+            }
+            if constexpr (TX_IN) q.ein = epi.load_nox(min(r, xlast));
+            else q.ein = epi.load(min(r, xlast));
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) q.xn[3 * (dy + 1) + dx + 1] = x[cl(r + S2 + dy * S1 + dx)];
+            return q;
+        };
+        // the -S2 - 1, -S2 and -S2 + 1 values of the first plane: the previous plane's +S2 + 1
+        // value of row r - S2 - 1, +S2 of row r - S2, +S2 - 1 of row r - S2 + 1
+        TA cnm1 = TA(0), ca9 = TA(0), cnp1 = TA(0);
+        if constexpr (SYM) {
+            const int32_t rp = min(max(rb - S2, 0), last);
+            cnm1 = V(10)[max(rp - 1, 0)];
+            ca9 = V(9)[rp];
+            cnp1 = V(8)[min(rp + 1, last)];
+        }
+        TX xp[9], xc[9];
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                xp[3 * (dy + 1) + dx + 1] = x[cl(rb - S2 + dy * S1 + dx)];
+                xc[3 * (dy + 1) + dx + 1] = x[cl(rb + dy * S1 + dx)];
+            }
+        Plane q[LA + 1];
+#pragma unroll
+        for (int i = 0; i < LA; ++i) q[i] = load(i);
+#pragma unroll(LA >= P ? P : 1)
+        for (int z = 0; z < P; ++z) {
+            if (z + LA < P) q[LA] = load(z + LA);
+            const Plane &c = q[0];
+            TA vals[27];
+            if constexpr (SYM) {
+                const TA lo[13] = {c.mz[0], c.mz[1], c.mz[2], cnm1,    ca9,     cnp1,   c.mz[3],
+                                   c.mz[4], c.mz[5], c.my[0], c.my[1], c.my[2], c.m1};
+#pragma unroll
+                for (int d = 0; d < 13; ++d) vals[d] = lo[d];
+#pragma unroll
+                for (int d = 0; d < 14; ++d) vals[13 + d] = c.v[d];
+            } else {
+#pragma unroll
+                for (int d = 0; d < 27; ++d) vals[d] = c.v[d];
+            }
+            TAcc acc = TAcc(0);
+#pragma unroll
+            for (int d = 0; d < 27; ++d) {
+                const TX xd = d < 9 ? xp[d] : (d < 18 ? xc[d - 9] : c.xn[d - 18]);
+                const TAcc pr = (TAcc)vals[d] * (TAcc)xd;
+                if constexpr (NANSEL) acc = (vals[d] == vals[d]) ? acc + pr : acc;
+                else acc = ((c.m >> d) & 1u) ? acc + pr : acc;
+            }
+            const int32_t r = rb + z * S2;
+            if (r < nrows) {
+                if constexpr (TX_IN) epi.apply_x(r, acc, c.ein, xc[4], part);
+                else epi.apply(r, acc, c.ein, part);
+            }
+            if constexpr (SYM) {
+                cnm1 = c.nm1;
+                ca9 = c.v[9];
+                cnp1 = c.np1;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                xp[k] = xc[k];
+                xc[k] = c.xn[k];
+            }
+#pragma unroll
+            for (int i = 0; i < LA; ++i) q[i] = q[i + 1];
+        }
+    }
+    if constexpr (Epi::kDots > 0) {
+#pragma unroll
+        for (int d = 0; d < Epi::kDots; ++d) {
+            const double t = block_sum<kDiaWG>(part[d]);
+            if (threadIdx.x == 0) partials[d * stride + b] = t;
+        }
+    }
+}
+
 // Rows per lane of the VI kernel: for at most 8 slots 4 (FP32) or 2 (FP64), else 1
 // (a 27-slot restriction row is latency-bound: more rows per lane only lengthen it).
 template <typename TA>
@@ -630,7 +810,16 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
     } while (0)
         // symmetric 7-point storage: the plane-walking kernel (its grid is nparts, the partial
         // sums the finish reads), selecting on NaN-encoded absent entries when the plan has them
-        if (A->dia_sym == 2) {
+        if (A->dia_w27) {
+            constexpr int P = Dia27Planes<TA>::P, LA = Dia27Planes<TA>::LA;
+#define UHIP_DIA27_LAUNCH(SYM, NANSEL)                                                                            \
+    hipLaunchKernelGGL((k_spmv_dia27r<TAcc, Epi, TA, TX, P, LA, SYM, NANSEL>), g, blk, 0, stream_of(ctx), A->nrows, \
+                       A->dia_s1, A->dia_s2, v, (const uint32_t *)A->dia_mask, x, epi, parts, pstride)
+            if (A->dia_sym) UHIP_DIA27_LAUNCH(true, false);
+            else if (A->dia_nan) UHIP_DIA27_LAUNCH(false, true);
+            else UHIP_DIA27_LAUNCH(false, false);
+#undef UHIP_DIA27_LAUNCH
+        } else if (A->dia_sym == 2) {
             constexpr int P = Dia7Planes<TA>::P, LA = Dia7Planes<TA>::LA;
             if (A->dia_nan)
                 hipLaunchKernelGGL((k_spmv_dia7r<TAcc, Epi, TA, TX, P, LA, true>), g, blk, 0, stream_of(ctx), A->nrows,
