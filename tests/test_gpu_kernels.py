@@ -310,6 +310,36 @@ def test_dia27_walk_epilogues(ctx, name, dt):
     assert abs(d1 - float(zr @ dw)) <= bound and abs(d2 - float(zr @ dw)) <= bound
 
 
+def _random27(nx, ny, nz, seed, symmetric):
+    """The 27-point pattern of an nx x ny x nz grid with random values (symmetrized bit for bit
+    when asked)."""
+    T = lambda k: sp.diags([np.ones(k - 1), np.ones(k), np.ones(k - 1)], [-1, 0, 1])  # noqa: E731
+    B = sp.kron(sp.kron(T(nz), T(ny)), T(nx)).tocsr()
+    B.data = np.random.default_rng(seed).uniform(-1.0, 1.0, B.nnz)
+    if symmetric:
+        B = ((B + B.T) * 0.5).tocsr()
+    return ua.Csr.from_scipy(B)
+
+
+@pytest.mark.parametrize("dims,symmetric", [((96, 96, 96), False), ((112, 112, 112), True)])
+def test_dia27_walk_streams_large_levels(ctx, dims, symmetric):
+    """Value arrays above 128 MB are streamed (non-temporal loads) instead of read through the
+    caches: FP64 27-point operators of 0.88 M rows (full layout, 191 MB) and 1.40 M rows
+    (symmetric layout, 157 MB) give the CSR sums bit for bit, plain and as a residual."""
+    A = _random27(*dims, seed=3, symmetric=symmetric)
+    n = A.shape[0]
+    D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_DIA)
+    assert D.m.dia_w27 == 1 and D.m.dia_sym == (1 if symmetric else 0)
+    assert (14 if symmetric else 27) * n * 8 > 128 << 20
+    x, b = ua.gen_uniform(5, n), ua.gen_uniform(6, n)
+    Ax = orc.spmv(A, x)
+    xd, bd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b), ua.DeviceArray(ctx, n)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), Ax)
+    D.residual(xd, bd, yd)
+    assert np.array_equal(yd.numpy(), b - Ax)
+
+
 def test_dia27_autotune_keeps_a_valid_plan(ctx):
     """The autotuner times the 27-point DIA plan with and without the plane walk; whichever it
     keeps computes the CSR sum bit for bit."""

@@ -422,6 +422,10 @@ struct Dia27Planes {
     static constexpr int LA = 1;
 };
 inline int dia27_planes(int dtype) { return dtype == UHIP_F64 ? Dia27Planes<double>::P : Dia27Planes<float>::P; }
+// value arrays up to this size are read through the caches by the 27-point kernel (half the
+// MI355X's 256 MB MALL: the level's vectors share it). Measured at 256^3: the FP32 128^3 level
+// (117 MB) gains 2.3 ms per MG-PCG solve, the FP64 one (453 MB) loses 5.5 ms.
+constexpr int64_t kDiaCachedBytes = int64_t(128) << 20;
 
 // 27-point DIA (offsets dz S2 + dy S1 + dx for dz, dy, dx in {-1, 0, 1}, S2 a multiple of 64:
 // a Galerkin operator of a 3D grid), streamed plane by plane as in k_spmv_dia7r: a wave owns 64
@@ -438,7 +442,10 @@ inline int dia27_planes(int dtype) { return dtype == UHIP_F64 ? Dia27Planes<doub
 // absent entries are dropped by the row's mask bits. The full layout streams all 27 values and
 // selects on the NaN of absent entries (NANSEL) or on the mask. Every row sum adds the row's
 // CSR entries in column order: bit-identical to the CSR sum.
-template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool SYM, bool NANSEL>
+// CACHED: the values are read through the caches instead of streamed (non-temporal), so that
+// value arrays small enough to stay in the 256 MB last-level cache (MALL) are served from it
+// by the next product on the same level (a V-cycle runs four per level).
+template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool SYM, bool NANSEL, bool CACHED>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S1, int32_t S2,
                                                         const TA *__restrict__ val, const uint32_t *__restrict__ mask,
                                                         const TX *__restrict__ x, Epi epi,
@@ -462,6 +469,14 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
         const int32_t last = (int32_t)ld - 1, xlast = nrows - 1;
         auto cl = [xlast](int32_t v) { return min(max(v, 0), xlast); };
         auto V = [&](int k) { return val + (int64_t)k * ld; };
+        auto lv = [](const TA *p) -> TA {
+            if constexpr (CACHED) return *p;
+            else return ld_stream(p);
+        };
+        auto lp = [](const TA *p) -> Pair {
+            if constexpr (CACHED) return *(const Pair *)p;
+            else return __builtin_nontemporal_load((const Pair *)p);
+        };
         const int32_t rb = (int32_t)slab0 + seg * 64 + lane;
         struct Plane {
             TA v[NV];
@@ -481,8 +496,8 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
             if constexpr (SYM) {
                 // stored k: offsets 0, 1, S1 - 1, S1, S1 + 1, S2 - S1 - 1, S2 - S1, S2 - S1 + 1,
                 // S2 - 1, S2, S2 + 1, S2 + S1 - 1, S2 + S1, S2 + S1 + 1
-                q.v[0] = ld_stream(V(0) + rc);
-                const Pair w1 = __builtin_nontemporal_load((const Pair *)(V(1) + max(rc - 1, 0)));
+                q.v[0] = lv(V(0) + rc);
+                const Pair w1 = lp(V(1) + max(rc - 1, 0));
                 q.m1 = w1.x;
                 q.v[1] = rc >= 1 ? w1.y : w1.x;
                 q.v[2] = V(2)[rc];
@@ -494,11 +509,11 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
                 q.v[5] = V(5)[rc];
                 q.v[6] = V(6)[rc];
                 q.v[7] = V(7)[rc];
-                const Pair w8 = __builtin_nontemporal_load((const Pair *)(V(8) + min(rc, last - 1)));
+                const Pair w8 = lp(V(8) + min(rc, last - 1));
                 q.v[8] = rc < last ? w8.x : w8.y;
                 q.np1 = w8.y;
-                q.v[9] = ld_stream(V(9) + rc);
-                const Pair w10 = __builtin_nontemporal_load((const Pair *)(V(10) + max(rc - 1, 0)));
+                q.v[9] = lv(V(9) + rc);
+                const Pair w10 = lp(V(10) + max(rc - 1, 0));
                 q.nm1 = w10.x;
                 q.v[10] = rc >= 1 ? w10.y : w10.x;
                 q.v[11] = V(11)[rc];
@@ -512,7 +527,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
                 q.mz[5] = V(5)[max(rc - S2 + S1 + 1, 0)];
             } else {
 #pragma unroll
-                for (int d = 0; d < 27; ++d) q.v[d] = ld_stream(V(d) + rc);
+                for (int d = 0; d < 27; ++d) q.v[d] = lv(V(d) + rc);
             }
             if constexpr (TX_IN) q.ein = epi.load_nox(min(r, xlast));
             else q.ein = epi.load(min(r, xlast));
@@ -812,13 +827,20 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         // sums the finish reads), selecting on NaN-encoded absent entries when the plan has them
         if (A->dia_w27) {
             constexpr int P = Dia27Planes<TA>::P, LA = Dia27Planes<TA>::LA;
+#define UHIP_DIA27_LAUNCH2(SYM, NANSEL, CACHED)                                                                   \
+    hipLaunchKernelGGL((k_spmv_dia27r<TAcc, Epi, TA, TX, P, LA, SYM, NANSEL, CACHED>), g, blk, 0, stream_of(ctx),   \
+                       A->nrows, A->dia_s1, A->dia_s2, v, (const uint32_t *)A->dia_mask, x, epi, parts, pstride)
 #define UHIP_DIA27_LAUNCH(SYM, NANSEL)                                                                            \
-    hipLaunchKernelGGL((k_spmv_dia27r<TAcc, Epi, TA, TX, P, LA, SYM, NANSEL>), g, blk, 0, stream_of(ctx), A->nrows, \
-                       A->dia_s1, A->dia_s2, v, (const uint32_t *)A->dia_mask, x, epi, parts, pstride)
+    do {                                                                                                          \
+        if (cached) UHIP_DIA27_LAUNCH2(SYM, NANSEL, true);                                                        \
+        else UHIP_DIA27_LAUNCH2(SYM, NANSEL, false);                                                              \
+    } while (0)
+            const bool cached = (int64_t)(A->dia_sym ? 14 : 27) * dia_ld(A->nrows) * (int64_t)sizeof(TA) <= kDiaCachedBytes;
             if (A->dia_sym) UHIP_DIA27_LAUNCH(true, false);
             else if (A->dia_nan) UHIP_DIA27_LAUNCH(false, true);
             else UHIP_DIA27_LAUNCH(false, false);
 #undef UHIP_DIA27_LAUNCH
+#undef UHIP_DIA27_LAUNCH2
         } else if (A->dia_sym == 2) {
             constexpr int P = Dia7Planes<TA>::P, LA = Dia7Planes<TA>::LA;
             if (A->dia_nan)
