@@ -199,19 +199,29 @@ def _with_nan(A, row):
     return ua.Csr(A.rowptr, A.col, val, A.shape)
 
 
+def _sign_flipped(A, row):
+    """A with the first (lower) entry of `row` negated: no longer symmetric up to rounding."""
+    val = A.val.copy()
+    assert A.col[A.rowptr[row]] < row
+    val[A.rowptr[row]] = -val[A.rowptr[row]]
+    return ua.Csr(A.rowptr, A.col, val, A.shape)
+
+
 # 27-point operators whose planes are whole multiples of 64 rows (the plane-walking kernel):
-# (coarse grid, expected dia_sym in FP64). Galerkin products are symmetric only to rounding
-# (full layout, NaN-encoded absent entries); symmetrized ones and the constant box stencil
-# take the symmetric layout. Plane counts leave partial 8-plane slabs; 16 x 16 and 32 x 32
-# planes span several 64-row segments (the mirrors one y line and one plane back come from
-# other waves).
+# (coarse grid, expected FP64 layout: "full", "sym", or "delta" = the upper half plus the lower
+# values' bit-pattern differences from their mirrors). Galerkin products are symmetric only to
+# rounding (delta); symmetrized ones and the constant box stencil are symmetric bit for bit; a
+# NaN value or a lower entry of the wrong sign keeps the full layout. Plane counts leave
+# partial 4-plane slabs; 16 x 16 and 32 x 32 planes span several 64-row segments (the mirrors
+# one y line and one plane back come from other waves).
 DIA27 = {
-    "galerkin_8x8x11": (lambda: _galerkin_dims(16, 16, 22), 0),
-    "galerkin_16x16x20": (lambda: _galerkin_dims(32, 32, 40), 0),
-    "galerkin_32x32x6": (lambda: _galerkin_dims(64, 64, 12), 0),
-    "galerkin_sym_16x8x10": (lambda: _symmetrized(_galerkin_dims(32, 16, 20)), 1),
-    "box27_16x8x13": (lambda: _box27(16, 8, 13), 1),
-    "galerkin_nan_8x8x11": (lambda: _with_nan(_galerkin_dims(16, 16, 22), 77), 0),
+    "galerkin_8x8x11": (lambda: _galerkin_dims(16, 16, 22), "delta"),
+    "galerkin_16x16x20": (lambda: _galerkin_dims(32, 32, 40), "delta"),
+    "galerkin_32x32x6": (lambda: _galerkin_dims(64, 64, 12), "delta"),
+    "galerkin_sym_16x8x10": (lambda: _symmetrized(_galerkin_dims(32, 16, 20)), "sym"),
+    "box27_16x8x13": (lambda: _box27(16, 8, 13), "sym"),
+    "galerkin_nan_8x8x11": (lambda: _with_nan(_galerkin_dims(16, 16, 22), 77), "full"),
+    "galerkin_signflip_16x16x20": (lambda: _sign_flipped(_galerkin_dims(32, 32, 40), 1000), "full"),
 }
 
 
@@ -219,16 +229,20 @@ DIA27 = {
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_spmv_dia27_walk_bit_exact(ctx, name, dt):
     """The plane-walking 27-point kernel (full layout selecting on NaN-encoded absent entries or
-    on the mask, symmetric layout with mirrors): y = A x equal to the CSR sum bit for bit; a NaN
-    entry keeps the mask and gives NaN exactly in its row."""
-    make, sym = DIA27[name]
+    on the mask, symmetric layout with mirrors, the upper half with the lower values' bit-pattern
+    differences): y = A x equal to the CSR sum bit for bit; a NaN entry keeps the mask and gives
+    NaN exactly in its row."""
+    make, layout = DIA27[name]
     A = make()
     n = A.shape[0]
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.format == ua.FMT_DIA and D.m.ndiag == 27 and D.m.dia_w27 == 1
     if dt is np.float64:
-        assert D.m.dia_sym == sym
+        assert D.m.dia_sym == (0 if layout == "full" else 1)
+        assert bool(D.m.dia_delta) == (layout == "delta")
         assert D.m.dia_nan == (0 if "nan" in name else 1)
+    else:
+        assert not D.m.dia_delta
     x = ua.gen_uniform(17, n)
     if dt is np.float32:
         x = x.astype(np.float32)
@@ -240,7 +254,8 @@ def test_spmv_dia27_walk_bit_exact(ctx, name, dt):
     assert np.array_equal(yd.numpy(), ref, equal_nan=True)
 
 
-@pytest.mark.parametrize("name", ["galerkin_16x16x20", "galerkin_sym_16x8x10", "box27_16x8x13"])
+@pytest.mark.parametrize("name", ["galerkin_16x16x20", "galerkin_sym_16x8x10", "box27_16x8x13",
+                                  "galerkin_signflip_16x16x20"])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_dia27_walk_epilogues(ctx, name, dt):
     """Every fused epilogue on the 27-point kernel against the row-block plan of the same matrix
@@ -310,27 +325,37 @@ def test_dia27_walk_epilogues(ctx, name, dt):
     assert abs(d1 - float(zr @ dw)) <= bound and abs(d2 - float(zr @ dw)) <= bound
 
 
-def _random27(nx, ny, nz, seed, symmetric):
-    """The 27-point pattern of an nx x ny x nz grid with random values (symmetrized bit for bit
-    when asked)."""
+def _random27(nx, ny, nz, seed, layout):
+    """The 27-point pattern of an nx x ny x nz grid with random values: "full" as drawn, "sym"
+    symmetrized bit for bit, "delta" symmetrized and then each lower value moved by up to 3 ulps."""
     T = lambda k: sp.diags([np.ones(k - 1), np.ones(k), np.ones(k - 1)], [-1, 0, 1])  # noqa: E731
     B = sp.kron(sp.kron(T(nz), T(ny)), T(nx)).tocsr()
-    B.data = np.random.default_rng(seed).uniform(-1.0, 1.0, B.nnz)
-    if symmetric:
+    rng = np.random.default_rng(seed)
+    B.data = rng.uniform(-1.0, 1.0, B.nnz)
+    if layout != "full":
         B = ((B + B.T) * 0.5).tocsr()
+        B.sort_indices()
+    if layout == "delta":
+        rows = np.repeat(np.arange(B.shape[0]), np.diff(B.indptr))
+        lower = B.indices < rows
+        bits = B.data.view(np.int64)
+        bits[lower] += rng.integers(-3, 4, int(lower.sum()))
     return ua.Csr.from_scipy(B)
 
 
-@pytest.mark.parametrize("dims,symmetric", [((96, 96, 96), False), ((112, 112, 112), True)])
-def test_dia27_walk_streams_large_levels(ctx, dims, symmetric):
+@pytest.mark.parametrize("dims,layout", [((96, 96, 96), "full"), ((112, 112, 112), "sym"),
+                                         ((112, 112, 112), "delta")])
+def test_dia27_walk_streams_large_levels(ctx, dims, layout):
     """Value arrays above 128 MB are streamed (non-temporal loads) instead of read through the
     caches: FP64 27-point operators of 0.88 M rows (full layout, 191 MB) and 1.40 M rows
-    (symmetric layout, 157 MB) give the CSR sums bit for bit, plain and as a residual."""
-    A = _random27(*dims, seed=3, symmetric=symmetric)
+    (symmetric layout, 157 MB, with and without lower-triangle deltas) give the CSR sums bit
+    for bit, plain and as a residual."""
+    A = _random27(*dims, seed=3, layout=layout)
     n = A.shape[0]
     D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_DIA)
-    assert D.m.dia_w27 == 1 and D.m.dia_sym == (1 if symmetric else 0)
-    assert (14 if symmetric else 27) * n * 8 > 128 << 20
+    assert D.m.dia_w27 == 1 and D.m.dia_sym == (0 if layout == "full" else 1)
+    assert bool(D.m.dia_delta) == (layout == "delta")
+    assert (27 if layout == "full" else 14) * n * 8 > 128 << 20
     x, b = ua.gen_uniform(5, n), ua.gen_uniform(6, n)
     Ax = orc.spmv(A, x)
     xd, bd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b), ua.DeviceArray(ctx, n)

@@ -55,6 +55,7 @@ class uhip_csr(ctypes.Structure):
         ("dia_s2", ctypes.c_int32),
         ("vi_nsig", ctypes.c_int32),
         ("dia_w27", ctypes.c_int32),
+        ("dia_delta", ctypes.c_void_p),
     ]
 
 

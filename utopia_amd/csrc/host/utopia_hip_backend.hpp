@@ -505,6 +505,7 @@ public:
         m_.dia_s1 = m_.dia_s2 = 0;
         m_.vi_nsig = 0;
         m_.dia_w27 = 0;
+        m_.dia_delta = nullptr;
         m_.format = UHIP_FMT_ROWBLOCK;
         plan(block_nnz);
     }

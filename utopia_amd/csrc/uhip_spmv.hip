@@ -280,6 +280,55 @@ __global__ __launch_bounds__(kBlock) void k_dia_symcheck(int32_t n, int32_t ndia
     if (!ok) flag[0] = 1;
 }
 
+// Near-symmetric FP64 values (a Galerkin product, symmetric up to rounding): flag[0] = 1 unless
+// every stored lower entry (r, r - o) has a stored mirror (r - o, r) of the same sign whose bit
+// pattern differs from its own by a 32-bit integer.
+__global__ __launch_bounds__(kBlock) void k_dia_deltacheck(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
+                                                           const double *__restrict__ dval,
+                                                           const uint32_t *__restrict__ mask,
+                                                           int32_t *__restrict__ flag) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t ld = dia_ld(n);
+    const int32_t k = ndiag >> 1;
+    const uint32_t m = mask[r];
+    bool ok = true;
+    for (int d = 0; d < k; ++d) {
+        if (!((m >> d) & 1u)) continue;
+        const int32_t rm = r + off[d], md = ndiag - 1 - d;
+        if (rm < 0 || !((mask[rm] >> md) & 1u)) {
+            ok = false;
+            continue;
+        }
+        const long long a = __double_as_longlong(dval[(int64_t)d * ld + r]);
+        const long long b = __double_as_longlong(dval[(int64_t)md * ld + rm]);
+        const long long e = a - b;
+        if ((a ^ b) < 0 || e < INT32_MIN || e > INT32_MAX) ok = false;
+    }
+    if (!ok) flag[0] = 1;
+}
+
+// delta[d][r] = bits(value of lower entry d of row r) - bits(its mirror), 0 where absent
+__global__ __launch_bounds__(kBlock) void k_dia_deltafill(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
+                                                          const double *__restrict__ dval,
+                                                          const uint32_t *__restrict__ mask,
+                                                          int32_t *__restrict__ delta) {
+    const int32_t r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t ld = dia_ld(n);
+    const int32_t k = ndiag >> 1;
+    const uint32_t m = mask[r];
+    for (int d = 0; d < k; ++d) {
+        int32_t e = 0;
+        if ((m >> d) & 1u) {
+            const int32_t rm = r + off[d], md = ndiag - 1 - d;
+            e = (int32_t)(__double_as_longlong(dval[(int64_t)d * ld + r]) -
+                          __double_as_longlong(dval[(int64_t)md * ld + rm]));
+        }
+        delta[(int64_t)d * ld + r] = e;
+    }
+}
+
 // Plan DIA arrays; UHIP_ERR_UNSUPPORTED (nothing allocated) when the matrix is
 // not square, has unsorted or duplicate columns in a row, more than 32 distinct
 // offsets, or ndiag * n > max_ratio * nnz.
@@ -346,6 +395,16 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
 #undef UHIP_DIA_ZERO
         UHIP_CHECK_LAUNCH();
     }
+    // the 27 offsets dz S2 + dy S1 + dx in this (sorted) order, S2 a multiple of 64: the
+    // plane-walking 27-point kernel
+    bool w27 = false;
+    if (nd == 27) {
+        const int32_t S1 = offs[16], S2 = offs[22];
+        w27 = S1 > 2 && S2 > 2 * S1 + 2 && S2 % 64 == 0 &&
+              (int64_t)n + (int64_t)(dia27_planes(A->dtype) + 2) * S2 < INT32_MAX;
+        for (int i = 0; i < 27 && w27; ++i)
+            w27 = offs[(size_t)i] == (i / 9 - 1) * S2 + (i / 3 % 3 - 1) * S1 + (i % 3 - 1);
+    }
     // symmetric offsets -o..0..+o: keep only 0..+o when the values are symmetric bit for bit
     bool sym_offs = (nd & 1) && offs[(size_t)nd / 2] == 0;
     for (int32_t d = 0; d < nd && sym_offs; ++d) sym_offs = offs[(size_t)d] == -offs[(size_t)(nd - 1 - d)];
@@ -379,6 +438,32 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
             UHIP_CHECK_HIP(hipFree(A->dia_val));
             A->dia_val = half;
             sym = 1;
+        } else if (w27 && A->dtype == UHIP_F64 && !has_nan) {
+            // symmetric up to rounding: the upper half plus each lower value's bit-pattern
+            // difference from its mirror (32-bit), for the 27-point kernel only
+            UHIP_CHECK_HIP(hipMemsetAsync(flags, 0, sizeof(int32_t), s));
+            hipLaunchKernelGGL(k_dia_deltacheck, g, b, 0, s, n, nd, table, (const double *)A->dia_val,
+                               (const uint32_t *)A->dia_mask, flags);
+            UHIP_CHECK_LAUNCH();
+            int32_t far = 1;
+            UHIP_CHECK_HIP(hipMemcpyAsync(&far, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            UHIP_CHECK_HIP(hipStreamSynchronize(s));
+            if (far == 0) {
+                const size_t ld = (size_t)dia_ld(n), keep = (size_t)(nd / 2 + 1);
+                void *half = nullptr, *delta = nullptr;
+                UHIP_CHECK_HIP(hipMalloc(&half, vb * keep * ld));
+                UHIP_CHECK_HIP(hipMalloc(&delta, sizeof(int32_t) * (size_t)(nd / 2) * ld));
+                hipLaunchKernelGGL(k_dia_deltafill, g, b, 0, s, n, nd, table, (const double *)A->dia_val,
+                                   (const uint32_t *)A->dia_mask, (int32_t *)delta);
+                UHIP_CHECK_LAUNCH();
+                UHIP_CHECK_HIP(hipMemcpyAsync(half, (const char *)A->dia_val + vb * (size_t)(nd / 2) * ld,
+                                              vb * keep * ld, hipMemcpyDeviceToDevice, s));
+                UHIP_CHECK_HIP(hipStreamSynchronize(s));
+                UHIP_CHECK_HIP(hipFree(A->dia_val));
+                A->dia_val = half;
+                A->dia_delta = delta;
+                sym = 1;
+            }
         }
     }
     UHIP_CHECK_HIP(hipStreamSynchronize(s));
@@ -388,16 +473,6 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
     if (sym && nd == 7 && offs[4] == 1 && offs[5] > 1 && offs[6] > offs[5] && offs[6] % 64 == 0 &&
         (int64_t)n + (int64_t)planes * offs[6] < INT32_MAX)
         sym = 2;
-    // the 27 offsets dz S2 + dy S1 + dx in this (sorted) order, S2 a multiple of 64: the
-    // plane-walking 27-point kernel
-    bool w27 = false;
-    if (nd == 27) {
-        const int32_t S1 = offs[16], S2 = offs[22];
-        w27 = S1 > 2 && S2 > 2 * S1 + 2 && S2 % 64 == 0 &&
-              (int64_t)n + (int64_t)(dia27_planes(A->dtype) + 2) * S2 < INT32_MAX;
-        for (int i = 0; i < 27 && w27; ++i)
-            w27 = offs[(size_t)i] == (i / 9 - 1) * S2 + (i / 3 % 3 - 1) * S1 + (i % 3 - 1);
-    }
     A->dia_off = table;  // first nd slots hold the sorted offsets
     A->ndiag = nd;
     A->dia_sym = sym;
@@ -843,11 +918,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_one(int64_t n, T *__restrict__ 
 }
 
 // A 27-point DIA plan run by the generic DIA kernel instead of the plane-walking one.
-static void dia_drop_walk(uhip_csr *A) {
-    if (A->format != UHIP_FMT_DIA || !A->dia_w27) return;
+static bool dia_drop_walk(uhip_csr *A) {
+    if (A->format != UHIP_FMT_DIA || !A->dia_w27) return true;
+    if (A->dia_delta) return false;  // the lower-triangle deltas are read by the 27-point kernel only
     A->dia_w27 = 0;
     A->dia_s1 = A->dia_s2 = 0;
     A->nparts = (A->nrows + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
+    return true;
 }
 
 // Autotuner: time y = A x (x = 1) for every applicable layout - DIA (27-point plans with and
@@ -881,8 +958,8 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
         if (st == UHIP_ERR_UNSUPPORTED) continue;  // DIA does not apply
         if (st != UHIP_OK) break;
         if (!c.walk) {
-            if (!A->dia_w27) continue;  // the same plan as the candidate before
-            dia_drop_walk(A);
+            if (!A->dia_w27) continue;        // the same plan as the candidate before
+            if (!dia_drop_walk(A)) continue;  // a layout only the plane walk reads
         }
         auto run = [&]() {
             return A->dtype == UHIP_F64 ? spmv_typed<double, double>(ctx, A, x, y) : spmv_typed<float, float>(ctx, A, x, y);
@@ -1033,7 +1110,7 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
         if (st != UHIP_OK) return st;
     }
     if (A->blocks || A->slice_off || A->sell_col || A->sell_val || A->dia_off || A->dia_val || A->dia_mask ||
-        A->vi_base || A->vi_table) {
+        A->dia_delta || A->vi_base || A->vi_table) {
         if (ctx) UHIP_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     }
     if (A->dia_off) UHIP_CHECK_HIP(hipFree(A->dia_off));
@@ -1053,6 +1130,8 @@ int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     A->dia_s1 = A->dia_s2 = 0;
     A->vi_nsig = 0;
     A->dia_w27 = 0;
+    if (A->dia_delta) UHIP_CHECK_HIP(hipFree(A->dia_delta));
+    A->dia_delta = nullptr;
     if (A->blocks) UHIP_CHECK_HIP(hipFree(A->blocks));
     if (A->slice_off) UHIP_CHECK_HIP(hipFree(A->slice_off));
     if (A->sell_col) UHIP_CHECK_HIP(hipFree(A->sell_col));

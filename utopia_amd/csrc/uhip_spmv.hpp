@@ -445,12 +445,16 @@ constexpr int64_t kDiaCachedBytes = int64_t(128) << 20;
 // CACHED: the values are read through the caches instead of streamed (non-temporal), so that
 // value arrays small enough to stay in the 256 MB last-level cache (MALL) are served from it
 // by the next product on the same level (a V-cycle runs four per level).
-template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool SYM, bool NANSEL, bool CACHED>
+// DELTA (SYM, FP64 values symmetric only up to rounding): lower entry d of row r is its mirror
+// with delta[d][r] added to the bit pattern, exact.
+template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool SYM, bool NANSEL, bool CACHED,
+          bool DELTA = false>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S1, int32_t S2,
                                                         const TA *__restrict__ val, const uint32_t *__restrict__ mask,
-                                                        const TX *__restrict__ x, Epi epi,
-                                                        double *__restrict__ partials, int64_t stride) {
+                                                        const int32_t *__restrict__ delta, const TX *__restrict__ x,
+                                                        Epi epi, double *__restrict__ partials, int64_t stride) {
     static_assert(!(SYM && NANSEL), "the symmetric layout selects on the mask");
+    static_assert(!DELTA || (SYM && sizeof(TA) == 8), "bit-pattern deltas belong to FP64 symmetric storage");
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
     constexpr bool TX_IN = EpiTakesX<Epi>::value;
     constexpr int NV = SYM ? 14 : 27;
@@ -481,6 +485,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
         struct Plane {
             TA v[NV];
             TA m1, nm1, np1, my[3], mz[6];  // SYM: mirrors
+            int32_t dl[DELTA ? 13 : 1];      // DELTA: the lower entries' bit-pattern differences
             uint32_t m;
             typename Epi::In ein;
             TX xn[9];
@@ -525,6 +530,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
                 q.mz[3] = V(7)[max(rc - S2 + S1 - 1, 0)];
                 q.mz[4] = V(6)[max(rc - S2 + S1, 0)];
                 q.mz[5] = V(5)[max(rc - S2 + S1 + 1, 0)];
+                if constexpr (DELTA)
+#pragma unroll
+                    for (int d = 0; d < 13; ++d) q.dl[d] = ld_stream(delta + (int64_t)d * ld + rc);
             } else {
 #pragma unroll
                 for (int d = 0; d < 27; ++d) q.v[d] = lv(V(d) + rc);
@@ -566,7 +574,11 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
                 const TA lo[13] = {c.mz[0], c.mz[1], c.mz[2], cnm1,    ca9,     cnp1,   c.mz[3],
                                    c.mz[4], c.mz[5], c.my[0], c.my[1], c.my[2], c.m1};
 #pragma unroll
-                for (int d = 0; d < 13; ++d) vals[d] = lo[d];
+                for (int d = 0; d < 13; ++d) {
+                    if constexpr (DELTA)
+                        vals[d] = __longlong_as_double(__double_as_longlong(lo[d]) + (long long)c.dl[d]);
+                    else vals[d] = lo[d];
+                }
 #pragma unroll
                 for (int d = 0; d < 14; ++d) vals[13 + d] = c.v[d];
             } else {
@@ -809,6 +821,8 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
     if (A && A->format == UHIP_FMT_DIA) {
         UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->ndiag >= 1 && A->ndiag <= 32,
                      "spmv: DIA plan missing");
+        UHIP_REQUIRE(!A->dia_delta || (A->dia_w27 && A->dia_sym == 1 && A->dtype == UHIP_F64),
+                     "spmv: DIA lower-triangle deltas need the FP64 27-point kernel");
         if (A->nrows == 0) return UHIP_OK;
         UHIP_TRY(partials_for(A->nparts));
         double *parts = parts_in ? parts_in : ctx->partials;
@@ -827,20 +841,28 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
         // sums the finish reads), selecting on NaN-encoded absent entries when the plan has them
         if (A->dia_w27) {
             constexpr int P = Dia27Planes<TA>::P, LA = Dia27Planes<TA>::LA;
-#define UHIP_DIA27_LAUNCH2(SYM, NANSEL, CACHED)                                                                   \
-    hipLaunchKernelGGL((k_spmv_dia27r<TAcc, Epi, TA, TX, P, LA, SYM, NANSEL, CACHED>), g, blk, 0, stream_of(ctx),   \
-                       A->nrows, A->dia_s1, A->dia_s2, v, (const uint32_t *)A->dia_mask, x, epi, parts, pstride)
+#define UHIP_DIA27_LAUNCH3(SYM, NANSEL, CACHED, DELTA)                                                            \
+    hipLaunchKernelGGL((k_spmv_dia27r<TAcc, Epi, TA, TX, P, LA, SYM, NANSEL, CACHED, DELTA>), g, blk, 0,            \
+                       stream_of(ctx), A->nrows, A->dia_s1, A->dia_s2, v, (const uint32_t *)A->dia_mask,          \
+                       (const int32_t *)A->dia_delta, x, epi, parts, pstride)
+#define UHIP_DIA27_LAUNCH2(SYM, NANSEL, CACHED) UHIP_DIA27_LAUNCH3(SYM, NANSEL, CACHED, false)
 #define UHIP_DIA27_LAUNCH(SYM, NANSEL)                                                                            \
     do {                                                                                                          \
         if (cached) UHIP_DIA27_LAUNCH2(SYM, NANSEL, true);                                                        \
         else UHIP_DIA27_LAUNCH2(SYM, NANSEL, false);                                                              \
     } while (0)
             const bool cached = (int64_t)(A->dia_sym ? 14 : 27) * dia_ld(A->nrows) * (int64_t)sizeof(TA) <= kDiaCachedBytes;
-            if (A->dia_sym) UHIP_DIA27_LAUNCH(true, false);
+            if (A->dia_delta) {
+                if constexpr (sizeof(TA) == 8) {
+                    if (cached) UHIP_DIA27_LAUNCH3(true, false, true, true);
+                    else UHIP_DIA27_LAUNCH3(true, false, false, true);
+                }
+            } else if (A->dia_sym) UHIP_DIA27_LAUNCH(true, false);
             else if (A->dia_nan) UHIP_DIA27_LAUNCH(false, true);
             else UHIP_DIA27_LAUNCH(false, false);
 #undef UHIP_DIA27_LAUNCH
 #undef UHIP_DIA27_LAUNCH2
+#undef UHIP_DIA27_LAUNCH3
         } else if (A->dia_sym == 2) {
             constexpr int P = Dia7Planes<TA>::P, LA = Dia7Planes<TA>::LA;
             if (A->dia_nan)
