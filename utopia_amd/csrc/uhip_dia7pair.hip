@@ -167,7 +167,10 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
                 acc = (hv[kk][d] == hv[kk][d]) ? acc + pr : acc;
             }
             if constexpr (MODE == 0) {
-                st_stream(out2 + r, (T)(hx[kk] - acc));
+                // r_R is read next by the restriction: in FP32 (67 MB at 256^3) a cached store
+                // leaves it in the MALL for that read (measured: mixed MG-PCG -0.7 ms)
+                if constexpr (sizeof(T) == 4) out2[r] = (T)(hx[kk] - acc);
+                else st_stream(out2 + r, (T)(hx[kk] - acc));
             } else {
                 const T pp = c0[i];
                 const T rk = -pp + a_prev * acc;

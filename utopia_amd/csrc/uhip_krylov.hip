@@ -79,14 +79,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_residual(int64_t n, const double 
         for (int u = 0; u < 4; ++u) {
             const double v = rv[u] - alpha * av[u];
             __builtin_nontemporal_store(v, rn + i + u * st);
-            if (rn32) __builtin_nontemporal_store((float)v, rn32 + i + u * st);
+            // the FP32 copy is the preconditioner's next input: a cached store keeps it in the MALL
+            if (rn32) rn32[i + u * st] = (float)v;
             acc += v * v;
         }
     }
     for (; i < n; i += st) {
         const double v = r[i] - alpha * Ap[i];
         __builtin_nontemporal_store(v, rn + i);
-        if (rn32) __builtin_nontemporal_store((float)v, rn32 + i);
+        if (rn32) rn32[i] = (float)v;
         acc += v * v;
     }
     acc = block_sum<kBlock>(acc);
