@@ -229,11 +229,11 @@ typedef struct uhip_csr {
     int32_t dia_w27;    /* DIA: 1 when the offsets are the 27 of dz S2 + dy S1 + dx (dz, dy, dx in
                          * {-1, 0, 1}, S2 a multiple of 64) and the plane-walking 27-point kernel
                          * runs (full or symmetric layout) */
-    void *dia_delta;    /* DIA, FP64, 27-point walk, dia_sym 1: values symmetric only up to rounding
+    void *dia_delta;    /* DIA, 27-point walk, dia_sym 1: values symmetric only up to rounding
                          * (a Galerkin product). dia_val holds the upper half; dia_delta the 13
-                         * lower diagonals as int32 differences of each value's bit pattern from
-                         * its mirror's (dia_ld(n) per diagonal), so every value is exact. NULL
-                         * otherwise */
+                         * lower diagonals as differences of each value's bit pattern from its
+                         * mirror's (int32 for FP64, int8 for FP32; dia_ld(n) per diagonal), so
+                         * every value is exact. NULL otherwise */
 } uhip_csr;
 
 #define UHIP_FMT_AUTO (-1)

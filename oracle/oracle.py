@@ -159,6 +159,8 @@ def transpose(A):
 def spgemm(A, B):
     out = or_csr()
     MA, MB = Mat.of(A), Mat.of(B)
+    if MA.shape[1] != MB.shape[0]:
+        raise ValueError(f"spgemm: shapes {MA.shape} and {MB.shape} do not chain")
     lib().or_spgemm(MA.ref, MB.ref, ctypes.byref(out))
     return _from_owned(out)
 
@@ -166,6 +168,8 @@ def spgemm(A, B):
 def ptap(A, P):
     out = or_csr()
     MA, MP = Mat.of(A), Mat.of(P)
+    if MA.shape[0] != MA.shape[1] or MA.shape[1] != MP.shape[0]:
+        raise ValueError(f"ptap: A {MA.shape} is not square or does not match P {MP.shape}")
     lib().or_ptap(MA.ref, MP.ref, ctypes.byref(out))
     return _from_owned(out)
 

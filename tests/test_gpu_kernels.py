@@ -222,6 +222,7 @@ DIA27 = {
     "box27_16x8x13": (lambda: _box27(16, 8, 13), "sym"),
     "galerkin_nan_8x8x11": (lambda: _with_nan(_galerkin_dims(16, 16, 22), 77), "full"),
     "galerkin_signflip_16x16x20": (lambda: _sign_flipped(_galerkin_dims(32, 32, 40), 1000), "full"),
+    "random_delta32_16x16x20": (lambda: _random27(16, 16, 20, seed=4, layout="delta32"), "delta"),
 }
 
 
@@ -237,12 +238,12 @@ def test_spmv_dia27_walk_bit_exact(ctx, name, dt):
     n = A.shape[0]
     D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.format == ua.FMT_DIA and D.m.ndiag == 27 and D.m.dia_w27 == 1
+    assert D.m.dia_sym == (0 if layout == "full" else 1)
     if dt is np.float64:
-        assert D.m.dia_sym == (0 if layout == "full" else 1)
         assert bool(D.m.dia_delta) == (layout == "delta")
         assert D.m.dia_nan == (0 if "nan" in name else 1)
-    else:
-        assert not D.m.dia_delta
+    elif "delta32" in name:
+        assert D.m.dia_delta  # FP32 values a few ulps apart: int8 deltas
     x = ua.gen_uniform(17, n)
     if dt is np.float32:
         x = x.astype(np.float32)
@@ -255,7 +256,7 @@ def test_spmv_dia27_walk_bit_exact(ctx, name, dt):
 
 
 @pytest.mark.parametrize("name", ["galerkin_16x16x20", "galerkin_sym_16x8x10", "box27_16x8x13",
-                                  "galerkin_signflip_16x16x20"])
+                                  "galerkin_signflip_16x16x20", "random_delta32_16x16x20"])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_dia27_walk_epilogues(ctx, name, dt):
     """Every fused epilogue on the 27-point kernel against the row-block plan of the same matrix
@@ -327,7 +328,8 @@ def test_dia27_walk_epilogues(ctx, name, dt):
 
 def _random27(nx, ny, nz, seed, layout):
     """The 27-point pattern of an nx x ny x nz grid with random values: "full" as drawn, "sym"
-    symmetrized bit for bit, "delta" symmetrized and then each lower value moved by up to 3 ulps."""
+    symmetrized bit for bit, "delta" symmetrized and then each lower value moved by up to 3 ulps,
+    "delta32" the same with FP32 values moved by up to 3 FP32 ulps (held exactly in FP64)."""
     T = lambda k: sp.diags([np.ones(k - 1), np.ones(k), np.ones(k - 1)], [-1, 0, 1])  # noqa: E731
     B = sp.kron(sp.kron(T(nz), T(ny)), T(nx)).tocsr()
     rng = np.random.default_rng(seed)
@@ -335,11 +337,15 @@ def _random27(nx, ny, nz, seed, layout):
     if layout != "full":
         B = ((B + B.T) * 0.5).tocsr()
         B.sort_indices()
+    rows = np.repeat(np.arange(B.shape[0]), np.diff(B.indptr))
+    lower = B.indices < rows
     if layout == "delta":
-        rows = np.repeat(np.arange(B.shape[0]), np.diff(B.indptr))
-        lower = B.indices < rows
         bits = B.data.view(np.int64)
         bits[lower] += rng.integers(-3, 4, int(lower.sum()))
+    elif layout == "delta32":
+        v32 = B.data.astype(np.float32)
+        v32.view(np.int32)[lower] += rng.integers(-3, 4, int(lower.sum())).astype(np.int32)
+        B.data = v32.astype(np.float64)
     return ua.Csr.from_scipy(B)
 
 

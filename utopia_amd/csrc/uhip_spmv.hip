@@ -9,6 +9,8 @@
 
 #include "uhip_spmv.hpp"
 
+#include <limits>
+
 namespace uhip {
 
 // ---- partial-sum scratch of uhip_spmv_dot_noalloc ---------------------------------
@@ -280,13 +282,28 @@ __global__ __launch_bounds__(kBlock) void k_dia_symcheck(int32_t n, int32_t ndia
     if (!ok) flag[0] = 1;
 }
 
-// Near-symmetric FP64 values (a Galerkin product, symmetric up to rounding): flag[0] = 1 unless
-// every stored lower entry (r, r - o) has a stored mirror (r - o, r) of the same sign whose bit
-// pattern differs from its own by a 32-bit integer.
+// Bit patterns of FP64 / FP32 values as signed integers, and the delta type of each: int32 for
+// FP64, int8 for FP32 (the FP32 copy of a near-symmetric operator differs by an ulp or two).
+__device__ __forceinline__ long long value_bits(double v) { return __double_as_longlong(v); }
+__device__ __forceinline__ long long value_bits(float v) { return (long long)__float_as_int(v); }
+template <typename T>
+struct DeltaOf {
+    typedef int32_t t;
+};
+template <>
+struct DeltaOf<float> {
+    typedef int8_t t;
+};
+
+// Near-symmetric values (a Galerkin product, symmetric up to rounding): flag[0] = 1 unless every
+// stored lower entry (r, r - o) has a stored mirror (r - o, r) of the same sign whose bit pattern
+// differs from its own by a value of the delta type.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dia_deltacheck(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
-                                                           const double *__restrict__ dval,
+                                                           const T *__restrict__ dval,
                                                            const uint32_t *__restrict__ mask,
                                                            int32_t *__restrict__ flag) {
+    typedef typename DeltaOf<T>::t TD;
     const int32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= n) return;
     const int64_t ld = dia_ld(n);
@@ -300,30 +317,31 @@ __global__ __launch_bounds__(kBlock) void k_dia_deltacheck(int32_t n, int32_t nd
             ok = false;
             continue;
         }
-        const long long a = __double_as_longlong(dval[(int64_t)d * ld + r]);
-        const long long b = __double_as_longlong(dval[(int64_t)md * ld + rm]);
+        const long long a = value_bits(dval[(int64_t)d * ld + r]);
+        const long long b = value_bits(dval[(int64_t)md * ld + rm]);
         const long long e = a - b;
-        if ((a ^ b) < 0 || e < INT32_MIN || e > INT32_MAX) ok = false;
+        if ((a ^ b) < 0 || e < std::numeric_limits<TD>::min() || e > std::numeric_limits<TD>::max()) ok = false;
     }
     if (!ok) flag[0] = 1;
 }
 
 // delta[d][r] = bits(value of lower entry d of row r) - bits(its mirror), 0 where absent
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dia_deltafill(int32_t n, int32_t ndiag, const int32_t *__restrict__ off,
-                                                          const double *__restrict__ dval,
+                                                          const T *__restrict__ dval,
                                                           const uint32_t *__restrict__ mask,
-                                                          int32_t *__restrict__ delta) {
+                                                          typename DeltaOf<T>::t *__restrict__ delta) {
+    typedef typename DeltaOf<T>::t TD;
     const int32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= n) return;
     const int64_t ld = dia_ld(n);
     const int32_t k = ndiag >> 1;
     const uint32_t m = mask[r];
     for (int d = 0; d < k; ++d) {
-        int32_t e = 0;
+        TD e = 0;
         if ((m >> d) & 1u) {
             const int32_t rm = r + off[d], md = ndiag - 1 - d;
-            e = (int32_t)(__double_as_longlong(dval[(int64_t)d * ld + r]) -
-                          __double_as_longlong(dval[(int64_t)md * ld + rm]));
+            e = (TD)(value_bits(dval[(int64_t)d * ld + r]) - value_bits(dval[(int64_t)md * ld + rm]));
         }
         delta[(int64_t)d * ld + r] = e;
     }
@@ -438,23 +456,33 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
             UHIP_CHECK_HIP(hipFree(A->dia_val));
             A->dia_val = half;
             sym = 1;
-        } else if (w27 && A->dtype == UHIP_F64 && !has_nan) {
+        } else if (w27 && !has_nan) {
             // symmetric up to rounding: the upper half plus each lower value's bit-pattern
-            // difference from its mirror (32-bit), for the 27-point kernel only
+            // difference from its mirror (int32 for FP64, int8 for FP32), for the 27-point kernel only
+            const bool f64 = A->dtype == UHIP_F64;
             UHIP_CHECK_HIP(hipMemsetAsync(flags, 0, sizeof(int32_t), s));
-            hipLaunchKernelGGL(k_dia_deltacheck, g, b, 0, s, n, nd, table, (const double *)A->dia_val,
-                               (const uint32_t *)A->dia_mask, flags);
+            if (f64)
+                hipLaunchKernelGGL(k_dia_deltacheck<double>, g, b, 0, s, n, nd, table, (const double *)A->dia_val,
+                                   (const uint32_t *)A->dia_mask, flags);
+            else
+                hipLaunchKernelGGL(k_dia_deltacheck<float>, g, b, 0, s, n, nd, table, (const float *)A->dia_val,
+                                   (const uint32_t *)A->dia_mask, flags);
             UHIP_CHECK_LAUNCH();
             int32_t far = 1;
             UHIP_CHECK_HIP(hipMemcpyAsync(&far, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
             UHIP_CHECK_HIP(hipStreamSynchronize(s));
             if (far == 0) {
                 const size_t ld = (size_t)dia_ld(n), keep = (size_t)(nd / 2 + 1);
+                const size_t db = f64 ? sizeof(int32_t) : sizeof(int8_t);
                 void *half = nullptr, *delta = nullptr;
                 UHIP_CHECK_HIP(hipMalloc(&half, vb * keep * ld));
-                UHIP_CHECK_HIP(hipMalloc(&delta, sizeof(int32_t) * (size_t)(nd / 2) * ld));
-                hipLaunchKernelGGL(k_dia_deltafill, g, b, 0, s, n, nd, table, (const double *)A->dia_val,
-                                   (const uint32_t *)A->dia_mask, (int32_t *)delta);
+                UHIP_CHECK_HIP(hipMalloc(&delta, db * (size_t)(nd / 2) * ld));
+                if (f64)
+                    hipLaunchKernelGGL(k_dia_deltafill<double>, g, b, 0, s, n, nd, table, (const double *)A->dia_val,
+                                       (const uint32_t *)A->dia_mask, (int32_t *)delta);
+                else
+                    hipLaunchKernelGGL(k_dia_deltafill<float>, g, b, 0, s, n, nd, table, (const float *)A->dia_val,
+                                       (const uint32_t *)A->dia_mask, (int8_t *)delta);
                 UHIP_CHECK_LAUNCH();
                 UHIP_CHECK_HIP(hipMemcpyAsync(half, (const char *)A->dia_val + vb * (size_t)(nd / 2) * ld,
                                               vb * keep * ld, hipMemcpyDeviceToDevice, s));

@@ -445,16 +445,29 @@ constexpr int64_t kDiaCachedBytes = int64_t(128) << 20;
 // CACHED: the values are read through the caches instead of streamed (non-temporal), so that
 // value arrays small enough to stay in the 256 MB last-level cache (MALL) are served from it
 // by the next product on the same level (a V-cycle runs four per level).
-// DELTA (SYM, FP64 values symmetric only up to rounding): lower entry d of row r is its mirror
-// with delta[d][r] added to the bit pattern, exact.
+// DELTA (SYM, values symmetric only up to rounding): lower entry d of row r is its mirror with
+// delta[d][r] added to the bit pattern (int32 for FP64, int8 for FP32), exact.
+template <typename TA>
+struct Dia27Delta {
+    typedef int32_t t;
+    static __device__ __forceinline__ double apply(double v, int32_t d) {
+        return __longlong_as_double(__double_as_longlong(v) + (long long)d);
+    }
+};
+template <>
+struct Dia27Delta<float> {
+    typedef int8_t t;
+    static __device__ __forceinline__ float apply(float v, int8_t d) { return __int_as_float(__float_as_int(v) + (int)d); }
+};
 template <typename TAcc, class Epi, typename TA, typename TX, int P, int LA, bool SYM, bool NANSEL, bool CACHED,
           bool DELTA = false>
 __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S1, int32_t S2,
                                                         const TA *__restrict__ val, const uint32_t *__restrict__ mask,
-                                                        const int32_t *__restrict__ delta, const TX *__restrict__ x,
-                                                        Epi epi, double *__restrict__ partials, int64_t stride) {
+                                                        const typename Dia27Delta<TA>::t *__restrict__ delta,
+                                                        const TX *__restrict__ x, Epi epi,
+                                                        double *__restrict__ partials, int64_t stride) {
     static_assert(!(SYM && NANSEL), "the symmetric layout selects on the mask");
-    static_assert(!DELTA || (SYM && sizeof(TA) == 8), "bit-pattern deltas belong to FP64 symmetric storage");
+    static_assert(!DELTA || SYM, "bit-pattern deltas belong to symmetric storage");
     constexpr int ND = Epi::kDots > 0 ? Epi::kDots : 1;
     constexpr bool TX_IN = EpiTakesX<Epi>::value;
     constexpr int NV = SYM ? 14 : 27;
@@ -485,7 +498,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
         struct Plane {
             TA v[NV];
             TA m1, nm1, np1, my[3], mz[6];  // SYM: mirrors
-            int32_t dl[DELTA ? 13 : 1];      // DELTA: the lower entries' bit-pattern differences
+            typename Dia27Delta<TA>::t dl[DELTA ? 13 : 1];  // DELTA: the lower entries' bit-pattern differences
             uint32_t m;
             typename Epi::In ein;
             TX xn[9];
@@ -575,8 +588,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia27r(int32_t nrows, int32_t S
                                    c.mz[4], c.mz[5], c.my[0], c.my[1], c.my[2], c.m1};
 #pragma unroll
                 for (int d = 0; d < 13; ++d) {
-                    if constexpr (DELTA)
-                        vals[d] = __longlong_as_double(__double_as_longlong(lo[d]) + (long long)c.dl[d]);
+                    if constexpr (DELTA) vals[d] = Dia27Delta<TA>::apply(lo[d], c.dl[d]);
                     else vals[d] = lo[d];
                 }
 #pragma unroll
@@ -821,8 +833,8 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
     if (A && A->format == UHIP_FMT_DIA) {
         UHIP_REQUIRE(A->dia_off && A->dia_val && A->dia_mask && A->ndiag >= 1 && A->ndiag <= 32,
                      "spmv: DIA plan missing");
-        UHIP_REQUIRE(!A->dia_delta || (A->dia_w27 && A->dia_sym == 1 && A->dtype == UHIP_F64),
-                     "spmv: DIA lower-triangle deltas need the FP64 27-point kernel");
+        UHIP_REQUIRE(!A->dia_delta || (A->dia_w27 && A->dia_sym == 1),
+                     "spmv: DIA lower-triangle deltas need the 27-point kernel");
         if (A->nrows == 0) return UHIP_OK;
         UHIP_TRY(partials_for(A->nparts));
         double *parts = parts_in ? parts_in : ctx->partials;
@@ -844,7 +856,7 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
 #define UHIP_DIA27_LAUNCH3(SYM, NANSEL, CACHED, DELTA)                                                            \
     hipLaunchKernelGGL((k_spmv_dia27r<TAcc, Epi, TA, TX, P, LA, SYM, NANSEL, CACHED, DELTA>), g, blk, 0,            \
                        stream_of(ctx), A->nrows, A->dia_s1, A->dia_s2, v, (const uint32_t *)A->dia_mask,          \
-                       (const int32_t *)A->dia_delta, x, epi, parts, pstride)
+                       (const typename Dia27Delta<TA>::t *)A->dia_delta, x, epi, parts, pstride)
 #define UHIP_DIA27_LAUNCH2(SYM, NANSEL, CACHED) UHIP_DIA27_LAUNCH3(SYM, NANSEL, CACHED, false)
 #define UHIP_DIA27_LAUNCH(SYM, NANSEL)                                                                            \
     do {                                                                                                          \
@@ -853,10 +865,8 @@ int launch_rowblock(uhip_ctx ctx, const uhip_csr *A, const TX *x, const Epi &epi
     } while (0)
             const bool cached = (int64_t)(A->dia_sym ? 14 : 27) * dia_ld(A->nrows) * (int64_t)sizeof(TA) <= kDiaCachedBytes;
             if (A->dia_delta) {
-                if constexpr (sizeof(TA) == 8) {
-                    if (cached) UHIP_DIA27_LAUNCH3(true, false, true, true);
-                    else UHIP_DIA27_LAUNCH3(true, false, false, true);
-                }
+                if (cached) UHIP_DIA27_LAUNCH3(true, false, true, true);
+                else UHIP_DIA27_LAUNCH3(true, false, false, true);
             } else if (A->dia_sym) UHIP_DIA27_LAUNCH(true, false);
             else if (A->dia_nan) UHIP_DIA27_LAUNCH(false, true);
             else UHIP_DIA27_LAUNCH(false, false);
