@@ -38,15 +38,17 @@ def spmv_bytes(n, nnz, val_bytes):
     return (val_bytes + 4) * nnz + (4 + 2 * val_bytes) * n + 4
 
 
-def dia_bytes(n, ndiag, val_bytes, sym=0, nan_select=False):
+def dia_bytes(n, ndiag, val_bytes, sym=0, nan_select=False, delta=False):
     # DIA: ndiag value slots per row (absent entries NaN - or zero - in the value arrays;
-    # symmetric storage keeps the ndiag // 2 + 1 offsets >= 0), the per-row mask (1/2/4 B for
-    # <= 8/16/32 offsets; not read by the 7-point kernel when it selects on NaN-encoded
-    # absent entries), x read once, y written; offsets table
+    # symmetric storage keeps the ndiag // 2 + 1 offsets >= 0, near-symmetric storage adds the
+    # lower values' bit-pattern deltas, 4 B in FP64 and 1 B in FP32), the per-row mask (1/2/4 B
+    # for <= 8/16/32 offsets; not read where the kernel selects on NaN-encoded absent entries:
+    # the 7-point and full layouts), x read once, y written; offsets table
     mb = 1 if ndiag <= 8 else (2 if ndiag <= 16 else 4)
     stored = ndiag // 2 + 1 if sym else ndiag
-    mask = 0 if (sym == 2 and nan_select) else mb * n
-    return val_bytes * stored * n + mask + 2 * val_bytes * n + 4 * ndiag
+    mask = 0 if (nan_select and sym != 1) else mb * n
+    dl = (ndiag // 2) * (4 if val_bytes == 8 else 1) * n if delta else 0
+    return val_bytes * stored * n + mask + dl + 2 * val_bytes * n + 4 * ndiag
 
 
 def vi_bytes(D, val_bytes, add=False):
@@ -240,6 +242,35 @@ def solve_kernels(ua, ctx, A, interps, n_rows, reps=20):
     out["interpolate_f32"] = {"ms": ms, "bytes": nb, "gbs": nb / (ms * 1e-3) / 1e9,
                               "label": f"EpiAddTo<float> c += P c_c, format {P.format} (VI = 3), {P.m.ndiag} slots"
                                        f"{vi_sig_note(P)}"}
+    del P, R
+    # level 1: the 128^3 Galerkin operator P^T A P (27 points), in the plan the solve gives it
+    DA = ua.DeviceCsr(ctx, A, np.float64, fmt=ua.FMT_ROWBLOCK)
+    DP = ua.DeviceCsr(ctx, interps[-1], np.float64, fmt=ua.FMT_ROWBLOCK)
+    raw = ua.uhip_csr()
+    ua.check(L.uhip_csr_ptap(ctx.h, DA.ref, DP.ref, ctypes.byref(raw)))
+    C = ua.csr_from_raw(ctx, raw)
+    ua.check(L.uhip_csr_free(ctx.h, ctypes.byref(raw)))
+    del DA, DP
+    for dt, vb, tag, code in ((np.float64, 8, "f64", F64), (np.float32, 4, "f32", F32)):
+        D = ua.DeviceCsr(ctx, C, dt)
+        base = dia_bytes(nc, D.m.ndiag, vb, D.m.dia_sym, D.m.dia_nan == 1, bool(D.m.dia_delta)) \
+            if D.format == ua.FMT_DIA else spmv_bytes(nc, C.nnz, vb)
+        u = [ua.DeviceArray.from_numpy(ctx, ua.gen_uniform(40 + i, nc).astype(dt)) for i in range(3)]
+        u0, u1, u2 = (ctypes.c_void_p(a.ptr) for a in u)
+        st = ChebStep(pprev=u0, p_dtype=code, rprev=None, xin=u1, xout=u2, x_dtype=code, rout=None, pout=None,
+                      a_prev=0.3, beta=0.1, a_cur=0.2, last=1, dot_with=None, dot_d=None)
+        layout = ("near-symmetric, bit-pattern deltas" if D.m.dia_delta else
+                  ("symmetric" if D.m.dia_sym else "full")) if D.format == ua.FMT_DIA else f"format {D.format}"
+        kname = "k_spmv_dia27r" if D.m.dia_w27 else f"format {D.format}"
+        for key, label, launch, nbytes in (
+                ("l1_residual", f"EpiResidual<{'double' if vb == 8 else 'float'}> r_R = r - A c, 128^3 Galerkin",
+                 lambda: L.uhip_spmv_residual(ctx.h, D.ref, u1, u0, code, u2), base + vb * nc),
+                ("l1_cheb", f"EpiCheb<{'double' if vb == 8 else 'float'}> last step, 128^3 Galerkin",
+                 lambda: L.uhip_cheb_step_apply(ctx.h, D.ref, ctypes.byref(st)), base + vb * nc)):
+            ms = time_kernel(ua, ctx, lambda launch=launch: ua.check(launch()), reps)
+            out[f"{key}_{tag}"] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                                   "label": f"{label} ({kname}, {layout}, {D.m.ndiag} offsets)"}
+        del D
     return out
 
 
