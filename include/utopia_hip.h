@@ -367,6 +367,10 @@ int uhip_cheb_step_apply(uhip_ctx ctx, const uhip_csr *A, const uhip_cheb_step *
  * pre-smoothing ends and the post-smoothing starts (Multigrid::standard_cycle,
  * utopia_Multigrid.hpp:280-300, 346-360). */
 int uhip_cheb_pair_supported(const uhip_csr *A);
+/* workgroups of a pair launch on A (one 1024- or 512-thread group per CU at a time; 0 when not
+ * supported): a grid that gives fewer than the GPU's CUs leaves CUs idle, and the multigrid then
+ * runs the two launches instead. No reference analogue. */
+int64_t uhip_cheb_pair_workgroups(const uhip_csr *A);
 int uhip_cheb_pre_residual(uhip_ctx ctx, const uhip_csr *A, const void *b, double a_prev, double beta,
                            double a_cur, void *c, void *rR);
 int uhip_cheb_residual_post(uhip_ctx ctx, const uhip_csr *A, const void *x, const void *b, double a_prev,
@@ -521,9 +525,11 @@ int uhip_solver_destroy(uhip_solver s);
 /* read(Input&)-style parameters: atol rtol stol max_it verbose reset_initial_guess refine
  * pre_smoothing_steps post_smoothing_steps mg_type must_generate_masks fix_semidefinite_operators
  * cycle_type (1 MULTIPLICATIVE_CYCLE, 3 FULL_CYCLE; utopia_MultiLevelBase.hpp:12-14) v_cycle_repetition
- * use_line_search (utopia_Multigrid.hpp:324-341) fused_smoothing (default 1: on plane-walking 7-point
- * levels the pre-smoothing's last step and the residual, and the post-smoothing's residual and step,
- * run as one pass each, uhip_cheb_pre_residual / uhip_cheb_residual_post; 0: one launch per product)
+ * use_line_search (utopia_Multigrid.hpp:324-341) fused_smoothing (1, the default: on plane-walking
+ * 7-point levels whose pair launch fills the GPU, uhip_cheb_pair_workgroups >= 256, the
+ * pre-smoothing's last step and the residual, and the post-smoothing's residual and step, run as
+ * one pass each, uhip_cheb_pre_residual / uhip_cheb_residual_post; 2: on every level that allows
+ * it; 0: one launch per product)
  * smoother(0 Chebyshev3level, 1 Jacobi, 2 GaussSeidel, multicolour, 3 PointJacobi) scale_max_eig scale_min_eig power_method_tol
  * power_method_max_it block_nnz eig_comp_tol use_symmetric_sweep check_s_norm_each coarse_space_it
  * active_set_tol; AlgebraicMultigrid: n_levels bmax weight block_size (2-4: BlockAgglomerate,

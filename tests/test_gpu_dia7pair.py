@@ -75,6 +75,18 @@ def test_residual_post_equals_residual_then_step(ctx, dims, dt, dot):
         assert abs(d2.numpy()[0] - d1.numpy()[0]) <= bound
 
 
+def test_pair_workgroups_follow_the_grid(ctx):
+    """Planes per workgroup shrink from 32 (the 256^3 grid: 256 groups in FP32) to 8 so that a
+    pair launch keeps 256 workgroups where it can: 256 x 256 x 32 (8 planes each: 128 groups in
+    FP32, 256 in FP64) and a small grid (8 planes each: 2 and 4 groups)."""
+    for dims, want32, want64 in (((256, 256, 32), 128, 256), ((64, 32, 16), 2, 4)):
+        A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, *dims, seed=1)
+        assert lib().uhip_cheb_pair_workgroups(ua.DeviceCsr(ctx, A, dtype=np.float32).ref) == want32
+        assert lib().uhip_cheb_pair_workgroups(ua.DeviceCsr(ctx, A).ref) == want64
+    R = ua.gen_stencil(ua.STENCIL_2D5, 64, 64)
+    assert lib().uhip_cheb_pair_workgroups(ua.DeviceCsr(ctx, R).ref) == 0
+
+
 def test_unsupported_operators_are_reported(ctx):
     """Planes of 256 and 1024 rows (no whole FP32 chunk), a 2D operator, overlapping vectors."""
     A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, 16, 16, 13, seed=1)
@@ -101,13 +113,14 @@ def test_unsupported_operators_are_reported(ctx):
 @pytest.mark.parametrize("kind", ["mg", "mg_mixed"])
 def test_multigrid_cycles_fused_equal_unfused(ctx, kind):
     """Multigrid as the solver (no dot in its cycles): fused and unfused smoothing give the same
-    iterates bit for bit; FP64 levels and, through the mixed CG, FP32 levels."""
+    iterates bit for bit; FP64 levels and, through the mixed CG, FP32 levels (fused_smoothing 2:
+    pairs also on a grid too small to give every CU a workgroup)."""
     n = 64
     A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n // 2, seed=5)
     b = ua.gen_uniform(105, A.shape[0])
     interps = ua.mg_hierarchy(n, n, n // 2, 3)
     out = []
-    for fused in (0, 1):
+    for fused in (0, 2):
         ua.srand(1)
         if kind == "mg":
             s = ua.Multigrid(ctx, atol=0, rtol=0, stol=0, max_it=3, pre_smoothing_steps=2, post_smoothing_steps=2,

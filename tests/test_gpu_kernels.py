@@ -372,8 +372,12 @@ def test_dia27_walk_streams_large_levels(ctx, dims, layout):
 
 
 def test_dia27_autotune_keeps_a_valid_plan(ctx):
-    """The autotuner times the 27-point DIA plan with and without the plane walk; whichever it
-    keeps computes the CSR sum bit for bit."""
+    """The autotuner times the 27-point DIA plan with and without the plane walk (and with it the
+    near-symmetric layout); whichever it keeps computes the CSR sum bit for bit. An automatic plan
+    walks only grids with enough waves to fill the GPU (1024): a 16 x 8 x 13 grid has 8."""
+    B = ua.DeviceCsr(ctx, _box27(16, 8, 13))
+    assert B.format == ua.FMT_DIA and B.m.dia_w27 == 0
+    assert ua.DeviceCsr(ctx, _box27(16, 8, 13), fmt=ua.FMT_DIA).m.dia_w27 == 1
     A = _galerkin_dims(32, 32, 40)
     n = A.shape[0]
     fmt = ctypes.c_int32()

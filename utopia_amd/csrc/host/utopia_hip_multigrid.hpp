@@ -298,14 +298,18 @@ public:
     const Factorization *coarse = nullptr;
     int pre = 3, post = 3, mg_type = 1, smoother = MG_CHEBYSHEV;
     bool masks = true, gs_symmetric = true, line_search = false;
-    bool fuse_pairs = true;  // levels whose operator allows it run each smoothing pair in one pass
+    // smoothing pairs in one pass: 0 never, 1 on levels whose operator allows it and whose pair
+    // launch gives every CU a workgroup (kPairFillGroups), 2 wherever the operator allows it
+    int fuse_pairs = 1;
+    static constexpr int64_t kPairFillGroups = 256;  // the CUs of an MI355X
 
     void allocate_work() {
         for (size_t l = 0; l < levels.size(); ++l) {
             auto &L = levels[l];
             const int64_t n = L.A->rows();
             auto ctx = L.A->context();
-            L.pair = fuse_pairs && smoother == MG_CHEBYSHEV && l > 0 && uhip_cheb_pair_supported(&L.A->raw());
+            L.pair = fuse_pairs > 0 && smoother == MG_CHEBYSHEV && l > 0 && uhip_cheb_pair_supported(&L.A->raw()) &&
+                     (fuse_pairs == 2 || uhip_cheb_pair_workgroups(&L.A->raw()) >= kPairFillGroups);
             for (HipVectorT<T> *v : {&L.r, &L.c, &L.rR, &L.rk, &L.pa, &L.pb}) *v = HipVectorT<T>(ctx, n, T(0));
             if (line_search && l > 0) L.cI = HipVectorT<T>(ctx, n, T(0));
             if (smoother == MG_CHEBYSHEV && l > 0) {
@@ -560,15 +564,16 @@ public:
     }
     bool use_line_search() const { return line_search_; }
     // the pairs of products where the pre-smoothing ends and the post-smoothing starts run in one
-    // pass each on operators that allow it (uhip_cheb_pair_supported); off: one launch per product
-    void fused_smoothing(bool v) {
+    // pass each on operators that allow it (uhip_cheb_pair_supported): 1 where the pair launch
+    // fills the GPU, 2 wherever allowed; 0: one launch per product
+    void fused_smoothing(int v) {
         fused_ = v;
         if (eng_.fuse_pairs != v && !eng_.levels.empty()) {
             eng_.fuse_pairs = v;
             eng_.allocate_work();
         }
     }
-    bool fused_smoothing() const { return fused_; }
+    int fused_smoothing() const { return fused_; }
     void must_generate_masks(bool v) { masks_ = v; }
     bool must_generate_masks() const { return masks_; }
     void fix_semidefinite_operators(bool v) { fix_semi_ = v; }
@@ -750,7 +755,8 @@ private:
     std::vector<HipVector> mask_;
     int n_levels_ = 0, pre_ = 3, post_ = 3, mg_type_ = 1, cycle_type_ = MG_MULTIPLICATIVE_CYCLE;
     int64_t v_cycle_repetition_ = 10;
-    bool masks_ = true, fix_semi_ = false, galerkin_ = true, line_search_ = false, fused_ = true;
+    bool masks_ = true, fix_semi_ = false, galerkin_ = true, line_search_ = false;
+    int fused_ = 1;
     VCycle<double> eng_;
     HipVector r_top_, c_top_;
     double setup_s_ = 0;

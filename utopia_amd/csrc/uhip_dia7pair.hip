@@ -26,6 +26,10 @@ namespace uhip {
 
 constexpr int kPairMaxS1 = 256;  // y halo rows on either side (one halo row per thread: 2 S1 <= NT)
 constexpr int kPairPlanes = 32;  // planes per workgroup (measured: 8 and 16 are slower at 256^3)
+// A grid with fewer planes takes fewer per workgroup, down to kPairMinPlanes, so that the pair
+// keeps kPairMinGroups workgroups (one per CU) where it can.
+constexpr int kPairMinGroups = 256;
+constexpr int kPairMinPlanes = 8;
 
 // threads per workgroup and chunk rows per thread: FP32 1024 x 2, FP64 512 x 2 (three planes of a
 // row's values live in registers)
@@ -241,13 +245,24 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
 
 // The pair applies to a plane-walking operator whose planes split into whole chunks of
 // PairShape rows (2048 FP32, 1024 FP64) and whose y halo fits one row per thread.
-static bool pair_ok(const uhip_csr *A) {
-    if (A->format != UHIP_FMT_DIA || A->dia_sym != 2 || A->dia_nan != 1) return false;
+static int pair_planes(const uhip_csr *A) {
+    if (A->format != UHIP_FMT_DIA || A->dia_sym != 2 || A->dia_nan != 1) return 0;
     const int32_t S1 = A->dia_s1, S2 = A->dia_s2;
-    if (S1 < 1 || S1 > kPairMaxS1 || S2 <= 0 || A->nrows % S2 != 0) return false;
+    if (S1 < 1 || S1 > kPairMaxS1 || S2 <= 0 || A->nrows % S2 != 0) return 0;
     const int32_t m = A->dtype == UHIP_F64 ? PairShape<double>::NT * PairShape<double>::CORE
                                            : PairShape<float>::NT * PairShape<float>::CORE;
-    return S2 % m == 0;
+    if (S2 % m != 0) return 0;
+    const int64_t planes = (int64_t)(A->nrows / S2) * (S2 / m);  // chunk-planes to share out
+    return (int)std::max<int64_t>(kPairMinPlanes, std::min<int64_t>(kPairPlanes, planes / kPairMinGroups));
+}
+static bool pair_ok(const uhip_csr *A) { return pair_planes(A) > 0; }
+static int64_t pair_groups(const uhip_csr *A) {
+    const int P = pair_planes(A);
+    if (P == 0) return 0;
+    const int32_t m = A->dtype == UHIP_F64 ? PairShape<double>::NT * PairShape<double>::CORE
+                                           : PairShape<float>::NT * PairShape<float>::CORE;
+    const int64_t nz = A->nrows / A->dia_s2;
+    return (int64_t)(A->dia_s2 / m) * ((nz + P - 1) / P);
 }
 
 template <typename T, int MODE, bool DOT>
@@ -256,7 +271,7 @@ static int pair_launch(uhip_ctx ctx, const uhip_csr *A, const T *x1, const T *b,
     constexpr int NT = PairShape<T>::NT, CORE = PairShape<T>::CORE;
     const int32_t S1 = A->dia_s1, S2 = A->dia_s2, nz = A->nrows / S2;
     const int32_t chunks = S2 / (CORE * NT);
-    const int32_t P = kPairPlanes;
+    const int32_t P = pair_planes(A);
     const int64_t nblk = (int64_t)chunks * ((nz + P - 1) / P);
     if (nblk <= 0) return UHIP_OK;
     if (nblk > INT32_MAX) return set_error(UHIP_ERR_INVALID, "dia7 pair: grid too large");
@@ -285,6 +300,8 @@ using namespace uhip;
 extern "C" {
 
 int uhip_cheb_pair_supported(const uhip_csr *A) { return A && pair_ok(A) ? 1 : 0; }
+
+int64_t uhip_cheb_pair_workgroups(const uhip_csr *A) { return A ? pair_groups(A) : 0; }
 
 int uhip_cheb_pre_residual(uhip_ctx ctx, const uhip_csr *A, const void *b, double a_prev, double beta,
                            double a_cur, void *c, void *rR) {

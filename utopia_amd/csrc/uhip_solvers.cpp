@@ -95,7 +95,7 @@ std::shared_ptr<Multigrid> make_multigrid(uhip_solver_s *s, bool with_transfers 
     mg->cycle_type((int)param(s, "cycle_type", MG_MULTIPLICATIVE_CYCLE));
     mg->v_cycle_repetition((int64_t)param(s, "v_cycle_repetition", 10));
     mg->use_line_search(param(s, "use_line_search", 0) != 0);
-    mg->fused_smoothing(param(s, "fused_smoothing", 1) != 0);
+    mg->fused_smoothing((int)param(s, "fused_smoothing", 1));
     mg->must_generate_masks(param(s, "must_generate_masks", 1) != 0);
     mg->fix_semidefinite_operators(param(s, "fix_semidefinite_operators", 0) != 0);
     if (with_transfers) mg->set_transfer_operators(s->transfers);

@@ -350,7 +350,10 @@ __global__ __launch_bounds__(kBlock) void k_dia_deltafill(int32_t n, int32_t ndi
 // Plan DIA arrays; UHIP_ERR_UNSUPPORTED (nothing allocated) when the matrix is
 // not square, has unsorted or duplicate columns in a row, more than 32 distinct
 // offsets, or ndiag * n > max_ratio * nnz.
-static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
+// walk: the plane-walking 27-point kernel (and with it the near-symmetric layout) 0 never, 1 when
+// the grid gives it enough waves (automatic plans), 2 whenever the offsets allow (an explicit DIA
+// plan; the autotuner times 2 against 0).
+static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio, int walk) {
     hipStream_t s = ctx->stream;
     const int32_t n = A->nrows;
     if (n != A->ncols || n == 0) return UHIP_ERR_UNSUPPORTED;
@@ -413,13 +416,16 @@ static int plan_dia(uhip_ctx ctx, uhip_csr *A, double max_ratio) {
 #undef UHIP_DIA_ZERO
         UHIP_CHECK_LAUNCH();
     }
-    // the 27 offsets dz S2 + dy S1 + dx in this (sorted) order, S2 a multiple of 64: the
-    // plane-walking 27-point kernel
+    // the 27 offsets dz S2 + dy S1 + dx in this (sorted) order, S2 a multiple of 64 (and for an
+    // automatic plan enough waves to fill the GPU, kDia27MinWaves: a small or flat grid keeps the
+    // generic kernels): the plane-walking 27-point kernel
     bool w27 = false;
-    if (nd == 27) {
+    if (nd == 27 && walk > 0) {
         const int32_t S1 = offs[16], S2 = offs[22];
         w27 = S1 > 2 && S2 > 2 * S1 + 2 && S2 % 64 == 0 &&
-              (int64_t)n + (int64_t)(dia27_planes(A->dtype) + 2) * S2 < INT32_MAX;
+              (int64_t)n + (int64_t)(dia27_planes(A->dtype) + 2) * S2 < INT32_MAX &&
+              (walk == 2 ||
+               (int64_t)dia7z_workgroups(n, S2, dia27_planes(A->dtype)) * (kDiaWG / 64) >= kDia27MinWaves);
         for (int i = 0; i < 27 && w27; ++i)
             w27 = offs[(size_t)i] == (i / 9 - 1) * S2 + (i / 3 % 3 - 1) * S1 + (i % 3 - 1);
     }
@@ -945,15 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_one(int64_t n, T *__restrict__ 
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) x[i] = T(1);
 }
 
-// A 27-point DIA plan run by the generic DIA kernel instead of the plane-walking one.
-static bool dia_drop_walk(uhip_csr *A) {
-    if (A->format != UHIP_FMT_DIA || !A->dia_w27) return true;
-    if (A->dia_delta) return false;  // the lower-triangle deltas are read by the 27-point kernel only
-    A->dia_w27 = 0;
-    A->dia_s1 = A->dia_s2 = 0;
-    A->nparts = (A->nrows + dia_rows_per_wg(A->dtype) - 1) / dia_rows_per_wg(A->dtype);
-    return true;
-}
+static int plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz, int walk);
 
 // Autotuner: time y = A x (x = 1) for every applicable layout - DIA (27-point plans with and
 // without the plane-walking kernel), VI, SELL-64, row-block CSR with 1024 / 2048 / 4096
@@ -962,12 +960,11 @@ static bool dia_drop_walk(uhip_csr *A) {
 static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out) {
     struct Cand {
         int32_t format, block_nnz;
-        bool walk;
+        int walk;
     };
-    const Cand cands[] = {{UHIP_FMT_DIA, 2048, true},       {UHIP_FMT_DIA, 2048, false},
-                          {UHIP_FMT_VI, 2048, true},        {UHIP_FMT_SELL64, 2048, true},
-                          {UHIP_FMT_ROWBLOCK, 1024, true},  {UHIP_FMT_ROWBLOCK, 2048, true},
-                          {UHIP_FMT_ROWBLOCK, 4096, true}};
+    const Cand cands[] = {{UHIP_FMT_DIA, 2048, 2},      {UHIP_FMT_DIA, 2048, 0},      {UHIP_FMT_VI, 2048, 1},
+                          {UHIP_FMT_SELL64, 2048, 1},   {UHIP_FMT_ROWBLOCK, 1024, 1}, {UHIP_FMT_ROWBLOCK, 2048, 1},
+                          {UHIP_FMT_ROWBLOCK, 4096, 1}};
     const size_t vb = A->dtype == UHIP_F64 ? sizeof(double) : sizeof(float);
     void *x = nullptr, *y = nullptr;
     UHIP_CHECK_HIP(hipMalloc(&x, vb * (size_t)std::max(A->ncols, 1)));
@@ -979,16 +976,15 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
     UHIP_CHECK_HIP(hipEventCreate(&e0));
     UHIP_CHECK_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    Cand win{UHIP_FMT_ROWBLOCK, 2048, true};
+    Cand win{UHIP_FMT_ROWBLOCK, 2048, 1};
     int st = UHIP_OK;
+    bool walked = false;  // the DIA plan with the walk allowed took the 27-point kernel
     for (const Cand &c : cands) {
-        st = uhip_csr_plan_format(ctx, A, c.format, c.block_nnz);
+        if (c.format == UHIP_FMT_DIA && c.walk == 0 && !walked) continue;  // the same plan as before
+        st = plan_format(ctx, A, c.format, c.block_nnz, c.walk);
         if (st == UHIP_ERR_UNSUPPORTED) continue;  // DIA does not apply
         if (st != UHIP_OK) break;
-        if (!c.walk) {
-            if (!A->dia_w27) continue;        // the same plan as the candidate before
-            if (!dia_drop_walk(A)) continue;  // a layout only the plane walk reads
-        }
+        if (c.format == UHIP_FMT_DIA && c.walk == 2) walked = A->dia_w27 != 0;
         auto run = [&]() {
             return A->dtype == UHIP_F64 ? spmv_typed<double, double>(ctx, A, x, y) : spmv_typed<float, float>(ctx, A, x, y);
         };
@@ -1020,9 +1016,7 @@ static int autotune_plan(uhip_ctx ctx, uhip_csr *A, int reps, float *best_ms_out
     hipFree(y);
     if (st != UHIP_OK && st != UHIP_ERR_UNSUPPORTED) return st;
     if (best_ms_out) *best_ms_out = best;
-    UHIP_TRY(uhip_csr_plan_format(ctx, A, win.format, win.block_nnz));
-    if (!win.walk) dia_drop_walk(A);
-    return UHIP_OK;
+    return plan_format(ctx, A, win.format, win.block_nnz, win.walk);
 }
 
 }  // namespace uhip
@@ -1057,6 +1051,14 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
                  "uhip_csr_plan: format %d", format);
     if (format == UHIP_FMT_AUTO && ctx->autotune_reps > 0 && A->nrows > 0 && A->nnz > 0)
         return autotune_plan(ctx, A, ctx->autotune_reps, nullptr);
+    return plan_format(ctx, A, format, block_nnz, format == UHIP_FMT_DIA ? 2 : 1);
+}
+
+}  // extern "C"
+
+namespace uhip {
+
+static int plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t block_nnz, int walk) {
     if (block_nnz == 0) block_nnz = 2048;
     int capk;
     if (block_nnz == 1024) capk = 5;
@@ -1069,7 +1071,7 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
     A->block_capk = capk;
     A->format = UHIP_FMT_ROWBLOCK;
     if (format == UHIP_FMT_DIA || (format == UHIP_FMT_AUTO && n > 0 && A->nnz > 0)) {
-        const int st = plan_dia(ctx, A, format == UHIP_FMT_DIA ? 1e30 : 1.25);
+        const int st = plan_dia(ctx, A, format == UHIP_FMT_DIA ? 1e30 : 1.25, walk);
         if (st == UHIP_OK) return UHIP_OK;
         if (st != UHIP_ERR_UNSUPPORTED) return st;
         if (format == UHIP_FMT_DIA)
@@ -1127,6 +1129,10 @@ int uhip_csr_plan_format(uhip_ctx ctx, uhip_csr *A, int32_t format, int32_t bloc
     A->nparts = nb;
     return UHIP_OK;
 }
+
+}  // namespace uhip
+
+extern "C" {
 
 int uhip_csr_plan_destroy(uhip_ctx ctx, uhip_csr *A) {
     UHIP_REQUIRE(A, "uhip_csr_plan_destroy: null matrix");

@@ -422,6 +422,9 @@ struct Dia27Planes {
     static constexpr int LA = 1;
 };
 inline int dia27_planes(int dtype) { return dtype == UHIP_F64 ? Dia27Planes<double>::P : Dia27Planes<float>::P; }
+// fewest waves for which the 27-point walk beats the generic kernels: 1024 (one 4-wave
+// workgroup per CU) - the 64^3 level of a 256^3 hierarchy; a 64 x 64 x 8 level has 128
+constexpr int64_t kDia27MinWaves = 1024;
 // value arrays up to this size are read through the caches by the 27-point kernel (half the
 // MI355X's 256 MB MALL: the level's vectors share it). Measured at 256^3: the FP32 128^3 level
 // (117 MB) gains 2.3 ms per MG-PCG solve, the FP64 one (453 MB) loses 5.5 ms.
