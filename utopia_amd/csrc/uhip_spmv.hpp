@@ -427,7 +427,7 @@ inline int dia27_planes(int dtype) { return dtype == UHIP_F64 ? Dia27Planes<doub
 constexpr int64_t kDia27MinWaves = 1024;
 // value arrays up to this size are read through the caches by the 27-point kernel (half the
 // MI355X's 256 MB MALL: the level's vectors share it). Measured at 256^3: the FP32 128^3 level
-// (117 MB) gains 2.3 ms per MG-PCG solve, the FP64 one (453 MB) loses 5.5 ms.
+// (117 MB) gains 2.3 ms per MG-PCG solve; the FP64 one (453 MB in its full layout) lost 5.5 ms.
 constexpr int64_t kDiaCachedBytes = int64_t(128) << 20;
 
 // 27-point DIA (offsets dz S2 + dy S1 + dx for dz, dy, dx in {-1, 0, 1}, S2 a multiple of 64:
