@@ -12,8 +12,14 @@
             every pair's residual, sampled systems against the oracle's FP64 MG-PCG, and the
             adjoint gradient against the transposed-solve gradient and central differences
 
+  config 3's 128^3 Galerkin level (P^T A P of the 256^3 operator, 2.1M rows, 27 offsets) in the
+            plans the solve gives it: y = A x bit for bit against the oracle's CSR sums in FP64
+            (the near-symmetric layout: upper half and lower bit-pattern deltas) and FP32
+
 Error bounds of the rtol-1e-8 batch solves are a-posteriori: for SPD A, |x - x*| <= |r| / lmin(A).
 """
+import ctypes
+
 import numpy as np
 import pytest
 import scipy.sparse.linalg as spla
@@ -233,3 +239,32 @@ def test_config5_full_batch_mixed_with_adjoint(ctx):
     fd = float((loss(As, B1) - loss(As, B2)) / (B1[3, 2] - B2[3, 2]))
     print(f"dL/db[3,2]: gpu {lam[a + 3, 2]:.10e} transposed solve {lam_d[3, 2]:.10e} fd {fd:.10e}", flush=True)
     assert abs(lam_d[3, 2] - fd) <= 1e-6 * abs(fd)
+
+
+def test_config3_galerkin_level_bit_exact(ctx):
+    n = 256
+    A = ua.gen_stencil(ua.STENCIL_3D7_VARCOEF, n, n, n, seed=3)
+    Pm = ua.mg_hierarchy(n, n, n, 2)[0]
+    L = ua.lib()
+    DA = ua.DeviceCsr(ctx, A, fmt=ua.FMT_ROWBLOCK)
+    DP = ua.DeviceCsr(ctx, Pm, fmt=ua.FMT_ROWBLOCK)
+    raw = ua.uhip_csr()
+    ua.check(L.uhip_csr_ptap(ctx.h, DA.ref, DP.ref, ctypes.byref(raw)))
+    C = ua.csr_from_raw(ctx, raw)
+    ua.check(L.uhip_csr_free(ctx.h, ctypes.byref(raw)))
+    del DA, DP
+    m = C.shape[0]
+    assert m == 128 ** 3
+    x = ua.gen_uniform(7, m)
+    D = ua.DeviceCsr(ctx, C)
+    assert D.format == ua.FMT_DIA and D.m.dia_w27 == 1 and D.m.dia_sym == 1 and D.m.dia_delta
+    xd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray(ctx, m)
+    D.spmv(xd, yd)
+    assert np.array_equal(yd.numpy(), orc.spmv(C, x))
+    del D
+    x32 = x.astype(np.float32)
+    D32 = ua.DeviceCsr(ctx, C, dtype=np.float32)
+    assert D32.format == ua.FMT_DIA and D32.m.dia_w27 == 1 and D32.m.dia_sym == 1
+    xd32, yd32 = ua.DeviceArray.from_numpy(ctx, x32), ua.DeviceArray(ctx, m, np.float32)
+    D32.spmv(xd32, yd32)
+    assert np.array_equal(yd32.numpy(), orc.spmv_f32(C, x32))
