@@ -349,22 +349,25 @@ def _random27(nx, ny, nz, seed, layout):
     return ua.Csr.from_scipy(B)
 
 
-@pytest.mark.parametrize("dims,layout", [((96, 96, 96), "full"), ((112, 112, 112), "sym"),
-                                         ((112, 112, 112), "delta")])
-def test_dia27_walk_streams_large_levels(ctx, dims, layout):
+@pytest.mark.parametrize("dims,layout,dt", [((96, 96, 96), "full", np.float64), ((112, 112, 112), "sym", np.float64),
+                                            ((112, 112, 112), "delta", np.float64),
+                                            ((136, 136, 136), "delta32", np.float32)])
+def test_dia27_walk_streams_large_levels(ctx, dims, layout, dt):
     """Value arrays above 128 MB are streamed (non-temporal loads) instead of read through the
     caches: FP64 27-point operators of 0.88 M rows (full layout, 191 MB) and 1.40 M rows
-    (symmetric layout, 157 MB, with and without lower-triangle deltas) give the CSR sums bit
-    for bit, plain and as a residual."""
+    (symmetric layout, 157 MB, with and without lower-triangle deltas), and an FP32 one of
+    2.52 M rows (symmetric with int8 deltas, 141 MB) give the CSR sums bit for bit, plain and as
+    a residual."""
     A = _random27(*dims, seed=3, layout=layout)
     n = A.shape[0]
-    D = ua.DeviceCsr(ctx, A, fmt=ua.FMT_DIA)
+    vb = np.dtype(dt).itemsize
+    D = ua.DeviceCsr(ctx, A, dtype=dt, fmt=ua.FMT_DIA)
     assert D.m.dia_w27 == 1 and D.m.dia_sym == (0 if layout == "full" else 1)
-    assert bool(D.m.dia_delta) == (layout == "delta")
-    assert (27 if layout == "full" else 14) * n * 8 > 128 << 20
-    x, b = ua.gen_uniform(5, n), ua.gen_uniform(6, n)
-    Ax = orc.spmv(A, x)
-    xd, bd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b), ua.DeviceArray(ctx, n)
+    assert bool(D.m.dia_delta) == layout.startswith("delta")
+    assert (27 if layout == "full" else 14) * n * vb > 128 << 20
+    x, b = ua.gen_uniform(5, n).astype(dt), ua.gen_uniform(6, n).astype(dt)
+    Ax = orc.spmv(A, x) if dt is np.float64 else orc.spmv_f32(A, x)
+    xd, bd, yd = ua.DeviceArray.from_numpy(ctx, x), ua.DeviceArray.from_numpy(ctx, b), ua.DeviceArray(ctx, n, dt)
     D.spmv(xd, yd)
     assert np.array_equal(yd.numpy(), Ax)
     D.residual(xd, bd, yd)
