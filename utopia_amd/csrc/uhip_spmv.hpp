@@ -700,7 +700,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_vi(int32_t nrows, int32_t ndiag
         for (int w = 0; w < NW; ++w) c[w] = row_code(rc, w);
         TX xv[8 * NW];
 #pragma unroll
-        for (int d = 0; d < 8 * NW; ++d) xv[d] = x[((m >> d) & 1u) ? rb + so[d] : rb];
+        for (int d = 0; d < 8 * NW; ++d) xv[d] = d < ndiag ? x[((m >> d) & 1u) ? rb + so[d] : rb] : TX(0);
         TAcc acc = TAcc(0);
 #pragma unroll
         for (int d = 0; d < 8 * NW; ++d) {
