@@ -2,8 +2,10 @@
 (uhip_cheb_pre_residual, uhip_cheb_residual_post) against the two separate launches they
 replace, whose results are bit-exact against the oracle (test_gpu_kernels.py): every output bit
 for bit, the post-smoothing dot within its rounding bound. Shapes cover one chunk per plane (2048
-rows FP32, two of 1024 FP64), y halos of 64, 128 and 256 rows, the benchmark's 65,536-row planes,
-and plane counts that leave a partial group of planes (32 planes per workgroup)."""
+rows FP32, two of 1024 FP64), y halos of 64, 128 and 256 rows and of 32 and 96 (not a multiple of
+the 64-lane wave: the halo rows' x neighbours are loaded, not moved across lanes), the benchmark's
+65,536-row planes, and plane counts that leave a partial group of planes (32 planes per
+workgroup)."""
 import ctypes
 
 import numpy as np
@@ -17,7 +19,7 @@ pytestmark = pytest.mark.gpu
 P = lambda a: ctypes.c_void_p(a.ptr)  # noqa: E731
 UNSUPPORTED = -5
 
-SHAPES = [(64, 32, 13), (128, 16, 9), (64, 64, 40), (64, 32, 70), (256, 256, 21)]
+SHAPES = [(64, 32, 13), (128, 16, 9), (64, 64, 40), (64, 32, 70), (256, 256, 21), (96, 64, 9), (32, 64, 11)]
 COEF = dict(a_prev=0.41, beta=0.073, a_cur=0.29)
 
 

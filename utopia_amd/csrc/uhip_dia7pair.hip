@@ -16,7 +16,8 @@
 // step). The first product also runs on the planes zs - 1 and ze next to the workgroup's own
 // planes. A chunk row's seven values, loaded for the first product, are held in registers for
 // its second product two planes later; the -S2 value and x at the planes below and at the row
-// roll in registers as in k_spmv_dia7r. Entries a row does not hold are NaN in the value arrays
+// roll in registers as in k_spmv_dia7r, and x at r -+ 1 comes from the neighbouring lane by DPP
+// where a wave's rows are consecutive (FP64 MG-PCG at 256^3: -3 ms per solve). Entries a row does not hold are NaN in the value arrays
 // (dia_nan), so ring slots that are never written (planes outside the grid, halo rows outside
 // the plane) meet only dropped products.
 #include "uhip_spmv.hpp"
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
     const int zc = blk / chunks, cc = blk - zc * chunks;
     const int zs = zc * P, ze = min(zs + P, nz);
     const int32_t o0 = cc * M, base = o0 - S1;  // ring index 0 <-> plane offset base
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
     const T nan = (T)__builtin_nan("");
     const T *v0 = val, *v1 = val + ld, *vS1 = val + 2 * ld, *vS2 = val + 3 * ld;
     const int32_t xlast = nz * S2 - 1;
@@ -93,8 +94,15 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
     }
     double part = 0.0;
     struct Ld {
-        T a0, m1, a1, aS1, mS1, aS2, xm1, xp1, xmS1, xpS1, xn, b;
+        T a0, m1, a1, aS1, mS1, aS2, xe, xe2, xmS1, xpS1, xn, b;
     };
+    // x at r -+ 1 is the neighbouring lane's x_r (a DPP move; lanes 0 and 63 load theirs) where a
+    // wave's 64 rows are consecutive and all active: always in the chunk slots, and in the halo
+    // slot when S1 is a multiple of 64 (the left and right halo rows then never share a wave,
+    // and a wave's halo rows are all inside the plane or all outside). Otherwise the halo slot
+    // loads both neighbours (xe, xe2).
+    const bool hdpp = (S1 & 63) == 0;
+    auto dpp = [hdpp](int j) { return j < CORE || hdpp; };
     // the first product's loads of plane f
     auto issue = [&](int f, Ld(&L)[NS]) {
 #pragma unroll
@@ -110,8 +118,10 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
             const T ms = vS1[max(r - S1, 0)];
             L[j].mS1 = r >= S1 ? ms : nan;
             L[j].aS2 = ld_stream(vS2 + r);
-            L[j].xm1 = x1[cl(r - 1)];
-            L[j].xp1 = x1[cl(r + 1)];
+            L[j].xe = T(0);
+            L[j].xe2 = T(0);
+            if (!dpp(j) || lane == 0 || lane == 63) L[j].xe = x1[cl(dpp(j) && lane == 63 ? r + 1 : r - 1)];
+            if (!dpp(j)) L[j].xe2 = x1[cl(r + 1)];
             L[j].xmS1 = x1[cl(r - S1)];
             L[j].xpS1 = x1[cl(r + S1)];
             L[j].xn = x1[cl(r + S2)];
@@ -124,7 +134,10 @@ __global__ __launch_bounds__(NT) void k_dia7_pair(int32_t S1, int32_t S2, int32_
         for (int j = 0; j < NS; ++j) {
             if (!on[j]) continue;
             const T vals[7] = {prev_a2[j], L[j].mS1, L[j].m1, L[j].a0, L[j].a1, L[j].aS1, L[j].aS2};
-            const T xs[7] = {xprev[j], L[j].xmS1, L[j].xm1, xcur[j], L[j].xp1, L[j].xpS1, L[j].xn};
+            const T fb = lane_from_below(xcur[j]), fa = lane_from_above(xcur[j]);
+            const T xm1 = dpp(j) && lane != 0 ? fb : L[j].xe;
+            const T xp1 = dpp(j) ? (lane == 63 ? L[j].xe : fa) : L[j].xe2;
+            const T xs[7] = {xprev[j], L[j].xmS1, xm1, xcur[j], xp1, L[j].xpS1, L[j].xn};
             T acc = T(0);
 #pragma unroll
             for (int d = 0; d < 7; ++d) {

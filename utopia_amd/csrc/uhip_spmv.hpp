@@ -278,6 +278,25 @@ inline int32_t dia7z_workgroups(int32_t nrows, int32_t S2, int planes) {
     return (int32_t)((waves + kDiaWG / 64 - 1) / (kDiaWG / 64));
 }
 
+// A value from the neighbouring lane of the wavefront, by DPP: lane i gets lane i - 1's value
+// (wave_shr:1) or lane i + 1's (wave_shl:1); lane 0, resp. 63, keeps its own.
+__device__ __forceinline__ double lane_from_below(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_above(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_from_below(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_from_above(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
 // Epilogues that read the product's own input at row r (the x of A x) take it from the
 // kernel's registers in the rolling 7-point kernel: they define load_nox (the other inputs)
 // and apply_x (the row sum, those inputs and x_r).
@@ -313,6 +332,8 @@ struct PairOf<float> {
 //    2-vector load (no cross-lane exchange, no extra load at the segment edge);
 //  - the -S1 entry is the +S1 value of row r - S1, read through the cache (the wave S1/64
 //    segments back read it);
+//  - x at r -+ 1 is the neighbouring lane's x_r, moved across lanes by DPP (lanes 0 and 63 load
+//    theirs): one load per row fewer (FP64 y = A x 147.6 -> 140.1 us at 256^3);
 //  - entries the row does not hold are NaN in the value arrays of a NaN-free matrix (NANSEL, the
 //    plan's dia_nan) and are dropped by a select on the value, or dropped by the row's mask bits.
 // Every row sum adds the row's CSR entries in column order: bit-identical to the CSR sum.
@@ -344,7 +365,7 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7r(int32_t nrows, const int3
             TA a0, a1, m1, aS1, aS2, mS1;
             uint32_t m;
             typename Epi::In ein;
-            TX xm1, xp1, xmS1, xpS1, xn;
+            TX xe, xmS1, xpS1, xn;
         };
         // x and the mirrors are read at clamped rows whether or not the entry exists (the select
         // drops it), so no load waits for another; mirrors before row 0 are NaN
@@ -365,8 +386,9 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7r(int32_t nrows, const int3
             q.aS2 = ld_stream(vS2 + rc);
             if constexpr (TX_IN) q.ein = epi.load_nox(min(r, xlast));
             else q.ein = epi.load(min(r, xlast));
-            q.xm1 = x[cl(r - 1)];
-            q.xp1 = x[cl(r + 1)];
+            // x at r -+ 1 is the neighbouring lane's x_r (a DPP move); lanes 0 and 63 load theirs
+            q.xe = TX(0);
+            if (lane == 0 || lane == 63) q.xe = x[cl(lane == 0 ? r - 1 : r + 1)];
             q.xmS1 = x[cl(r - S1)];
             q.xpS1 = x[cl(r + S1)];
             q.xn = x[cl(r + S2)];
@@ -385,7 +407,8 @@ __global__ __launch_bounds__(kDiaWG) void k_spmv_dia7r(int32_t nrows, const int3
             if (z + LA < P) q[LA] = load(z + LA);
             const Plane &c = q[0];
             const TA vals[7] = {prev_aS2, c.mS1, c.m1, c.a0, c.a1, c.aS1, c.aS2};
-            const TX xs[7] = {prev_xc, c.xmS1, c.xm1, cur_xc, c.xp1, c.xpS1, c.xn};
+            const TX fb = lane_from_below(cur_xc), fa = lane_from_above(cur_xc);
+            const TX xs[7] = {prev_xc, c.xmS1, lane == 0 ? c.xe : fb, cur_xc, lane == 63 ? c.xe : fa, c.xpS1, c.xn};
             TAcc acc = TAcc(0);
 #pragma unroll
             for (int d = 0; d < 7; ++d) {
